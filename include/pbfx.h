@@ -1,0 +1,222 @@
+/*
+ * pbfx.h -- C-ABI of the MI355X-native Position Based Fluids solver step.
+ *
+ * The drop-in boundary for the reference's PBF step (yodaplusplus/ssf_pbf_cuda,
+ * paths below relative to cuda_pbf_with_ssf/src/).  Plain pointers and sizes
+ * only; device pointers are HIP device memory (or any memory the device can
+ * read and write); `stream` is a hipStream_t passed as void* (NULL = default
+ * stream).  A context is re-entrant: all parameters travel as kernel
+ * arguments, nothing lives in global device state, and each context owns its
+ * scratch, allocated at create time -- no allocation happens inside a step.
+ *
+ * Semantics follow pbf::one_step (pbf/solver/pbf_solve.cpp:147-216):
+ *  predict -> hashed-grid neighbour search (stable counting sort, frozen
+ *  capped neighbour lists) -> I Jacobi iterations (lambda, delta-p with
+ *  tensile correction, boundary response, apply) -> velocity update
+ *  (+ vorticity confinement and XSPH when enabled).  Positions and velocities
+ *  are written back in the new cell-sorted order, and particles whose
+ *  predicted position leaves the simulation area are deleted.
+ */
+#ifndef PBFX_H
+#define PBFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBFX_ABI_VERSION 1
+#define PBFX_EMPTY 0xFFFFFFFFu
+
+typedef enum pbfx_status {
+    PBFX_OK = 0,
+    PBFX_INVALID_ARG = 1,      /* null pointer, non-positive parameter, bad iteration count */
+    PBFX_CAPACITY = 2,         /* num exceeds the context capacity (pbf_type.cpp:42-44 exits) */
+    PBFX_GRID_TOO_LARGE = 3,   /* a grid dimension exceeds 1028 (pbf_type.cpp:155-158 exits) */
+    PBFX_HIP_ERROR = 4,        /* a HIP runtime call failed; see pbfx_last_hip_error */
+    PBFX_NOT_CONVERGED = 5,    /* information: a collision sweep loop hit its cap */
+    PBFX_NEIGHBOR_TRUNCATED = 6,/* information: some particle had more in-range
+                                  candidates than neighbour slots (the reference
+                                  truncates silently, pbf_neighbor_search_util.cu:367) */
+    PBFX_TILE_OVERFLOW = 7,    /* a work unit's 27-cell neighbourhood held more than
+                                  65535 particles; the step result is invalid */
+    PBFX_NOT_SUPPORTED = 8
+} pbfx_status;
+
+enum { PBFX_SCORR_REFERENCE = 0, PBFX_SCORR_PAPER = 1 };
+
+/* Superset of pbf_parameter (pbf/pbf_type.h:49-63) and pbf_external
+ * (pbf/pbf_type.h:69-72). */
+typedef struct pbfx_params {
+    float stable_density;      /* rho0 [kg/m^3] */
+    float stable_distance;     /* lattice spacing [m] (informational) */
+    float particle_mass;       /* m [kg] */
+    float smoothing_length;    /* h [m]; also the hash cell width */
+    float time_step;           /* dt [s] */
+    float relaxation;          /* epsilon in the lambda denominator [m^-2] */
+    float xsph_parameter;      /* XSPH c */
+    float vc_parameter;        /* vorticity confinement epsilon */
+    float body_force[3];       /* [m/s^2] */
+    int32_t scorr_mode;        /* PBFX_SCORR_REFERENCE: 10*(max(-s_i,0)+max(-s_j,0))*(W/W(0.4h))^4
+                                  (pbf/solver/cuda/pbf_position_update.cu:131-151);
+                                  PBFX_SCORR_PAPER: scorr_k*(W/W(scorr_dq_over_h*h))^scorr_n */
+    float scorr_k;
+    int32_t scorr_n;           /* >= 1 */
+    float scorr_dq_over_h;
+    int32_t max_pair;          /* neighbour slots per particle, 2..64; the reference uses 30
+                                  and keeps at most max_pair-1 neighbours */
+    int32_t xsph_on;           /* 0 reproduces the shipped update (pbf_solve.cpp:125-145) */
+    int32_t vort_on;
+    int32_t vort_h_as_inv_h;   /* 1 reproduces the reference's h-for-inv_h argument in the
+                                  vorticity kernels (pbf_vorticity_confinement.cu:28,134) */
+    int32_t outer_as_inner;    /* 1 reproduces pbf_response_collision.cu:46-51 (outer spheres
+                                  answered as inner spheres) */
+    int32_t max_sweeps;        /* collision sweeps per Jacobi iteration (reference: unbounded) */
+} pbfx_params;
+
+/* Fills `p` with the reference's defaults for the drop-in path: REFERENCE
+ * s_corr, 30 neighbour slots, update as shipped (no XSPH, no vorticity),
+ * 32 collision sweeps; physical constants zero (the caller must set them). */
+void pbfx_params_default(pbfx_params* p);
+
+typedef struct pbfx_ctx pbfx_ctx;
+
+/* Replaces pbf_particle::allocate(num, domain_end) + neighbor_search::
+ * neighbor_search (pbf/pbf_type.cpp:165-175, pbf/interaction/pbf_neighbor_search.cpp:11-69)
+ * and pbf_buffer::allocate (pbf/pbf_type.cpp:80-101): grid = calcGridSize(2h, domain_end)
+ * (pbf/pbf_type.cpp:141-162), cell width h, scratch for `capacity` particles.
+ * Allocates on the current HIP device. */
+pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const float domain_end[3],
+                        pbfx_ctx** out);
+
+/* Replaces pbf_particle::free / pbf_buffer::free / ~neighbor_search. */
+void pbfx_destroy(pbfx_ctx* ctx);
+
+/* Updates the step parameters (the reference reads pbf_particle::parameter
+ * at every one_step).  smoothing_length must keep the grid of pbfx_create. */
+pbfx_status pbfx_set_params(pbfx_ctx* ctx, const pbfx_params* params);
+
+/* Replaces pbf_boundary::allocate (pbf/pbf_boundary.cpp:4-25): inner and
+ * outer spheres as float4 (center, radius), planes as point + normal float3.
+ * Host pointers; copied to device memory owned by the context.  Counts may
+ * be zero (no boundary). */
+pbfx_status pbfx_set_boundary(pbfx_ctx* ctx, const float* inner_spheres4, uint32_t n_inner,
+                              const float* outer_spheres4, uint32_t n_outer, const float* plane_points3,
+                              const float* plane_normals3, uint32_t n_planes);
+
+/* Replaces pbf::one_step(pbf_particle&, pbf_buffer&, const pbf_boundary&,
+ * simulation_area, num_solver_iteration) (pbf/solver/pbf_solve.cpp:182-216;
+ * pbf/solver/pbf_solve.h:64-70).
+ *  x, v     device AoS float3 [num]; overwritten in the new sorted order.
+ *  num_inout  live particles before; after the call, live particles after
+ *             deletion (this call synchronises `stream` once, as the
+ *             reference's D2H copy at pbf_neighbor_search.cpp:156 does).
+ *  lo, hi   simulation area; particles whose predicted position is outside
+ *           [lo, hi] (inclusive bounds) are deleted.
+ *  iters    Jacobi iterations (>= 1).
+ * Returns PBFX_OK, an error, or an information status
+ * (PBFX_NEIGHBOR_TRUNCATED / PBFX_NOT_CONVERGED) when the corresponding
+ * event occurred during the step and the context was created with
+ * report_info enabled (pbfx_set_report_info). */
+pbfx_status pbfx_step(pbfx_ctx* ctx, float* x, float* v, uint32_t* num_inout, const float lo[3],
+                      const float hi[3], int iters, void* stream);
+
+/* Same step without any host synchronisation: `num` is an upper bound of
+ * the live count (normally the previous step's result), the new live count
+ * stays on the device (pbfx_get_num reads it).  Use this form for streams of
+ * steps and for graph capture. */
+pbfx_status pbfx_step_async(pbfx_ctx* ctx, float* x, float* v, uint32_t num, const float lo[3],
+                            const float hi[3], int iters, void* stream);
+
+/* Live particle count after the last step on `stream` (synchronises it). */
+pbfx_status pbfx_get_num(pbfx_ctx* ctx, void* stream, uint32_t* num);
+
+/* Replaces the no-boundary pbf::one_step (pbf/solver/pbf_solve.cpp:147-180),
+ * whose collision is the hard-coded inner sphere centre (3,3,3), radius 3
+ * (pbf/solver/cuda/pbf_constraint.cu:262-315). */
+pbfx_status pbfx_step_no_boundary(pbfx_ctx* ctx, float* x, float* v, uint32_t* num_inout,
+                                  const float lo[3], const float hi[3], int iters, void* stream);
+
+/* Views of the last neighbour search, for bit-exact checks.  Replaces the
+ * neighbor_search getters (pbf/interaction/pbf_neighbor_search.h:49-56).
+ * Device pointers owned by the context, valid until the next step. */
+typedef struct pbfx_ns_view {
+    uint32_t grid[3];
+    uint32_t num_cells;          /* C = grid[0]*grid[1]*grid[2] */
+    float cell_width;
+    uint32_t max_pair;
+    const uint32_t* sorted_index;/* [num_in] slot -> index before the sort */
+    const uint32_t* cell_offset; /* [C+2] exclusive scan of the cell counts; cell c owns
+                                    slots [off[c], off[c+1]); off[C] = live count */
+    const uint32_t* hash;        /* [num_in] key per index before the sort (C = deleted) */
+} pbfx_ns_view;
+pbfx_status pbfx_ns_view_get(pbfx_ctx* ctx, pbfx_ns_view* out);
+
+/* Writes the last step's neighbour lists as sorted slot indices into
+ * `out` (device, [num_live * max_pair], row per particle, PBFX_EMPTY
+ * padded) and counts into `counts` (device, [num_live]); the reference
+ * stores the same sets interleaved (pbf_neighbor_search_util.cu:471). */
+pbfx_status pbfx_ns_neighbors(pbfx_ctx* ctx, uint32_t* out, uint32_t* counts, void* stream);
+
+/* Writes cell_start / cell_end in the reference's layout
+ * (pbf_neighbor_search_util.cu:39-81: empty cells get PBFX_EMPTY in start;
+ * here end is 0 for empty cells).  Device buffers of C entries. */
+pbfx_status pbfx_ns_cell_ranges(pbfx_ctx* ctx, uint32_t* cell_start, uint32_t* cell_end, void* stream);
+
+/* Writes the scaling factor s (= -lambda of the paper) of the last Jacobi
+ * iteration, in sorted order, into `out` (device float [num_live]). */
+pbfx_status pbfx_scaling_factor(pbfx_ctx* ctx, float* out, void* stream);
+
+typedef struct pbfx_stats {
+    uint32_t num_in;           /* particles entering the last step */
+    uint32_t num_live;         /* particles after deletion */
+    uint32_t truncated;        /* particles that lost in-range neighbours to the slot cap */
+    uint32_t sweep_cap_hits;   /* particle-iterations whose collision loop hit max_sweeps */
+    uint32_t max_sweeps_used;
+    uint32_t xsph_nan;         /* particles whose XSPH sum was NaN (velocity kept) */
+    uint32_t units;            /* work units (cell blocks) of the last step */
+    uint32_t slow_units;       /* units whose neighbourhood exceeded the on-chip tile */
+    uint32_t slow_particles;   /* particles whose final cell left their unit's rows */
+    uint32_t tile_overflow;    /* units beyond 65535 neighbourhood particles (error) */
+    uint32_t max_tile;         /* largest neighbourhood tile seen */
+    uint32_t steps;            /* steps run by this context */
+} pbfx_stats;
+
+/* Statistics of the last step (synchronises `stream`). */
+pbfx_status pbfx_get_stats(pbfx_ctx* ctx, void* stream, pbfx_stats* out);
+
+/* Tuning knobs.  tile_cap: particles held on chip per work unit (0 forces
+ * the global-memory path everywhere, for testing; default 2048, max 4096);
+ * unit_width: max cells per work unit along x (power of two 1..16, default 16). */
+pbfx_status pbfx_set_tuning(pbfx_ctx* ctx, uint32_t tile_cap, uint32_t unit_width);
+
+/* When non-zero, pbfx_step returns PBFX_NEIGHBOR_TRUNCATED / PBFX_NOT_CONVERGED
+ * after a step in which those events happened (default 0: they are only
+ * counted in pbfx_stats, matching the reference's silent behaviour). */
+void pbfx_set_report_info(pbfx_ctx* ctx, int enable);
+
+/* Grid the context uses (calcGridSize(2h, domain_end)). */
+pbfx_status pbfx_get_grid(pbfx_ctx* ctx, uint32_t grid[3]);
+
+/* pbf/pbf_type.cpp:141-162 as a free function: 0 on success,
+ * PBFX_GRID_TOO_LARGE past 1028 cells per axis. */
+pbfx_status pbfx_calc_grid_size(double cell_size, const float domain_end[3], uint32_t grid[3]);
+
+/* Last HIP error code recorded by the context (0 if none). */
+int pbfx_last_hip_error(pbfx_ctx* ctx);
+
+const char* pbfx_status_string(pbfx_status s);
+
+/* Arithmetic self-test on the device: for n inputs computes
+ * out[4*i+0] = a/b, out[4*i+1] = sqrtf(a), out[4*i+2] = fmaf(a,b,a),
+ * out[4*i+3] = floorf(a/b), bitwise comparable with IEEE host results.
+ * Device pointers. */
+pbfx_status pbfx_selftest_arith(const float* a, const float* b, float* out, uint32_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBFX_H */

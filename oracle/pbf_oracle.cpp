@@ -1,0 +1,855 @@
+// pbf_oracle.cpp -- CPU restatement of the reference PBF solver step.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for libpbfx.so, never part of it.
+// Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+// may load this library.
+//
+// Every function cites the reference file:line it restates; paths are
+// relative to cuda_pbf_with_ssf/src/ of yodaplusplus/ssf_pbf_cuda.
+// Arithmetic is IEEE (built with -ffp-contract=off, no fast math); an
+// explicit fma appears exactly where the reference calls fmaf, and GLM
+// expressions keep GLM 0.9.7.1's evaluation order
+// (external/include/glm/detail/func_geometric.inl:65-72,95-100,134-142):
+//   dot(a,b)   = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   length(a)  = sqrt(dot(a,a))
+//   cross(x,y) = (x.y*y.z - y.y*x.z, x.z*y.x - y.z*x.x, x.x*y.y - y.x*x.y)
+//
+// Pinning (tests/test_oracle_*.py): closed forms of the SPH kernels, the
+// known-answer values computed from the reference's own header maths,
+// brute-force neighbour sets, numpy stable argsort, finite differences of
+// the spiky kernel and of the constraint, and conservation invariants.
+
+#include "pbf_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+template <class R>
+struct V3 {
+    R x, y, z;
+};
+
+template <class R>
+inline V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R>
+inline V3<R> operator-(V3<R> a, V3<R> b) { return v3<R>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class R>
+inline V3<R> operator+(V3<R> a, V3<R> b) { return v3<R>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class R>
+inline V3<R>& operator+=(V3<R>& a, V3<R> b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+// glm: vec * scalar and scalar * vec are componentwise products
+template <class R>
+inline V3<R> mulvs(V3<R> a, R s) { return v3<R>(a.x * s, a.y * s, a.z * s); }
+template <class R>
+inline V3<R> mulsv(R s, V3<R> a) { return v3<R>(s * a.x, s * a.y, s * a.z); }
+template <class R>
+inline V3<R> divvs(V3<R> a, R s) { return v3<R>(a.x / s, a.y / s, a.z / s); }
+template <class R>
+inline R glm_dot(V3<R> a, V3<R> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class R>
+inline R glm_length(V3<R> a) { return std::sqrt(glm_dot(a, a)); }
+template <class R>
+inline V3<R> glm_cross(V3<R> x, V3<R> y) {
+    return v3<R>(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y);
+}
+// fmaf(a.z,b.z, fmaf(a.y,b.y, a.x*b.x)): the hand-fused dot used throughout the solver kernels
+template <class R>
+inline R fdot(V3<R> a, V3<R> b) { return std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x)); }
+
+template <class R>
+struct KC;
+template <>
+struct KC<float> {
+    static constexpr float poly6 = 1.56668147065f;   // 315/(64 pi), pbf_kernel.inl:40
+    static constexpr float spiky = -14.3239448745f;  // -45/pi,    pbf_kernel.inl:58
+    static constexpr float ten = 10.f;
+    static constexpr float nudge_in = 1.0001f;
+    static constexpr float nudge_out = 0.9999f;
+};
+template <>
+struct KC<double> {
+    static constexpr double poly6 = 1.56668147065;
+    static constexpr double spiky = -14.3239448745;
+    static constexpr double ten = 10.0;
+    static constexpr double nudge_in = 1.0001;
+    static constexpr double nudge_out = 0.9999;
+};
+
+// pbf/kernel/cuda/detail/pbf_kernel.inl:34-48 (Wpoly6)
+template <class R>
+inline R wpoly6(R r, R inv_h) {
+    R q = r * inv_h;
+    R alpha = KC<R>::poly6 * inv_h * inv_h * inv_h;
+    if (q <= R(1)) {
+        R q2 = q * q;
+        R t = R(1) - q2;
+        return alpha * t * t * t;
+    }
+    return R(0);
+}
+
+// pbf/kernel/cuda/detail/pbf_kernel.inl:50-63 (WspikyDerivative)
+template <class R>
+inline R wspiky_deriv(R r, R inv_h) {
+    R q = r * inv_h;
+    R inv_h2 = inv_h * inv_h;
+    if (q <= R(1)) {
+        R t = R(1) - q;
+        R inv_h4 = inv_h2 * inv_h2;
+        return KC<R>::spiky * inv_h4 * t * t;
+    }
+    return R(0);
+}
+
+// float -> int with a defined result everywhere: NaN -> 0, saturating (the
+// device conversion's behaviour; a plain cast is undefined out of range).
+inline int f2i(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f <= -2147483648.0f) return INT32_MIN;
+    return static_cast<int>(f);
+}
+
+// pbf/interaction/cuda/pbf_grid.h:20-28: cell = (int)floor(p / cell_width), IEEE division
+inline void grid_pos(float px, float py, float pz, float cw, int g[3]) {
+    g[0] = f2i(std::floor(px / cw));
+    g[1] = f2i(std::floor(py / cw));
+    g[2] = f2i(std::floor(pz / cw));
+}
+
+// pbf/interaction/cuda/pbf_grid.h:54-61: wrap each axis (power-of-two grid), z-major key
+inline uint32_t grid_hash(int gx, int gy, int gz, const uint32_t G[3]) {
+    uint32_t x = static_cast<uint32_t>(gx) & (G[0] - 1u);
+    uint32_t y = static_cast<uint32_t>(gy) & (G[1] - 1u);
+    uint32_t z = static_cast<uint32_t>(gz) & (G[2] - 1u);
+    return z * G[0] * G[1] + y * G[0] + x;
+}
+
+// Integer power by binary exponentiation, the definition shared with the
+// HIP solver for the PAPER s_corr (t^n): n=4 evaluates (t*t)*(t*t).
+template <class R>
+inline R powi(R t, int n) {
+    R result = R(1);
+    R base = t;
+    bool first = true;
+    while (n > 0) {
+        if (n & 1) {
+            result = first ? base : result * base;
+            first = false;
+        }
+        n >>= 1;
+        if (n) base = base * base;
+    }
+    return result;
+}
+
+// Discrete outcome of the neighbour search, computed once in FP32 and shared
+// by the FP32 step and its FP64 twin.
+struct NSData {
+    uint32_t n_in = 0, n_live = 0, C = 0;
+    uint32_t G[3] = {1, 1, 1};
+    float cw = 0.f;
+    std::vector<uint32_t> key;       // per original particle
+    std::vector<uint32_t> idx;       // slot -> original
+    std::vector<uint32_t> off;       // C+2 exclusive offsets
+    std::vector<uint32_t> nb;        // n_live * max_pair
+    std::vector<uint32_t> nbc;       // n_live
+    std::vector<float> pred;         // sorted predicted positions (float), AoS
+    uint32_t truncated = 0;
+    int max_pair = 30;
+};
+
+// pbf/solver/cuda/pbf_predict.cu:15-45 via pbf_solve.cpp:16-25:
+//   v* = fmaf(g, dt, v);  x* = fmaf(v*, dt, x)
+template <class R>
+inline void predict_one(const orc_params* p, const float* x, const float* v, uint32_t i, R out[3]) {
+    R dt = R(p->time_step);
+    for (int c = 0; c < 3; ++c) {
+        R vs = std::fma(R(p->body_force[c]), dt, R(v[3 * i + c]));
+        out[c] = std::fma(vs, dt, R(x[3 * i + c]));
+    }
+}
+
+// pbf/interaction/pbf_neighbor_search.cpp:92-170 (counting-sort branch) with
+// the stable order of its USE_CUB/USE_THRUST branches (:112-131):
+//   hash (pbf_neighbor_search_util.cu:20-37) -> deletion flag
+//   (pbf/util/cuda/pbf_delete.cu:9-28) -> counting sort
+//   (pbf/util/cuda/pbf_counting_sort.cu:167-178) -> deleted count
+//   (pbf_delete.cu:58-77) -> cell ranges (pbf_neighbor_search_util.cu:39-81)
+//   -> reorder (:143-154) -> neighbour list (:361-482).
+void neighbor_search(const orc_params* p, const float* x, const float* v, uint32_t n,
+                     const float lo[3], const float hi[3], NSData& ns) {
+    ns.n_in = n;
+    ns.G[0] = p->grid[0];
+    ns.G[1] = p->grid[1];
+    ns.G[2] = p->grid[2];
+    ns.C = ns.G[0] * ns.G[1] * ns.G[2];
+    ns.cw = p->smoothing_length;
+    ns.max_pair = p->max_pair;
+    const uint32_t C = ns.C;
+
+    std::vector<float> xs(3 * (size_t)n);
+    ns.key.assign(n, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        uint32_t i = (uint32_t)ii;
+        float q[3];
+        predict_one<float>(p, x, v, i, q);
+        xs[3 * i + 0] = q[0];
+        xs[3 * i + 1] = q[1];
+        xs[3 * i + 2] = q[2];
+        int g[3];
+        grid_pos(q[0], q[1], q[2], ns.cw, g);
+        uint32_t k = grid_hash(g[0], g[1], g[2], ns.G);
+        if (q[0] < lo[0] || hi[0] < q[0] || q[1] < lo[1] || hi[1] < q[1] || q[2] < lo[2] || hi[2] < q[2])
+            k = C;  // sentinel: sorts after every cell
+        ns.key[i] = k;
+    }
+
+    // Stable counting sort: a sequential scatter in index order is stable.
+    ns.off.assign((size_t)C + 2, 0);
+    for (uint32_t i = 0; i < n; ++i) ns.off[ns.key[i] + 1]++;
+    for (size_t c = 1; c < ns.off.size(); ++c) ns.off[c] += ns.off[c - 1];
+    // off[c] = first slot of cell c; off[C] = live count; off[C+1] = n
+    {
+        std::vector<uint32_t> cur(ns.off.begin(), ns.off.end() - 1);
+        ns.idx.assign(n, 0);
+        for (uint32_t i = 0; i < n; ++i) ns.idx[cur[ns.key[i]]++] = i;
+    }
+    ns.n_live = ns.off[C];
+    const uint32_t nl = ns.n_live;
+
+    ns.pred.assign(3 * (size_t)n, 0.f);
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < (int64_t)n; ++s) {
+        uint32_t i = ns.idx[s];
+        ns.pred[3 * s + 0] = xs[3 * i + 0];
+        ns.pred[3 * s + 1] = xs[3 * i + 1];
+        ns.pred[3 * s + 2] = xs[3 * i + 2];
+    }
+
+    // detectNeighborsCUDA: 27 cells z,y,x in -1..1; ascending slots; keep
+    // r2 < h*h while count < max_pair-1 (pbf_neighbor_search_util.cu:386-408,428-452)
+    const int M = ns.max_pair;
+    const int cap = M - 1;
+    const float h = p->smoothing_length;
+    const float h2 = h * h;
+    ns.nb.assign((size_t)nl * M, ORC_EMPTY);
+    ns.nbc.assign(nl, 0);
+    uint32_t truncated = 0;
+#pragma omp parallel for schedule(static) reduction(+ : truncated)
+    for (int64_t ss = 0; ss < (int64_t)nl; ++ss) {
+        uint32_t s = (uint32_t)ss;
+        V3<float> self = v3<float>(ns.pred[3 * s], ns.pred[3 * s + 1], ns.pred[3 * s + 2]);
+        int g[3];
+        grid_pos(self.x, self.y, self.z, ns.cw, g);
+        int cnt = 0;
+        bool trunc = false;
+        uint32_t* out = &ns.nb[(size_t)s * M];
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    uint32_t c = grid_hash(g[0] + dx, g[1] + dy, g[2] + dz, ns.G);
+                    for (uint32_t j = ns.off[c]; j < ns.off[c + 1]; ++j) {
+                        V3<float> pj = v3<float>(ns.pred[3 * j], ns.pred[3 * j + 1], ns.pred[3 * j + 2]);
+                        V3<float> d = self - pj;
+                        float r2 = std::fma(d.z, d.z, std::fma(d.y, d.y, d.x * d.x));
+                        if (r2 < h2) {
+                            if (cnt < cap)
+                                out[cnt++] = j;
+                            else
+                                trunc = true;
+                        }
+                    }
+                }
+        ns.nbc[s] = (uint32_t)cnt;
+        if (trunc) truncated++;
+    }
+    ns.truncated = truncated;
+}
+
+template <class R>
+struct Consts {
+    R h, inv_h, m, inv_rho0, relax, a, ad, inv_k, inv_wdq, sk, inv_dt, dtc, xsph, vc;
+    int mode, n;
+};
+
+// Host-side constants of pbf_solve.cpp:79-106 / :147-216 and
+// pbf_position_update.cu:224-238 (inv_k), plus the PAPER constants.
+template <class R>
+Consts<R> make_consts(const orc_params* p) {
+    Consts<R> k;
+    const float hf = p->smoothing_length;
+    const float inv_rho0f = 1.f / p->stable_density;
+    const float inv_tf = 1.f / p->time_step;
+    k.h = R(hf);
+    k.inv_h = R(1) / R(hf);
+    k.m = R(p->particle_mass);
+    k.inv_rho0 = R(inv_rho0f);
+    k.relax = R(p->relaxation);
+    R inv_h2 = k.inv_h * k.inv_h;
+    // pbf_kernel_memorization.cu:96-101 / :110-114
+    k.a = KC<R>::poly6 * k.inv_h * inv_h2;
+    k.ad = KC<R>::spiky * inv_h2 * inv_h2;
+    float ref_inv_k, inv_wdq;
+    orc_scorr_constants(p, &ref_inv_k, &inv_wdq);
+    k.inv_k = R(ref_inv_k);
+    k.inv_wdq = R(inv_wdq);
+    k.sk = R(p->scorr_k);
+    k.mode = p->scorr_mode;
+    k.n = p->scorr_n;
+    k.inv_dt = R(inv_tf);
+    k.dtc = R(1.f / inv_tf);  // pbf_solve.cpp:138 passes 1.f / inv_time_step
+    k.xsph = R(p->xsph_parameter);
+    k.vc = R(p->vc_parameter);
+    if (!std::is_same<R, float>::value) {
+        // FP64 twin: recompute the derived constants in double
+        k.inv_rho0 = R(1) / R(p->stable_density);
+        k.inv_dt = R(1) / R(p->time_step);
+        k.dtc = R(p->time_step);
+        R w04 = wpoly6<R>(R(hf) * R(0.4f), k.inv_h);
+        k.inv_k = R(1) / (w04 * w04 * w04 * w04);
+        k.inv_wdq = R(1) / wpoly6<R>(R(p->scorr_dq_over_h) * R(hf), k.inv_h);
+    }
+    return k;
+}
+
+// pbf_kernel_memorization.cu:15-56 (memorizeKernelPair): W and grad W of one
+// listed pair; rsqrtf restated as 1/sqrt.
+template <class R>
+inline void pair_terms(V3<R> self, V3<R> pj, const Consts<R>& k, R& W, V3<R>& kg) {
+    V3<R> d = self - pj;
+    R r2 = std::fma(d.z, d.z, std::fma(d.y, d.y, d.x * d.x));
+    R inv_r = R(1) / std::sqrt(r2);
+    R q2 = r2 * k.inv_h * k.inv_h;
+    W = R(0);
+    if (q2 <= R(1)) {
+        R t = R(1) - q2;
+        W = k.a * t * t * t;
+    }
+    R r = r2 * inv_r;
+    R u = std::fma(-r, k.inv_h, R(1));
+    R kd = R(0);
+    if (k.inv_h <= inv_r) kd = k.ad * u * u;
+    kg = v3<R>(R(0), R(0), R(0));
+    if (r2 > R(0)) {
+        V3<R> dir = mulvs(d, inv_r);
+        kg = mulsv(kd, dir);
+    }
+}
+
+// pbf/solver/cuda/boundary/pbf_plane_boundary.cu:6-45
+template <class R>
+inline void response_plane(bool& hit, V3<R>& dp, V3<R> old_p, V3<R> pred, V3<R> P, V3<R> n) {
+    V3<R> pa = old_p - P;
+    V3<R> pb = pred - P;
+    R pa_n = fdot(pa, n);
+    R pb_n = fdot(pb, n);
+    V3<R> I;
+    if (pa_n == R(0) && pb_n == R(0)) {
+        return;
+    } else if (pa_n >= R(0) && pb_n < R(0)) {
+        hit = true;
+        R frac = std::fabs(pa_n) / (std::fabs(pa_n) + std::fabs(pb_n));
+        I = old_p + mulvs(pred - old_p, frac);
+    } else if (pa_n < R(0) && pb_n < R(0)) {
+        hit = true;
+        R d = -fdot(P, n);
+        R t = -(fdot(pred, n) + d) / fdot(n, n);
+        I = pred + mulsv(t, n);
+    } else {
+        return;
+    }
+    V3<R> di = pred - I;
+    R c = fdot(di, n);
+    dp += mulsv(-c, n);
+}
+
+// pbf/solver/cuda/boundary/pbf_sphere_boundary.cu:8-34 (inner) and :36-64 (outer)
+template <class R>
+inline void response_sphere(bool& hit, V3<R>& dp, V3<R> pred, V3<R> center, R radius, bool inner) {
+    V3<R> cd = center - pred;
+    R d2 = fdot(cd, cd);
+    R rinv = R(1) / std::sqrt(d2);
+    bool coll = inner ? (d2 > radius * radius) : (d2 < radius * radius);
+    if (!coll) return;
+    hit = true;
+    R nudge = inner ? KC<R>::nudge_in : KC<R>::nudge_out;
+    dp.x = std::fma(cd.x, nudge, dp.x);
+    dp.y = std::fma(cd.y, nudge, dp.y);
+    dp.z = std::fma(cd.z, nudge, dp.z);
+    V3<R> tc = mulvs(cd, rinv);
+    dp.x = std::fma(-radius, tc.x, dp.x);
+    dp.y = std::fma(-radius, tc.y, dp.y);
+    dp.z = std::fma(-radius, tc.z, dp.z);
+}
+
+// pbf/solver/cuda/pbf_response_collision.cu:12-66: sweep spheres and planes
+// until a sweep sees no collision (capped at max_sweeps).
+template <class R>
+inline int collide(const orc_params* p, const orc_boundary* b, V3<R>& dp, V3<R> pk, bool& capped) {
+    V3<R> pred_x = pk, old_x = pk;
+    int sweeps = 0;
+    capped = false;
+    while (true) {
+        if (sweeps >= p->max_sweeps) {
+            capped = true;
+            break;
+        }
+        ++sweeps;
+        V3<R> pp = pred_x + dp;
+        bool hit = false;
+        if (b) {
+            for (uint32_t i = 0; i < b->n_inner; ++i) {
+                const float* s = b->inner + 4 * i;
+                response_sphere<R>(hit, dp, pp, v3<R>(R(s[0]), R(s[1]), R(s[2])), R(s[3]), true);
+                pp = pred_x + dp;
+            }
+            for (uint32_t i = 0; i < b->n_outer; ++i) {
+                const float* s = b->outer + 4 * i;
+                response_sphere<R>(hit, dp, pp, v3<R>(R(s[0]), R(s[1]), R(s[2])), R(s[3]), p->outer_as_inner != 0);
+                pp = pred_x + dp;
+            }
+            for (uint32_t i = 0; i < b->n_planes; ++i) {
+                const float* P = b->plane_points + 3 * i;
+                const float* n = b->plane_normals + 3 * i;
+                response_plane<R>(hit, dp, old_x, pp, v3<R>(R(P[0]), R(P[1]), R(P[2])),
+                                  v3<R>(R(n[0]), R(n[1]), R(n[2])));
+                pp = pred_x + dp;
+            }
+        }
+        if (!hit) break;
+    }
+    return sweeps;
+}
+
+template <class R>
+inline V3<R> load3(const std::vector<R>& a, uint32_t s) {
+    return v3<R>(a[3 * (size_t)s], a[3 * (size_t)s + 1], a[3 * (size_t)s + 2]);
+}
+template <class R>
+inline void store3(std::vector<R>& a, uint32_t s, V3<R> v) {
+    a[3 * (size_t)s] = v.x;
+    a[3 * (size_t)s + 1] = v.y;
+    a[3 * (size_t)s + 2] = v.z;
+}
+
+// calcContribution (pbf/interaction/cuda/detail/pbf_contribution.inl:7-57 and
+// its copies in pbf_xsph.cu:51-82, pbf_vorticity_confinement.cu:51-82,156-186):
+// the 27 cells around the cell of self_pos (its CURRENT position, hashed in
+// float), cell ranges from the search, per-cell partial sums added in z,y,x order.
+template <class R, class F>
+inline V3<R> sum27(const NSData& ns, const float selfp[3], F&& pair) {
+    int g[3];
+    grid_pos(selfp[0], selfp[1], selfp[2], ns.cw, g);
+    V3<R> sum = v3<R>(R(0), R(0), R(0));
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                uint32_t c = grid_hash(g[0] + dx, g[1] + dy, g[2] + dz, ns.G);
+                V3<R> cs = v3<R>(R(0), R(0), R(0));
+                for (uint32_t j = ns.off[c]; j < ns.off[c + 1]; ++j) cs += pair(j);
+                sum += cs;
+            }
+    return sum;
+}
+
+template <class R>
+inline void hash_pos(V3<R> p, float out[3]) {
+    out[0] = (float)p.x;
+    out[1] = (float)p.y;
+    out[2] = (float)p.z;
+}
+
+struct SolveStats {
+    uint32_t sweep_cap_hits = 0, max_sweeps_used = 0, xsph_nan = 0;
+};
+
+// Jacobi loop (pbf_solve.cpp:79-106: memo, scaling factor, position update,
+// collision, interim update) and update (pbf_solve.cpp:125-145; the full
+// order is the commented block :135-142: velocity, vorticity, confinement,
+// XSPH).  P is the sorted predicted state (AoS, R), X the sorted old state.
+template <class R>
+void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& ns, std::vector<R>& P,
+                      const std::vector<R>& X, int iters, std::vector<R>& x_out, std::vector<R>& v_out,
+                      float* lambda_last, SolveStats& st) {
+    const Consts<R> k = make_consts<R>(p);
+    const uint32_t nl = ns.n_live;
+    const int M = ns.max_pair;
+    std::vector<R> s(nl), Pn(3 * (size_t)nl);
+    uint32_t cap_hits = 0, max_sw = 0;
+
+    for (int it = 0; it < iters; ++it) {
+        // calcScalingFactorCUDA (memo) pbf/solver/cuda/pbf_constraint.cu:168-209
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
+            uint32_t i = (uint32_t)ii;
+            V3<R> self = load3(P, i);
+            R ks = R(0), kg2s = R(0);
+            V3<R> kgs = v3<R>(R(0), R(0), R(0));
+            const uint32_t* lst = &ns.nb[(size_t)i * M];
+            for (uint32_t c = 0; c < ns.nbc[i]; ++c) {
+                R W;
+                V3<R> kg;
+                pair_terms<R>(self, load3(P, lst[c]), k, W, kg);
+                R kg2 = fdot(kg, kg);
+                ks += W;
+                kgs += kg;
+                kg2s += kg2;
+            }
+            R C = std::fma(ks * k.m, k.inv_rho0, R(-1));
+            R G = std::fma(kgs.x, kgs.x, std::fma(kgs.z, kgs.z, std::fma(kgs.y, kgs.y, kg2s)));
+            R m2r2 = k.m * k.m * k.inv_rho0 * k.inv_rho0;
+            s[i] = C / std::fma(m2r2, G, k.relax);
+        }
+        // calcPositionUpdateCUDA (memo) pbf/solver/cuda/pbf_position_update.cu:131-188,
+        // then responseCollision and updateInterimPosition (pbf_constraint.cu:335-345)
+#pragma omp parallel for schedule(static) reduction(+ : cap_hits) reduction(max : max_sw)
+        for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
+            uint32_t i = (uint32_t)ii;
+            V3<R> self = load3(P, i);
+            R si = s[i];
+            R self_cl = (si < R(0)) ? -si : R(0);
+            V3<R> sum = v3<R>(R(0), R(0), R(0));
+            const uint32_t* lst = &ns.nb[(size_t)i * M];
+            for (uint32_t c = 0; c < ns.nbc[i]; ++c) {
+                uint32_t j = lst[c];
+                R W;
+                V3<R> kg;
+                pair_terms<R>(self, load3(P, j), k, W, kg);
+                R sj = s[j];
+                R corr;
+                if (k.mode == ORC_SCORR_REFERENCE) {
+                    R kr2 = W * W;
+                    R kr4 = kr2 * kr2;
+                    R tc = KC<R>::ten * kr4 * k.inv_k;
+                    R sup = (sj < R(0)) ? -sj : R(0);
+                    R cl = self_cl + sup;
+                    corr = cl * tc;
+                } else {
+                    corr = k.sk * powi<R>(W * k.inv_wdq, k.n);
+                }
+                R a = -(si + sj + corr);
+                sum += mulsv(a, kg);
+            }
+            V3<R> dp = v3<R>(sum.x * k.m * k.inv_rho0, sum.y * k.m * k.inv_rho0, sum.z * k.m * k.inv_rho0);
+            bool capped = false;
+            int sw = collide<R>(p, b, dp, self, capped);
+            if (capped) cap_hits++;
+            max_sw = std::max<uint32_t>(max_sw, (uint32_t)sw);
+            store3(Pn, i, self + dp);
+        }
+        std::copy(Pn.begin(), Pn.end(), P.begin());
+    }
+    st.sweep_cap_hits = cap_hits;
+    st.max_sweeps_used = max_sw;
+    if (lambda_last)
+        for (uint32_t i = 0; i < nl; ++i) lambda_last[i] = (float)s[i];
+
+    // pbf/solver/cuda/pbf_update.cu:25-39: v = inv_dt * (x* - x_sorted)
+    std::vector<R> V(3 * (size_t)nl);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
+        uint32_t i = (uint32_t)ii;
+        store3(V, i, mulsv(k.inv_dt, load3(P, i) - load3(X, i)));
+    }
+
+    const R warg = p->vort_h_as_inv_h ? k.h : k.inv_h;
+    if (p->vort_on) {
+        // calcVorticityCUDA pbf/solver/cuda/pbf_vorticity_confinement.cu:15-115
+        std::vector<R> Wv(3 * (size_t)nl);
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
+            uint32_t i = (uint32_t)ii;
+            V3<R> self = load3(P, i), selfv = load3(V, i);
+            float hp[3];
+            hash_pos(self, hp);
+            V3<R> w = sum27<R>(ns, hp, [&](uint32_t j) {
+                V3<R> pd = load3(P, j) - self;
+                R r = glm_length(pd);
+                V3<R> vd = load3(V, j) - selfv;
+                V3<R> vort = v3<R>(R(0), R(0), R(0));
+                if (r > R(0)) {
+                    V3<R> dir = divvs(pd, r);
+                    vort = mulvs(glm_cross(vd, dir), wspiky_deriv<R>(r, warg));
+                }
+                return vort;
+            });
+            store3(Wv, i, mulsv(k.m * k.inv_rho0, w));
+        }
+        // addVorticityConfinementCUDA pbf_vorticity_confinement.cu:121-222
+        std::vector<R> Vn(V);
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
+            uint32_t i = (uint32_t)ii;
+            V3<R> self = load3(P, i), selfw = load3(Wv, i);
+            float hp[3];
+            hash_pos(self, hp);
+            V3<R> N = sum27<R>(ns, hp, [&](uint32_t j) {
+                V3<R> pd = load3(P, j) - self;
+                R r = glm_length(pd);
+                R w = glm_length(load3(Wv, j));
+                V3<R> loc = v3<R>(R(0), R(0), R(0));
+                if (r > R(0)) {
+                    V3<R> dir = divvs(pd, r);
+                    loc = mulvs(mulsv(w, dir), wspiky_deriv<R>(r, warg));
+                }
+                return loc;
+            });
+            if (glm_length(N) > R(0)) {
+                N = mulvs(mulvs(N, k.m), k.inv_rho0);
+                N = divvs(N, glm_length(N));
+                V3<R> add = mulvs(divvs(mulsv(k.vc, glm_cross(N, selfw)), k.inv_rho0), k.dtc);
+                store3(Vn, i, load3(V, i) + add);
+            }
+        }
+        V.swap(Vn);
+    }
+
+    x_out.assign(3 * (size_t)nl, R(0));
+    v_out.assign(3 * (size_t)nl, R(0));
+    uint32_t nan_count = 0;
+    if (p->xsph_on) {
+        // calcXSPHCUDA pbf/solver/cuda/pbf_xsph.cu:15-113
+#pragma omp parallel for schedule(static) reduction(+ : nan_count)
+        for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
+            uint32_t i = (uint32_t)ii;
+            V3<R> self = load3(P, i), selfv = load3(V, i);
+            float hp[3];
+            hash_pos(self, hp);
+            V3<R> acc = sum27<R>(ns, hp, [&](uint32_t j) {
+                V3<R> pd = load3(P, j) - self;
+                R r = glm_length(pd);
+                V3<R> vd = load3(V, j) - selfv;
+                R kr = wpoly6<R>(r, k.inv_h);
+                return mulvs(vd, kr);
+            });
+            V3<R> out;
+            if (std::isnan(acc.x) || std::isnan(acc.y) || std::isnan(acc.z)) {
+                out = selfv;
+                nan_count++;
+            } else {
+                out = selfv + mulsv(k.xsph * k.m * k.inv_rho0, acc);
+            }
+            store3(v_out, i, out);
+        }
+    } else {
+        std::copy(V.begin(), V.end(), v_out.begin());
+    }
+    std::copy(P.begin(), P.begin() + 3 * (size_t)nl, x_out.begin());
+    st.xsph_nan = nan_count;
+}
+
+int g_threads = 0;
+
+void apply_threads() {
+#ifdef _OPENMP
+    if (g_threads > 0) omp_set_num_threads(g_threads);
+#endif
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_set_threads(int n) { g_threads = n; }
+
+void orc_arith_f32(const float* a, const float* b, float* out, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        out[4 * i + 0] = a[i] / b[i];
+        out[4 * i + 1] = std::sqrt(a[i]);
+        out[4 * i + 2] = std::fma(a[i], b[i], a[i]);
+        out[4 * i + 3] = std::floor(a[i] / b[i]);
+    }
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    apply_threads();
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+// pbf/pbf_type.cpp:141-162
+int orc_calc_grid_size(double cell_size, const float end[3], uint32_t out[3]) {
+    uint32_t g[3] = {1, 1, 1};
+    while (true) {
+        if (g[0] * cell_size > end[0] && g[1] * cell_size > end[1] && g[2] * cell_size > end[2]) break;
+        for (int a = 0; a < 3; ++a) g[a] = (g[a] * cell_size > end[a]) ? g[a] : g[a] * 2;
+        if (g[0] > 1028 || g[1] > 1028 || g[2] > 1028) return 1;
+    }
+    out[0] = g[0];
+    out[1] = g[1];
+    out[2] = g[2];
+    return 0;
+}
+
+float orc_wpoly6(float r, float inv_h) { return wpoly6<float>(r, inv_h); }
+float orc_wspiky_deriv(float r, float inv_h) { return wspiky_deriv<float>(r, inv_h); }
+double orc_wpoly6_f64(double r, double inv_h) { return wpoly6<double>(r, inv_h); }
+double orc_wspiky_deriv_f64(double r, double inv_h) { return wspiky_deriv<double>(r, inv_h); }
+
+void orc_hash(const float* x, uint32_t n, float cw, const uint32_t grid[3], uint32_t* key) {
+    for (uint32_t i = 0; i < n; ++i) {
+        int g[3];
+        grid_pos(x[3 * i], x[3 * i + 1], x[3 * i + 2], cw, g);
+        key[i] = grid_hash(g[0], g[1], g[2], grid);
+    }
+}
+
+// pbf_solve.cpp:90-93: static k = pow(weight(h*0.4f, inv_h), 4) in double,
+// narrowed to float by setConstantMemory, then inv_k = 1.f / k
+// (pbf_position_update.cu:224-238).  PAPER: inv_wdq = 1/W(dq).
+void orc_scorr_constants(const orc_params* p, float* ref_inv_k, float* paper_inv_wdq) {
+    const float h = p->smoothing_length;
+    const float inv_h = 1.f / h;
+    const float w04 = wpoly6<float>(h * 0.4f, inv_h);
+    const float kf = (float)std::pow((double)w04, 4);
+    if (ref_inv_k) *ref_inv_k = 1.f / kf;
+    const float wdq = wpoly6<float>(p->scorr_dq_over_h * h, inv_h);
+    if (paper_inv_wdq) *paper_inv_wdq = 1.f / wdq;
+}
+
+int orc_step_f32(const orc_params* p, const orc_boundary* b, float* x, float* v, uint32_t* num,
+                 const float lo[3], const float hi[3], int iters, orc_ns_out* nso, orc_stats* st) {
+    if (!p || !x || !v || !num || p->max_pair < 2) return 1;
+    apply_threads();
+    const uint32_t n = *num;
+    NSData ns;
+    neighbor_search(p, x, v, n, lo, hi, ns);
+    const uint32_t nl = ns.n_live;
+
+    std::vector<float> P(3 * (size_t)nl), X(3 * (size_t)nl);
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < (int64_t)nl; ++s) {
+        uint32_t i = ns.idx[s];
+        for (int c = 0; c < 3; ++c) {
+            P[3 * s + c] = ns.pred[3 * s + c];
+            X[3 * s + c] = x[3 * i + c];
+        }
+    }
+    if (nso) {
+        if (nso->key) std::memcpy(nso->key, ns.key.data(), sizeof(uint32_t) * n);
+        if (nso->sorted_index) std::memcpy(nso->sorted_index, ns.idx.data(), sizeof(uint32_t) * n);
+        if (nso->cell_offset) std::memcpy(nso->cell_offset, ns.off.data(), sizeof(uint32_t) * ns.off.size());
+        if (nso->neighbors) {
+            std::fill(nso->neighbors, nso->neighbors + (size_t)n * ns.max_pair, ORC_EMPTY);
+            std::memcpy(nso->neighbors, ns.nb.data(), sizeof(uint32_t) * ns.nb.size());
+        }
+        if (nso->neighbor_count) {
+            std::fill(nso->neighbor_count, nso->neighbor_count + n, 0u);
+            std::memcpy(nso->neighbor_count, ns.nbc.data(), sizeof(uint32_t) * nl);
+        }
+        if (nso->pred0) std::memcpy(nso->pred0, ns.pred.data(), sizeof(float) * 3 * (size_t)n);
+    }
+    std::vector<float> xo, vo;
+    SolveStats ss;
+    solve_and_update<float>(p, b, ns, P, X, iters, xo, vo, nso ? nso->lambda_last : nullptr, ss);
+    std::memcpy(x, xo.data(), sizeof(float) * xo.size());
+    std::memcpy(v, vo.data(), sizeof(float) * vo.size());
+    *num = nl;
+    if (st) {
+        st->num_in = n;
+        st->num_live = nl;
+        st->num_deleted = n - nl;
+        st->truncated = ns.truncated;
+        st->sweep_cap_hits = ss.sweep_cap_hits;
+        st->max_sweeps_used = ss.max_sweeps_used;
+        st->xsph_nan = ss.xsph_nan;
+        st->reserved = 0;
+    }
+    return 0;
+}
+
+int orc_step_f64(const orc_params* p, const orc_boundary* b, const float* x, const float* v, uint32_t num,
+                 const float lo[3], const float hi[3], int iters, double* x_out, double* v_out,
+                 uint32_t* num_out) {
+    if (!p || !x || !v || p->max_pair < 2) return 1;
+    apply_threads();
+    NSData ns;
+    neighbor_search(p, x, v, num, lo, hi, ns);
+    const uint32_t nl = ns.n_live;
+    std::vector<double> P(3 * (size_t)nl), X(3 * (size_t)nl);
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < (int64_t)nl; ++s) {
+        uint32_t i = ns.idx[s];
+        double q[3];
+        predict_one<double>(p, x, v, i, q);
+        for (int c = 0; c < 3; ++c) {
+            P[3 * s + c] = q[c];
+            X[3 * s + c] = (double)x[3 * i + c];
+        }
+    }
+    std::vector<double> xo, vo;
+    SolveStats ss;
+    solve_and_update<double>(p, b, ns, P, X, iters, xo, vo, nullptr, ss);
+    std::memcpy(x_out, xo.data(), sizeof(double) * xo.size());
+    std::memcpy(v_out, vo.data(), sizeof(double) * vo.size());
+    if (num_out) *num_out = nl;
+    return 0;
+}
+
+int orc_metrics(const orc_params* p, const float* x, const float* v, uint32_t n, double out[4]) {
+    apply_threads();
+    const uint32_t G[3] = {p->grid[0], p->grid[1], p->grid[2]};
+    const uint32_t C = G[0] * G[1] * G[2];
+    const float cw = p->smoothing_length;
+    std::vector<uint32_t> key(n), off((size_t)C + 1, 0), idx(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        int g[3];
+        grid_pos(x[3 * i], x[3 * i + 1], x[3 * i + 2], cw, g);
+        key[i] = grid_hash(g[0], g[1], g[2], G);
+        off[key[i] + 1]++;
+    }
+    for (size_t c = 1; c < off.size(); ++c) off[c] += off[c - 1];
+    {
+        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+        for (uint32_t i = 0; i < n; ++i) idx[cur[key[i]]++] = i;
+    }
+    const double inv_h = 1.0 / (double)p->smoothing_length;
+    const double m = p->particle_mass, rho0 = p->stable_density;
+    double sumC = 0, maxC = -1e300, sumPos = 0, ke = 0;
+#pragma omp parallel for schedule(static) reduction(+ : sumC, sumPos, ke) reduction(max : maxC)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        uint32_t i = (uint32_t)ii;
+        int g[3];
+        grid_pos(x[3 * i], x[3 * i + 1], x[3 * i + 2], cw, g);
+        double rho = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    uint32_t c = grid_hash(g[0] + dx, g[1] + dy, g[2] + dz, G);
+                    for (uint32_t t = off[c]; t < off[c + 1]; ++t) {
+                        uint32_t j = idx[t];
+                        double d0 = (double)x[3 * i] - x[3 * j], d1 = (double)x[3 * i + 1] - x[3 * j + 1],
+                               d2 = (double)x[3 * i + 2] - x[3 * j + 2];
+                        rho += m * wpoly6<double>(std::sqrt(d0 * d0 + d1 * d1 + d2 * d2), inv_h);
+                    }
+                }
+        double Ci = rho / rho0 - 1.0;
+        sumC += Ci;
+        sumPos += Ci > 0 ? Ci : 0;
+        maxC = std::max(maxC, Ci);
+        double vv = (double)v[3 * i] * v[3 * i] + (double)v[3 * i + 1] * v[3 * i + 1] + (double)v[3 * i + 2] * v[3 * i + 2];
+        ke += 0.5 * m * vv;
+    }
+    out[0] = n ? sumC / n : 0;
+    out[1] = n ? maxC : 0;
+    out[2] = n ? sumPos / n : 0;
+    out[3] = ke;
+    return 0;
+}
+
+}  // extern "C"

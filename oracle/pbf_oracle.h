@@ -1,0 +1,152 @@
+/*
+ * pbf_oracle.h -- CPU restatement of the reference PBF solver step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP solver
+ * (libpbfx.so).  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it; the solver product never links or calls it.
+ *
+ * It restates, in host C++ with IEEE FP32 arithmetic and the reference's
+ * operation order, the step driven by pbf::one_step
+ * (cuda_pbf_with_ssf/src/pbf/solver/pbf_solve.cpp:182-216), with an FP64
+ * twin that replays the same discrete decisions (sort, neighbour lists,
+ * cell lookups) in double precision to state the FP32 rounding error.
+ *
+ * Deliberate deviations from the shipped reference (all documented in the
+ * repository design notes):
+ *   - stable counting sort (ties by original index) instead of the
+ *     atomicAdd-order ranks of pbf/util/cuda/pbf_counting_sort.cu:21-32;
+ *   - deleted particles use the sentinel key C = Gx*Gy*Gz, not the
+ *     gz*gy*gz-1 cell of pbf/interaction/pbf_neighbor_search.cpp:146;
+ *   - 1/sqrtf(x) (correctly rounded) where the reference calls rsqrtf;
+ *   - the collision sweep loop is capped (max_sweeps) instead of unbounded;
+ *   - vorticity uses inv_h in the spiky derivative (the reference passes h,
+ *     pbf/solver/cuda/pbf_vorticity_confinement.cu:28,134) unless
+ *     vort_h_as_inv_h is set;
+ *   - outer spheres use the outer-sphere response (the reference calls the
+ *     inner one, pbf/solver/cuda/pbf_response_collision.cu:46-51) unless
+ *     outer_as_inner is set.
+ */
+#ifndef PBF_ORACLE_H
+#define PBF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_EMPTY 0xFFFFFFFFu
+
+enum { ORC_SCORR_REFERENCE = 0, ORC_SCORR_PAPER = 1 };
+
+typedef struct orc_params {
+    /* pbf_parameter (pbf/pbf_type.h:49-63) */
+    float stable_density;
+    float stable_distance;
+    float particle_mass;
+    float smoothing_length;   /* h; also the hash cell width */
+    float time_step;
+    float relaxation;         /* epsilon of the lambda denominator */
+    float xsph_parameter;
+    float vc_parameter;
+    /* pbf_external (pbf/pbf_type.h:69-72) */
+    float body_force[3];
+    /* tensile correction */
+    int32_t scorr_mode;       /* ORC_SCORR_REFERENCE or ORC_SCORR_PAPER */
+    float scorr_k;
+    int32_t scorr_n;
+    float scorr_dq_over_h;
+    /* neighbour slots per particle; the reference uses 30 (29 usable) */
+    int32_t max_pair;
+    /* update stage switches: both 0 reproduces the shipped update
+       (pbf_solve.cpp:125-145); both 1 is the commented full order */
+    int32_t xsph_on;
+    int32_t vort_on;
+    int32_t vort_h_as_inv_h;
+    int32_t outer_as_inner;
+    int32_t max_sweeps;       /* collision sweep cap per Jacobi iteration */
+    uint32_t grid[3];         /* power-of-two grid from calc_grid_size */
+} orc_params;
+
+typedef struct orc_boundary {
+    uint32_t n_inner;
+    const float* inner;       /* 4 floats each: center xyz, radius */
+    uint32_t n_outer;
+    const float* outer;
+    uint32_t n_planes;
+    const float* plane_points;  /* 3 floats each */
+    const float* plane_normals; /* 3 floats each */
+} orc_boundary;
+
+/* Optional views of the neighbour search; any pointer may be NULL. */
+typedef struct orc_ns_out {
+    uint32_t* key;           /* [n_in]  hash key per ORIGINAL particle (sentinel C if deleted) */
+    uint32_t* sorted_index;  /* [n_in]  slot -> original particle */
+    uint32_t* cell_offset;   /* [C+2]   exclusive scan of cell counts (sentinel cell C last) */
+    uint32_t* neighbors;     /* [n_in*max_pair] sorted neighbour slots, row per particle, ORC_EMPTY padded */
+    uint32_t* neighbor_count;/* [n_in] */
+    float* pred0;            /* [3*n_in] sorted predicted positions (AoS) frozen for the list */
+    float* lambda_last;      /* [n_in]  scaling factor s of the last Jacobi iteration */
+} orc_ns_out;
+
+typedef struct orc_stats {
+    uint32_t num_in;
+    uint32_t num_live;
+    uint32_t num_deleted;
+    uint32_t truncated;       /* particles that lost at least one neighbour to the cap */
+    uint32_t sweep_cap_hits;  /* particle-iterations that hit max_sweeps */
+    uint32_t max_sweeps_used;
+    uint32_t xsph_nan;        /* particles whose XSPH sum was NaN (velocity kept) */
+    uint32_t reserved;
+} orc_stats;
+
+/* pbf_type.cpp:141-162 (calcGridSize).  Returns 0, or 1 if a dimension
+   exceeds 1028. */
+int orc_calc_grid_size(double cell_size, const float domain_end[3], uint32_t out[3]);
+
+/* pbf/kernel/cuda/detail/pbf_kernel.inl:34-63 */
+float orc_wpoly6(float r, float inv_h);
+float orc_wspiky_deriv(float r, float inv_h);
+double orc_wpoly6_f64(double r, double inv_h);
+double orc_wspiky_deriv_f64(double r, double inv_h);
+
+/* pbf_grid.h:20-28,54-61: key per position (n positions, AoS). */
+void orc_hash(const float* x, uint32_t n, float cell_width, const uint32_t grid[3], uint32_t* key);
+
+/* Tensile-correction constants as the host computes them:
+   ref_inv_k = 1/(float)pow(W(0.4h),4) (pbf_solve.cpp:92, pbf_position_update.cu:224-238),
+   paper_inv_wdq = 1/W(dq). */
+void orc_scorr_constants(const orc_params* p, float* ref_inv_k, float* paper_inv_wdq);
+
+/* One full step, FP32.  x, v: AoS [num*3], overwritten in the new sorted
+   order (F10).  *num is shrunk by deletion.  Returns 0 on success. */
+int orc_step_f32(const orc_params* p, const orc_boundary* b,
+                 float* x, float* v, uint32_t* num,
+                 const float lo[3], const float hi[3], int iters,
+                 orc_ns_out* ns, orc_stats* st);
+
+/* FP64 twin: same inputs (float), same discrete decisions as orc_step_f32,
+   continuous arithmetic in double.  Writes x_out, v_out (double AoS). */
+int orc_step_f64(const orc_params* p, const orc_boundary* b,
+                 const float* x, const float* v, uint32_t num,
+                 const float lo[3], const float hi[3], int iters,
+                 double* x_out, double* v_out, uint32_t* num_out);
+
+/* Density-constraint and energy metrics on a state (sorted or not):
+   C_i = rho_i/rho0 - 1 with an uncapped 27-cell poly6 sum.
+   out[0]=mean C, out[1]=max C, out[2]=mean max(C,0), out[3]=kinetic energy. */
+int orc_metrics(const orc_params* p, const float* x, const float* v, uint32_t num, double out[4]);
+
+/* Host IEEE reference of pbfx_selftest_arith: out[4i..4i+3] =
+   a/b, sqrtf(a), fmaf(a,b,a), floorf(a/b). */
+void orc_arith_f32(const float* a, const float* b, float* out, uint32_t n);
+
+/* Number of OpenMP threads the oracle uses (0 = runtime default). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

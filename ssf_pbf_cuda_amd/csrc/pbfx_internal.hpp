@@ -1,0 +1,118 @@
+// pbfx_internal.hpp -- shared definitions of the pbfx HIP library (gfx950).
+//
+// Layout in HBM (capacity `cap` particles, C cells):
+//   key, rank, tmp, sorted_index   u32[cap]      hash / histogram rank / scatter / slot->index
+//   PA, PB                         f32[3*cap]    SoA ping-pong of the predicted positions
+//   X, V, OM                       f32[3*cap]    SoA sorted old positions, velocities, vorticity
+//   lam                            f32[cap]      scaling factor s (= -lambda)
+//   list                           u16[cap*stride] frozen neighbour lists as tile indices,
+//                                                 count in the row's last entry
+//   count                          u32[C+1]      cell histogram (cell C = deleted)
+//   offset                         u32[C+2]      exclusive scan: cell c owns [off[c], off[c+1])
+//   tsum, tprefix                  u32[nT+1]     per-4096-cell-tile totals and their scan
+//   units                          uint2[cap]    work units (first cell key, width in cells)
+//   ctr                            u32[16]       counters of the last step
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pbfx {
+
+constexpr int kBlock = 256;               // threads per work-unit workgroup
+constexpr int kWaves = kBlock / 64;
+constexpr uint32_t kMaxW = 16;            // widest work unit along x (cells)
+constexpr uint32_t kMaxCells = 9 * (kMaxW + 2);
+constexpr uint32_t kTileCapMax = 4096;
+constexpr uint32_t kTileIndexLimit = 65535;  // tile indices are stored as u16
+constexpr uint32_t kScanTile = 4096;      // cells per scan tile
+
+enum Ctr : int {
+    CTR_UNITS = 0,
+    CTR_TRUNC,
+    CTR_SWEEPCAP,
+    CTR_MAXSWEEP,
+    CTR_XSPHNAN,
+    CTR_SLOWUNITS,
+    CTR_SLOWPART,
+    CTR_OVERFLOW,
+    CTR_MAXTILE,
+    CTR_NUM = 16
+};
+
+struct Grid {
+    uint32_t gx, gy, gz;
+    uint32_t lgx, lgy;
+    uint32_t C;
+    float cw;
+};
+
+// Constants of one step, all computed on the host in IEEE float exactly as
+// the reference's host code does (pbf_solve.cpp:147-216, pbf_position_update.cu:224-238).
+struct Phys {
+    float h, inv_h, h2;
+    float a, ad;            // memo: poly6*inv_h*inv_h^2, -spiky*inv_h^2*inv_h^2
+    float m, inv_rho0, relax, m2r2;
+    float inv_k;            // REFERENCE s_corr: 1/(float)pow(W(0.4h),4)
+    float inv_wdq, sk;      // PAPER s_corr: 1/W(dq), k
+    int32_t mode, sn;
+    float dt, inv_dt, dtc;  // dtc = 1/inv_dt, the confinement time step
+    float g[3];
+    float xsph_c, vc, warg; // warg: inv_h, or h to reproduce the vorticity argument mix-up
+    int32_t cap;            // neighbours kept per particle (max_pair - 1)
+    int32_t stride;         // u16 per list row
+    int32_t max_sweeps;
+    int32_t outer_as_inner;
+};
+
+struct Boundary {
+    const float4* inner;
+    uint32_t n_inner;
+    const float4* outer;
+    uint32_t n_outer;
+    const float* pts;
+    const float* nrm;
+    uint32_t n_planes;
+};
+
+struct Bufs {
+    uint32_t cap;
+    uint32_t* key;
+    uint32_t* rank;
+    uint32_t* tmp;
+    uint32_t* sidx;
+    float* PA;
+    float* PB;
+    float* X;
+    float* V;
+    float* OM;
+    float* lam;
+    uint16_t* list;
+    uint32_t* count;
+    uint32_t* offset;
+    uint32_t* tsum;
+    uint32_t* tprefix;
+    uint2* units;
+    uint32_t* ctr;
+    uint32_t lgT;
+    uint32_t nT;
+};
+
+struct Tuning {
+    uint32_t tile_cap;      // particles staged on chip per unit
+    uint32_t unit_width;    // max cells per unit along x
+    uint32_t grid_units;    // persistent workgroups for unit kernels
+};
+
+// Launchers (pbfx_kernels.hip).  All enqueue on `s` and never synchronise.
+hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bufs& b, const Tuning& T,
+                       float* x, float* v, uint32_t n_in, const float lo[3], const float hi[3], int iters,
+                       int xsph_on, int vort_on, hipStream_t s);
+hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, const Tuning& T, uint32_t* out,
+                                 uint32_t* counts, uint32_t max_pair, hipStream_t s);
+hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uint32_t* end, hipStream_t s);
+hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s);
+hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t n, hipStream_t s);
+int unit_blocks_per_cu(uint32_t tile_cap);
+
+}  // namespace pbfx

@@ -1,0 +1,1127 @@
+// pbfx_kernels.hip -- HIP kernels of the PBF solver step for gfx950 (CDNA4).
+//
+// One step = 5 bookkeeping kernels (predict+hash+histogram, two-level scan,
+// scatter, stable reorder) followed by persistent work-unit kernels that
+// stage the 27-cell neighbourhood of a block of cells in LDS:
+//   k_ns_lambda  neighbour lists (frozen, capped) + lambda of iteration 0
+//   k_deltap     delta-p with tensile correction + boundary response + apply
+//   k_lambda     lambda of iterations 1..I-1
+//   k_vort       velocity + vorticity            (vort_on)
+//   k_confine    vorticity confinement           (vort_on)
+//   k_xsph_out   XSPH + AoS write-back           (xsph_on)
+//   k_out        AoS write-back                  (otherwise)
+// All kernels are HBM/LDS-bound integer or FP32 vector work; nothing is a
+// contraction, so MFMA is not used.  Arithmetic follows the reference's
+// operation order with IEEE division/sqrt and explicit fmaf (the library is
+// compiled with -ffp-contract=off), so results are reproducible bit-for-bit
+// on the host.
+#include "pbfx_internal.hpp"
+
+#include <climits>
+
+namespace pbfx {
+namespace {
+
+// ---------------------------------------------------------------- vector math
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 mk3(float x, float y, float z) {
+    F3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ void acc(F3& a, F3 b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+}
+__device__ __forceinline__ F3 mulvs(F3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 mulsv(float s, F3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ F3 divvs(F3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+// GLM 0.9.7.1 dot/length/cross (external/include/glm/detail/func_geometric.inl:65-72,95-100,134-142)
+__device__ __forceinline__ float glm_dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float glm_length(F3 a) { return __builtin_sqrtf(glm_dot(a, a)); }
+__device__ __forceinline__ F3 glm_cross(F3 x, F3 y) {
+    return mk3(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y);
+}
+// the hand-fused dot of the solver kernels: fmaf(a.z,b.z, fmaf(a.y,b.y, a.x*b.x))
+__device__ __forceinline__ float fdot(F3 a, F3 b) { return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x)); }
+
+__device__ __forceinline__ F3 ld3(const float* __restrict__ a, uint32_t cap, uint32_t i) {
+    return mk3(a[i], a[cap + i], a[2 * cap + i]);
+}
+__device__ __forceinline__ void st3(float* __restrict__ a, uint32_t cap, uint32_t i, F3 v) {
+    a[i] = v.x;
+    a[cap + i] = v.y;
+    a[2 * cap + i] = v.z;
+}
+
+// pbf/kernel/cuda/detail/pbf_kernel.inl:34-48
+__device__ __forceinline__ float wpoly6(float r, float inv_h) {
+    float q = r * inv_h;
+    float alpha = 1.56668147065f * inv_h * inv_h * inv_h;
+    if (q <= 1.f) {
+        float q2 = q * q;
+        float t = 1.f - q2;
+        return alpha * t * t * t;
+    }
+    return 0.f;
+}
+
+// pbf/kernel/cuda/detail/pbf_kernel.inl:50-63
+__device__ __forceinline__ float wspiky_deriv(float r, float inv_h) {
+    float q = r * inv_h;
+    float inv_h2 = inv_h * inv_h;
+    if (q <= 1.f) {
+        float t = 1.f - q;
+        float inv_h4 = inv_h2 * inv_h2;
+        return -14.3239448745f * inv_h4 * t * t;
+    }
+    return 0.f;
+}
+
+// float -> int with a defined result everywhere (NaN -> 0, saturation), the
+// behaviour of the device conversion the reference relies on.
+__device__ __forceinline__ int f2i(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483648.0f) return INT_MAX;
+    if (f <= -2147483648.0f) return INT_MIN;
+    return (int)f;
+}
+
+// pbf/interaction/cuda/pbf_grid.h:20-28 (IEEE division) and :54-61 (wrap)
+__device__ __forceinline__ void grid_pos(F3 p, float cw, int g[3]) {
+    g[0] = f2i(floorf(p.x / cw));
+    g[1] = f2i(floorf(p.y / cw));
+    g[2] = f2i(floorf(p.z / cw));
+}
+__device__ __forceinline__ uint32_t grid_hash(int gx, int gy, int gz, const Grid& G) {
+    uint32_t x = (uint32_t)gx & (G.gx - 1u);
+    uint32_t y = (uint32_t)gy & (G.gy - 1u);
+    uint32_t z = (uint32_t)gz & (G.gz - 1u);
+    return z * G.gx * G.gy + y * G.gx + x;
+}
+
+// t^n by binary exponentiation; n = 4 evaluates (t*t)*(t*t)
+__device__ __forceinline__ float powi(float t, int n) {
+    float result = 1.f, base = t;
+    bool first = true;
+    while (n > 0) {
+        if (n & 1) {
+            result = first ? base : result * base;
+            first = false;
+        }
+        n >>= 1;
+        if (n) base = base * base;
+    }
+    return result;
+}
+
+// pbf/solver/cuda/pbf_kernel_memorization.cu:15-56 (memorizeKernelPair):
+// W and grad W of one listed pair (rsqrtf restated as 1/sqrtf).
+__device__ __forceinline__ void pair_terms(F3 self, F3 pj, const Phys& P, float& W, F3& kg) {
+    F3 d = self - pj;
+    float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+    float inv_r = 1.f / __builtin_sqrtf(r2);
+    float q2 = r2 * P.inv_h * P.inv_h;
+    W = 0.f;
+    if (q2 <= 1.f) {
+        float t = 1.f - q2;
+        W = P.a * t * t * t;
+    }
+    float r = r2 * inv_r;
+    float u = __builtin_fmaf(-r, P.inv_h, 1.f);
+    float kd = 0.f;
+    if (P.inv_h <= inv_r) kd = P.ad * u * u;
+    kg = mk3(0.f, 0.f, 0.f);
+    if (r2 > 0.f) kg = mulsv(kd, mulvs(d, inv_r));
+}
+
+// pbf/solver/cuda/boundary/pbf_plane_boundary.cu:6-45
+__device__ __forceinline__ void response_plane(bool& hit, F3& dp, F3 old_p, F3 pred, F3 Pp, F3 n) {
+    F3 pa = old_p - Pp;
+    F3 pb = pred - Pp;
+    float pa_n = fdot(pa, n);
+    float pb_n = fdot(pb, n);
+    F3 I;
+    if (pa_n == 0.f && pb_n == 0.f) {
+        return;
+    } else if (pa_n >= 0.f && pb_n < 0.f) {
+        hit = true;
+        float frac = fabsf(pa_n) / (fabsf(pa_n) + fabsf(pb_n));
+        I = old_p + mulvs(pred - old_p, frac);
+    } else if (pa_n < 0.f && pb_n < 0.f) {
+        hit = true;
+        float d = -fdot(Pp, n);
+        float t = -(fdot(pred, n) + d) / fdot(n, n);
+        I = pred + mulsv(t, n);
+    } else {
+        return;
+    }
+    F3 di = pred - I;
+    float c = fdot(di, n);
+    acc(dp, mulsv(-c, n));
+}
+
+// pbf/solver/cuda/boundary/pbf_sphere_boundary.cu:8-34 (inner), :36-64 (outer)
+__device__ __forceinline__ void response_sphere(bool& hit, F3& dp, F3 pred, F3 center, float radius, bool inner) {
+    F3 cd = center - pred;
+    float d2 = fdot(cd, cd);
+    float rinv = 1.f / __builtin_sqrtf(d2);
+    bool coll = inner ? (d2 > radius * radius) : (d2 < radius * radius);
+    if (!coll) return;
+    hit = true;
+    float nudge = inner ? 1.0001f : 0.9999f;
+    dp.x = __builtin_fmaf(cd.x, nudge, dp.x);
+    dp.y = __builtin_fmaf(cd.y, nudge, dp.y);
+    dp.z = __builtin_fmaf(cd.z, nudge, dp.z);
+    F3 tc = mulvs(cd, rinv);
+    dp.x = __builtin_fmaf(-radius, tc.x, dp.x);
+    dp.y = __builtin_fmaf(-radius, tc.y, dp.y);
+    dp.z = __builtin_fmaf(-radius, tc.z, dp.z);
+}
+
+// pbf/solver/cuda/pbf_response_collision.cu:12-66: sweeps until none collides
+// (capped).  Returns the sweeps used; `capped` when the cap stopped it.
+__device__ __forceinline__ int collide(const Phys& P, const Boundary& B, F3& dp, F3 pk, bool& capped) {
+    const F3 pred_x = pk, old_x = pk;
+    int sweeps = 0;
+    capped = false;
+    while (true) {
+        if (sweeps >= P.max_sweeps) {
+            capped = true;
+            break;
+        }
+        ++sweeps;
+        F3 pp = pred_x + dp;
+        bool hit = false;
+        for (uint32_t i = 0; i < B.n_inner; ++i) {
+            float4 s = B.inner[i];
+            response_sphere(hit, dp, pp, mk3(s.x, s.y, s.z), s.w, true);
+            pp = pred_x + dp;
+        }
+        for (uint32_t i = 0; i < B.n_outer; ++i) {
+            float4 s = B.outer[i];
+            response_sphere(hit, dp, pp, mk3(s.x, s.y, s.z), s.w, P.outer_as_inner != 0);
+            pp = pred_x + dp;
+        }
+        for (uint32_t i = 0; i < B.n_planes; ++i) {
+            F3 Pp = mk3(B.pts[3 * i], B.pts[3 * i + 1], B.pts[3 * i + 2]);
+            F3 n = mk3(B.nrm[3 * i], B.nrm[3 * i + 1], B.nrm[3 * i + 2]);
+            response_plane(hit, dp, old_x, pp, Pp, n);
+            pp = pred_x + dp;
+        }
+        if (!hit) break;
+    }
+    return sweeps;
+}
+
+// ------------------------------------------------------------ block helpers
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Exclusive scan over the block (kBlock threads); `total` = block sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    const uint32_t nw = blockDim.x >> 6;
+    for (uint32_t w = 0; w < nw; ++w) {
+        uint32_t s = wsum[w];
+        if (w < wid) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + x - v;
+}
+
+// Wave-aggregated atomicAdd(&ctr[k], 1) returning a unique rank per lane
+// (ranks among equal keys are arbitrary; the reorder makes them stable).
+__device__ __forceinline__ uint32_t wave_rank(uint32_t* ctr, uint32_t k, bool active) {
+    uint64_t pending = __ballot(active);
+    const uint64_t lt = (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id()));
+    uint32_t mine = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lk = __shfl(k, leader, 64);
+        const uint64_t same = __ballot(active && k == lk) & pending;
+        uint32_t base = 0;
+        if ((int)lane_id() == leader) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if (active && k == lk) mine = base + (uint32_t)__popcll(same & lt);
+        pending &= ~same;
+    }
+    return mine;
+}
+
+// ------------------------------------------------- K1: predict + hash + histogram
+// pbf_predict.cu:15-45 (v* = fmaf(g,dt,v), x* = fmaf(v*,dt,x)), calcHashCUDA
+// (pbf_neighbor_search_util.cu:20-37), checkDeleteParticleCUDA
+// (pbf_delete.cu:9-28, sentinel key C) and the histogram of insertAndCountCUDA
+// (pbf_counting_sort.cu:21-32) with wave-aggregated atomics; the per-tile
+// totals feed the two-level scan.
+__global__ __launch_bounds__(256) void k_predict_hash(const float* __restrict__ x, const float* __restrict__ v,
+                                                      uint32_t n, Grid G, float dt, float g0, float g1, float g2,
+                                                      float lo0, float lo1, float lo2, float hi0, float hi1,
+                                                      float hi2, uint32_t* __restrict__ key,
+                                                      uint32_t* __restrict__ rank, uint32_t* __restrict__ count,
+                                                      uint32_t* __restrict__ tsum, uint32_t lgT) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    uint32_t k = 0;
+    if (active) {
+        float q[3];
+        const float gg[3] = {g0, g1, g2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float vs = __builtin_fmaf(gg[c], dt, v[3 * i + c]);
+            q[c] = __builtin_fmaf(vs, dt, x[3 * i + c]);
+        }
+        int gp[3];
+        grid_pos(mk3(q[0], q[1], q[2]), G.cw, gp);
+        k = grid_hash(gp[0], gp[1], gp[2], G);
+        if (q[0] < lo0 || hi0 < q[0] || q[1] < lo1 || hi1 < q[1] || q[2] < lo2 || hi2 < q[2]) k = G.C;
+    }
+    const uint32_t r = wave_rank(count, k, active);
+    wave_rank(tsum, k >> lgT, active);
+    if (active) {
+        key[i] = k;
+        rank[i] = r;
+    }
+}
+
+// ------------------------------------------------- K2: scan of the tile totals
+// One workgroup: exclusive scan of tsum[0..nT] (nT real tiles + the deleted
+// bucket), and reset of the step counters.
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict__ tsum,
+                                                     uint32_t* __restrict__ tprefix, uint32_t nT1,
+                                                     uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t per = (nT1 + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t s = 0;
+    for (uint32_t j = 0; j < per; ++j)
+        if (b0 + j < nT1) s += tsum[b0 + j];
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    uint32_t xs = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(xs, d, 64);
+        if (lane >= (uint32_t)d) xs += y;
+    }
+    if (lane == 63) wsum[wid] = xs;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t w = 0; w < wid; ++w) off += wsum[w];
+    uint32_t run = off + xs - s;
+    for (uint32_t j = 0; j < per; ++j)
+        if (b0 + j < nT1) {
+            tprefix[b0 + j] = run;
+            run += tsum[b0 + j];
+        }
+    if (threadIdx.x < CTR_NUM) ctr[threadIdx.x] = 0;
+}
+
+// ------------------------------------------------- K3: scan of the cell counts
+// Workgroup t scans the counts of cell tile t (empty tiles only write their
+// prefix), clears the counts it read and its tile total; the last workgroup
+// writes the deleted bucket: off[C] = live count, off[C+1] = n.
+// (CUB DeviceScan::ExclusiveSum in pbf_counting_sort.cu:62-71.)
+__global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
+                                                    uint32_t* __restrict__ tsum,
+                                                    const uint32_t* __restrict__ tprefix, uint32_t nT,
+                                                    uint32_t lgT, uint32_t C) {
+    __shared__ uint32_t wsum[kWaves];
+    const uint32_t t = blockIdx.x;
+    if (t == nT) {
+        if (threadIdx.x == 0) {
+            offset[C] = tprefix[nT];
+            offset[C + 1] = tprefix[nT] + tsum[nT];
+            count[C] = 0;
+            tsum[nT] = 0;
+        }
+        return;
+    }
+    const uint32_t T = 1u << lgT;
+    const uint32_t c0 = t << lgT;
+    const uint32_t tot = tsum[t];
+    const uint32_t base = tprefix[t];
+    const uint32_t per = (T + blockDim.x - 1) / blockDim.x;
+    const uint32_t my0 = c0 + threadIdx.x * per;
+    const uint32_t myn = (threadIdx.x * per < T) ? min(per, T - threadIdx.x * per) : 0u;
+    if (tot == 0) {
+        for (uint32_t j = 0; j < myn; ++j) offset[my0 + j] = base;
+        return;
+    }
+    uint32_t s = 0;
+    for (uint32_t j = 0; j < myn; ++j) s += count[my0 + j];
+    uint32_t total;
+    uint32_t run = base + block_excl_scan(s, wsum, total);
+    for (uint32_t j = 0; j < myn; ++j) {
+        uint32_t c = count[my0 + j];
+        offset[my0 + j] = run;
+        run += c;
+        count[my0 + j] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) tsum[t] = 0;
+}
+
+// ------------------------------------------------- K4: scatter
+// sortByCountingCUDA (pbf_counting_sort.cu:120-134): slot = off[key] + rank.
+__global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
+                                                 const uint32_t* __restrict__ offset, uint32_t* __restrict__ tmp,
+                                                 uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tmp[offset[key[i]] + rank[i]] = i;
+}
+
+// ------------------------------------------------- K5: stable reorder
+// Makes the in-cell order ascending by original index (the stable order of
+// the reference's radix-sort paths, pbf_neighbor_search.cpp:112-131), gathers
+// x* and x into SoA (reorderDataCUDA, pbf_neighbor_search_util.cu:143-154)
+// and emits the work units: each run of up to `uw` cells along x is split into
+// narrower units where it is dense, so that a unit's 27-cell neighbourhood
+// fits the on-chip tile.
+__global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, const float* __restrict__ v, Grid G,
+                                                 float dt, float g0, float g1, float g2,
+                                                 const uint32_t* __restrict__ key, const uint32_t* __restrict__ tmp,
+                                                 const uint32_t* __restrict__ offset, uint32_t* __restrict__ sidx,
+                                                 float* __restrict__ PA, float* __restrict__ X, uint32_t cap,
+                                                 uint32_t n, uint32_t uw, uint32_t budget, uint2* __restrict__ units,
+                                                 uint32_t* __restrict__ ctr) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = tmp[s];
+    const uint32_t k = key[i];
+    const uint32_t o0 = offset[k], o1 = offset[k + 1];
+    uint32_t r = 0;
+    for (uint32_t t = o0; t < o1; ++t) r += (tmp[t] < i) ? 1u : 0u;
+    const uint32_t f = o0 + r;
+    sidx[f] = i;
+    const float gg[3] = {g0, g1, g2};
+    float q[3], xo[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        xo[c] = x[3 * i + c];
+        float vs = __builtin_fmaf(gg[c], dt, v[3 * i + c]);
+        q[c] = __builtin_fmaf(vs, dt, xo[c]);
+    }
+    st3(PA, cap, f, mk3(q[0], q[1], q[2]));
+    st3(X, cap, f, mk3(xo[0], xo[1], xo[2]));
+    if (k < G.C) {
+        const uint32_t kb = k & ~(uw - 1u);
+        if (f == offset[kb]) {
+            const uint32_t nseg = offset[kb + uw] - offset[kb];
+            uint32_t W = uw;
+            while (W > 1u && 9ull * (W + 2u) * nseg > (unsigned long long)budget * uw) W >>= 1;
+            for (uint32_t q0 = 0; q0 < uw; q0 += W) {
+                const uint32_t kq = kb + q0;
+                if (offset[kq + W] > offset[kq]) {
+                    const uint32_t u = atomicAdd(&ctr[CTR_UNITS], 1u);
+                    units[u] = make_uint2(kq, W);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------- work units
+struct UnitShared {
+    uint32_t tstart[kMaxCells + 1];  // tile start of each neighbourhood cell (row-major 9 x (W+2))
+    uint32_t gstart[kMaxCells];      // sorted slot of each neighbourhood cell's first particle
+    uint32_t wsum[kWaves];
+};
+
+struct UnitInfo {
+    uint32_t W, ncell, xb, y, z, total, own_t0, own_g0, n_own;
+    bool staged, ok;
+};
+
+// XCD-aware split: workgroup b runs on XCD b % 8; each XCD takes a contiguous
+// share of the (roughly key-ordered) unit list so neighbouring units share an L2.
+__device__ __forceinline__ void unit_range(uint32_t nU, uint32_t& u0, uint32_t& u1, uint32_t& step) {
+    const uint32_t nx = 8, b = blockIdx.x, Gd = gridDim.x;
+    if (Gd < nx) {
+        u0 = b;
+        u1 = nU;
+        step = Gd;
+        return;
+    }
+    const uint32_t xcd = b % nx, lb = b / nx;
+    const uint32_t nbx = (Gd - xcd + nx - 1) / nx;
+    const uint32_t chunk = (nU + nx - 1) / nx;
+    u0 = xcd * chunk + lb;
+    u1 = min(nU, (xcd + 1) * chunk);
+    step = nbx;
+}
+
+// Builds the neighbourhood table of a unit: 9 rows (dz outer, dy inner -- the
+// reference's z,y order) of W+2 cells (x0-1 .. x0+W, wrapped), their tile
+// starts and sorted slots.  Contains barriers: call uniformly.
+__device__ UnitInfo unit_setup(const Grid& G, const uint32_t* __restrict__ offset, uint2 unit, uint32_t tile_cap,
+                               UnitShared& S, uint32_t* __restrict__ ctr) {
+    UnitInfo I;
+    I.W = unit.y;
+    I.ncell = 9u * (I.W + 2u);
+    const uint32_t kb = unit.x;
+    I.xb = kb & (G.gx - 1u);
+    I.y = (kb >> G.lgx) & (G.gy - 1u);
+    I.z = kb >> (G.lgx + G.lgy);
+    const uint32_t e = threadIdx.x;
+    uint32_t cnt = 0;
+    if (e < I.ncell) {
+        const uint32_t r = e / (I.W + 2u), kk = e - r * (I.W + 2u);
+        const uint32_t cx = (I.xb + kk - 1u) & (G.gx - 1u);
+        const uint32_t cy = (I.y + (r % 3u) - 1u) & (G.gy - 1u);
+        const uint32_t cz = (I.z + (r / 3u) - 1u) & (G.gz - 1u);
+        const uint32_t key = (cz * G.gy + cy) * G.gx + cx;
+        const uint32_t gs = offset[key];
+        cnt = offset[key + 1] - gs;
+        S.gstart[e] = gs;
+    }
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(cnt, S.wsum, total);
+    if (e < I.ncell) S.tstart[e] = ex;
+    if (e == 0) S.tstart[I.ncell] = total;
+    __syncthreads();
+    const uint32_t base = 4u * (I.W + 2u);
+    I.total = total;
+    I.own_t0 = S.tstart[base + 1];
+    I.own_g0 = S.gstart[base + 1];
+    I.n_own = S.tstart[base + I.W + 1] - I.own_t0;
+    I.ok = total <= kTileIndexLimit;
+    I.staged = I.ok && total <= tile_cap;
+    if (threadIdx.x == 0) {
+        atomicMax(&ctr[CTR_MAXTILE], total);
+        if (!I.ok) atomicAdd(&ctr[CTR_OVERFLOW], 1u);
+        else if (!I.staged) atomicAdd(&ctr[CTR_SLOWUNITS], 1u);
+    }
+    return I;
+}
+
+// tile index -> sorted slot (largest cell e with tstart[e] <= t)
+__device__ __forceinline__ uint32_t tile_to_global(const UnitShared& S, uint32_t ncell, uint32_t t) {
+    uint32_t lo = 0, hi = ncell - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (S.tstart[mid] <= t) lo = mid;
+        else hi = mid - 1u;
+    }
+    return S.gstart[lo] + (t - S.tstart[lo]);
+}
+
+// cell column (1..W) of an own particle's tile index
+__device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, uint32_t t) {
+    const uint32_t base = 4u * (W + 2u);
+    uint32_t lo = base + 1u, hi = base + W;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (S.tstart[mid] <= t) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo - base;
+}
+
+__device__ __forceinline__ F3 f3of(float4 f) { return mk3(f.x, f.y, f.z); }
+
+// ------------------------------------------------- K6: neighbour lists + lambda_0
+// detectNeighborsCUDA (pbf_neighbor_search_util.cu:361-482): per particle the
+// 27 cells in z,y,x order, slots ascending, keep r2 < h^2 while fewer than
+// cap are kept; the list is frozen for all Jacobi iterations.  Entries are
+// tile indices of the unit's neighbourhood.  Then calcScalingFactorCUDA
+// (memo form, pbf_constraint.cu:168-209) for iteration 0.
+__global__ __launch_bounds__(kBlock) void k_ns_lambda(Grid G, Phys P, Bufs b, uint32_t tile_cap,
+                                                      const float* __restrict__ Pc) {
+    __shared__ UnitShared S;
+    extern __shared__ float4 smem[];
+    float4* tile = smem;
+    uint16_t* lst = reinterpret_cast<uint16_t*>(smem + tile_cap);
+    const uint32_t cap = b.cap;
+    const uint32_t nU = b.ctr[CTR_UNITS];
+    uint32_t u0, u1, us;
+    unit_range(nU, u0, u1, us);
+    for (uint32_t u = u0; u < u1; u += us) {
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        if (I.ok) {
+            if (I.staged)
+                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
+                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                    tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], 0.f);
+                }
+            __syncthreads();
+            uint16_t* my = lst + threadIdx.x * P.stride;
+            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
+                const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
+                const uint32_t kc = own_column(S, I.W, ts);
+                auto pos = [&](uint32_t t) -> F3 {
+                    if (I.staged) return f3of(tile[t]);
+                    return ld3(Pc, cap, tile_to_global(S, I.ncell, t));
+                };
+                const F3 self = pos(ts);
+                int cnt = 0;
+                bool trunc = false;
+                for (uint32_t r = 0; r < 9u && !trunc; ++r) {
+                    const uint32_t e0 = r * (I.W + 2u) + kc - 1u;
+                    const uint32_t lo = S.tstart[e0], hi = S.tstart[e0 + 3u];
+                    for (uint32_t t = lo; t < hi; ++t) {
+                        const F3 d = self - pos(t);
+                        const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+                        if (r2 < P.h2) {
+                            if (cnt < P.cap) {
+                                my[cnt++] = (uint16_t)t;
+                            } else {
+                                trunc = true;
+                                break;
+                            }
+                        }
+                    }
+                }
+                float ks = 0.f, kg2s = 0.f;
+                F3 kgs = mk3(0.f, 0.f, 0.f);
+                for (int c = 0; c < cnt; ++c) {
+                    float W;
+                    F3 kg;
+                    pair_terms(self, pos(my[c]), P, W, kg);
+                    const float kg2 = fdot(kg, kg);
+                    ks += W;
+                    acc(kgs, kg);
+                    kg2s += kg2;
+                }
+                const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
+                const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
+                b.lam[g] = Cc / __builtin_fmaf(P.m2r2, Gs, P.relax);
+                my[P.stride - 1] = (uint16_t)cnt;
+                uint4* dst = reinterpret_cast<uint4*>(b.list + (size_t)g * P.stride);
+                const uint4* src = reinterpret_cast<const uint4*>(my);
+                for (int c = 0; c < P.stride / 8; ++c) dst[c] = src[c];
+                if (trunc) atomicAdd(&b.ctr[CTR_TRUNC], 1u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------- K8: lambda of iterations >= 1
+__global__ __launch_bounds__(kBlock) void k_lambda(Grid G, Phys P, Bufs b, uint32_t tile_cap,
+                                                   const float* __restrict__ Pc) {
+    __shared__ UnitShared S;
+    extern __shared__ float4 smem[];
+    float4* tile = smem;
+    const uint32_t cap = b.cap;
+    const uint32_t nU = b.ctr[CTR_UNITS];
+    uint32_t u0, u1, us;
+    unit_range(nU, u0, u1, us);
+    for (uint32_t u = u0; u < u1; u += us) {
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        if (I.ok) {
+            if (I.staged)
+                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
+                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                    tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], 0.f);
+                }
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
+                const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
+                auto pos = [&](uint32_t t) -> F3 {
+                    if (I.staged) return f3of(tile[t]);
+                    return ld3(Pc, cap, tile_to_global(S, I.ncell, t));
+                };
+                const F3 self = pos(ts);
+                const uint16_t* row = b.list + (size_t)g * P.stride;
+                const int cnt = row[P.stride - 1];
+                float ks = 0.f, kg2s = 0.f;
+                F3 kgs = mk3(0.f, 0.f, 0.f);
+                for (int c = 0; c < cnt; ++c) {
+                    float W;
+                    F3 kg;
+                    pair_terms(self, pos(row[c]), P, W, kg);
+                    const float kg2 = fdot(kg, kg);
+                    ks += W;
+                    acc(kgs, kg);
+                    kg2s += kg2;
+                }
+                const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
+                const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
+                b.lam[g] = Cc / __builtin_fmaf(P.m2r2, Gs, P.relax);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------- K7: delta-p + collision + apply
+// calcPositionUpdateCUDA (memo form, pbf_position_update.cu:131-188),
+// responseCollisionCUDA (pbf_response_collision.cu:12-66) and
+// updateInterimPositionCUDA (pbf_constraint.cu:335-345), written to the
+// other position buffer.
+__global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, Bufs b, uint32_t tile_cap,
+                                                   const float* __restrict__ Pc, float* __restrict__ Pn) {
+    __shared__ UnitShared S;
+    extern __shared__ float4 smem[];
+    float4* tile = smem;
+    const uint32_t cap = b.cap;
+    const uint32_t nU = b.ctr[CTR_UNITS];
+    uint32_t u0, u1, us;
+    unit_range(nU, u0, u1, us);
+    uint32_t capped_count = 0, max_sw = 0;
+    for (uint32_t u = u0; u < u1; u += us) {
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        if (I.ok) {
+            if (I.staged)
+                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
+                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                    tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], b.lam[g]);
+                }
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
+                const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
+                auto elem = [&](uint32_t t) -> float4 {
+                    if (I.staged) return tile[t];
+                    const uint32_t gg = tile_to_global(S, I.ncell, t);
+                    return make_float4(Pc[gg], Pc[cap + gg], Pc[2 * cap + gg], b.lam[gg]);
+                };
+                const float4 se = elem(ts);
+                const F3 self = f3of(se);
+                const float si = se.w;
+                const float self_cl = (si < 0.f) ? -si : 0.f;
+                const uint16_t* row = b.list + (size_t)g * P.stride;
+                const int cnt = row[P.stride - 1];
+                F3 sum = mk3(0.f, 0.f, 0.f);
+                for (int c = 0; c < cnt; ++c) {
+                    const float4 pe = elem(row[c]);
+                    float W;
+                    F3 kg;
+                    pair_terms(self, f3of(pe), P, W, kg);
+                    const float sj = pe.w;
+                    float corr;
+                    if (P.mode == 0) {
+                        const float kr2 = W * W;
+                        const float kr4 = kr2 * kr2;
+                        const float tc = 10.f * kr4 * P.inv_k;
+                        const float sup = (sj < 0.f) ? -sj : 0.f;
+                        const float cl = self_cl + sup;
+                        corr = cl * tc;
+                    } else {
+                        corr = P.sk * powi(W * P.inv_wdq, P.sn);
+                    }
+                    const float a = -(si + sj + corr);
+                    acc(sum, mulsv(a, kg));
+                }
+                F3 dp = mk3(sum.x * P.m * P.inv_rho0, sum.y * P.m * P.inv_rho0, sum.z * P.m * P.inv_rho0);
+                bool capped = false;
+                const int sw = collide(P, B, dp, self, capped);
+                capped_count += capped ? 1u : 0u;
+                max_sw = max(max_sw, (uint32_t)sw);
+                st3(Pn, cap, g, self + dp);
+            }
+        }
+        __syncthreads();
+    }
+    if (capped_count) atomicAdd(&b.ctr[CTR_SWEEPCAP], capped_count);
+    if (max_sw) atomicMax(&b.ctr[CTR_MAXSWEEP], max_sw);
+}
+
+// Sum over the 27 cells around the particle's CURRENT cell, cell ranges from
+// the search (calcContribution, pbf/interaction/cuda/detail/pbf_contribution.inl:7-57),
+// per-cell partial sums added in z,y,x order.  Fast path: the current cell is
+// one of the unit's own cells, so all 27 cells are in the staged tile.
+template <class PairF, class GlobF>
+__device__ __forceinline__ F3 sum27(const Grid& G, const UnitInfo& I, const UnitShared& S,
+                                    const uint32_t* __restrict__ offset, F3 self, PairF&& pair_tile,
+                                    GlobF&& pair_glob, uint32_t* __restrict__ ctr) {
+    int gp[3];
+    grid_pos(self, G.cw, gp);
+    const uint32_t cx = (uint32_t)gp[0] & (G.gx - 1u);
+    const uint32_t cy = (uint32_t)gp[1] & (G.gy - 1u);
+    const uint32_t cz = (uint32_t)gp[2] & (G.gz - 1u);
+    F3 sum = mk3(0.f, 0.f, 0.f);
+    if (I.staged && cy == I.y && cz == I.z && (cx - I.xb) < I.W) {
+        const uint32_t kc = cx - I.xb + 1u;
+        for (uint32_t r = 0; r < 9u; ++r)
+            for (uint32_t dx = 0; dx < 3u; ++dx) {
+                const uint32_t e = r * (I.W + 2u) + kc - 1u + dx;
+                F3 cs = mk3(0.f, 0.f, 0.f);
+                for (uint32_t t = S.tstart[e]; t < S.tstart[e + 1]; ++t) acc(cs, pair_tile(t));
+                acc(sum, cs);
+            }
+    } else {
+        atomicAdd(&ctr[CTR_SLOWPART], 1u);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint32_t c = grid_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, G);
+                    F3 cs = mk3(0.f, 0.f, 0.f);
+                    for (uint32_t j = offset[c]; j < offset[c + 1]; ++j) acc(cs, pair_glob(j));
+                    acc(sum, cs);
+                }
+    }
+    return sum;
+}
+
+// vorticity pair term (pbf_vorticity_confinement.cu:15-31)
+__device__ __forceinline__ F3 vort_pair(F3 self, F3 selfv, F3 pj, F3 vj, float warg) {
+    const F3 pd = pj - self;
+    const float r = glm_length(pd);
+    const F3 vd = vj - selfv;
+    F3 w = mk3(0.f, 0.f, 0.f);
+    if (r > 0.f) w = mulvs(glm_cross(vd, divvs(pd, r)), wspiky_deriv(r, warg));
+    return w;
+}
+
+// ------------------------------------------------- K9: velocity + vorticity
+// updateVelocityCUDA (pbf_update.cu:25-39): v = inv_dt*(x* - x_sorted), then
+// calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
+// spiky derivative (or h, when reproducing the reference's argument).
+__global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_t tile_cap,
+                                                 const float* __restrict__ Pc) {
+    __shared__ UnitShared S;
+    extern __shared__ float4 smem[];
+    float4* tp = smem;
+    float4* tv = smem + tile_cap;
+    const uint32_t cap = b.cap;
+    const uint32_t nU = b.ctr[CTR_UNITS];
+    uint32_t u0, u1, us;
+    unit_range(nU, u0, u1, us);
+    auto vel = [&](uint32_t g, F3 p) -> F3 { return mulsv(P.inv_dt, p - ld3(b.X, cap, g)); };
+    for (uint32_t u = u0; u < u1; u += us) {
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        if (I.ok) {
+            if (I.staged)
+                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
+                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                    const F3 p = ld3(Pc, cap, g);
+                    const F3 vv = vel(g, p);
+                    tp[t] = make_float4(p.x, p.y, p.z, 0.f);
+                    tv[t] = make_float4(vv.x, vv.y, vv.z, 0.f);
+                }
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
+                const uint32_t g = I.own_g0 + q;
+                const F3 self = ld3(Pc, cap, g);
+                const F3 selfv = vel(g, self);
+                const F3 w = sum27(
+                    G, I, S, b.offset, self,
+                    [&](uint32_t t) { return vort_pair(self, selfv, f3of(tp[t]), f3of(tv[t]), P.warg); },
+                    [&](uint32_t j) {
+                        const F3 pj = ld3(Pc, cap, j);
+                        return vort_pair(self, selfv, pj, vel(j, pj), P.warg);
+                    },
+                    b.ctr);
+                st3(b.OM, cap, g, mulsv(P.m * P.inv_rho0, w));
+                st3(b.V, cap, g, selfv);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// confinement location-vector pair term (pbf_vorticity_confinement.cu:121-138)
+__device__ __forceinline__ F3 loc_pair(F3 self, F3 pj, float wj, float warg) {
+    const F3 pd = pj - self;
+    const float r = glm_length(pd);
+    F3 l = mk3(0.f, 0.f, 0.f);
+    if (r > 0.f) l = mulvs(mulsv(wj, divvs(pd, r)), wspiky_deriv(r, warg));
+    return l;
+}
+
+// ------------------------------------------------- K10: vorticity confinement
+// addVorticityConfinementCUDA (pbf_vorticity_confinement.cu:192-222), in place on V.
+__global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint32_t tile_cap,
+                                                    const float* __restrict__ Pc) {
+    __shared__ UnitShared S;
+    extern __shared__ float4 smem[];
+    float4* tp = smem;
+    const uint32_t cap = b.cap;
+    const uint32_t nU = b.ctr[CTR_UNITS];
+    uint32_t u0, u1, us;
+    unit_range(nU, u0, u1, us);
+    for (uint32_t u = u0; u < u1; u += us) {
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        if (I.ok) {
+            if (I.staged)
+                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
+                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                    const F3 p = ld3(Pc, cap, g);
+                    tp[t] = make_float4(p.x, p.y, p.z, glm_length(ld3(b.OM, cap, g)));
+                }
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
+                const uint32_t g = I.own_g0 + q;
+                const F3 self = ld3(Pc, cap, g);
+                const F3 selfw = ld3(b.OM, cap, g);
+                F3 N = sum27(
+                    G, I, S, b.offset, self,
+                    [&](uint32_t t) {
+                        const float4 e = tp[t];
+                        return loc_pair(self, f3of(e), e.w, P.warg);
+                    },
+                    [&](uint32_t j) { return loc_pair(self, ld3(Pc, cap, j), glm_length(ld3(b.OM, cap, j)), P.warg); },
+                    b.ctr);
+                if (glm_length(N) > 0.f) {
+                    N = mulvs(mulvs(N, P.m), P.inv_rho0);
+                    N = divvs(N, glm_length(N));
+                    const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, selfw)), P.inv_rho0), P.dtc);
+                    st3(b.V, cap, g, ld3(b.V, cap, g) + add);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// XSPH pair term (pbf_xsph.cu:15-29)
+__device__ __forceinline__ F3 xsph_pair(F3 self, F3 selfv, F3 pj, F3 vj, float inv_h) {
+    const F3 pd = pj - self;
+    const float r = glm_length(pd);
+    const F3 vd = vj - selfv;
+    return mulvs(vd, wpoly6(r, inv_h));
+}
+
+// ------------------------------------------------- K11: XSPH + write-back
+// calcXSPHCUDA (pbf_xsph.cu:84-113) and updatePosition (pbf_update.cu:16-23):
+// x_out = x*, v_out = v + c*m*inv_rho0*sum unless the sum is NaN.
+// Vs: velocities after confinement, or null to derive them from x* and x.
+__global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uint32_t tile_cap,
+                                                     const float* __restrict__ Pc, const float* __restrict__ Vs,
+                                                     float* __restrict__ xo, float* __restrict__ vo) {
+    __shared__ UnitShared S;
+    extern __shared__ float4 smem[];
+    float4* tp = smem;
+    float4* tv = smem + tile_cap;
+    const uint32_t cap = b.cap;
+    const uint32_t nU = b.ctr[CTR_UNITS];
+    uint32_t u0, u1, us;
+    unit_range(nU, u0, u1, us);
+    auto vel = [&](uint32_t g, F3 p) -> F3 {
+        if (Vs) return ld3(Vs, cap, g);
+        return mulsv(P.inv_dt, p - ld3(b.X, cap, g));
+    };
+    uint32_t nan_count = 0;
+    for (uint32_t u = u0; u < u1; u += us) {
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        if (I.ok) {
+            if (I.staged)
+                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
+                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                    const F3 p = ld3(Pc, cap, g);
+                    const F3 vv = vel(g, p);
+                    tp[t] = make_float4(p.x, p.y, p.z, 0.f);
+                    tv[t] = make_float4(vv.x, vv.y, vv.z, 0.f);
+                }
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
+                const uint32_t g = I.own_g0 + q;
+                const F3 self = ld3(Pc, cap, g);
+                const F3 selfv = vel(g, self);
+                const F3 a = sum27(
+                    G, I, S, b.offset, self,
+                    [&](uint32_t t) { return xsph_pair(self, selfv, f3of(tp[t]), f3of(tv[t]), P.inv_h); },
+                    [&](uint32_t j) {
+                        const F3 pj = ld3(Pc, cap, j);
+                        return xsph_pair(self, selfv, pj, vel(j, pj), P.inv_h);
+                    },
+                    b.ctr);
+                F3 out;
+                if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
+                    out = selfv;
+                    ++nan_count;
+                } else {
+                    out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
+                }
+                xo[3 * g + 0] = self.x;
+                xo[3 * g + 1] = self.y;
+                xo[3 * g + 2] = self.z;
+                vo[3 * g + 0] = out.x;
+                vo[3 * g + 1] = out.y;
+                vo[3 * g + 2] = out.z;
+            }
+        }
+        __syncthreads();
+    }
+    if (nan_count) atomicAdd(&b.ctr[CTR_XSPHNAN], nan_count);
+}
+
+// ------------------------------------------------- K12: write-back without XSPH
+// pbf_solve.cpp:125-145 as shipped: v = inv_dt*(x* - x_sorted), x = x*
+// (or v = the confined velocity when vorticity ran without XSPH).
+__global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float* __restrict__ Pc,
+                                             const float* __restrict__ Vs, uint32_t C, float* __restrict__ xo,
+                                             float* __restrict__ vo) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nl = b.offset[C];
+    if (s >= nl) return;
+    const uint32_t cap = b.cap;
+    const F3 p = ld3(Pc, cap, s);
+    const F3 vv = Vs ? ld3(Vs, cap, s) : mulsv(P.inv_dt, p - ld3(b.X, cap, s));
+    xo[3 * s + 0] = p.x;
+    xo[3 * s + 1] = p.y;
+    xo[3 * s + 2] = p.z;
+    vo[3 * s + 0] = vv.x;
+    vo[3 * s + 1] = vv.y;
+    vo[3 * s + 2] = vv.z;
+}
+
+// ------------------------------------------------- views
+__global__ __launch_bounds__(kBlock) void k_neighbors_view(Grid G, Phys P, Bufs b, uint32_t* __restrict__ out,
+                                                           uint32_t* __restrict__ counts, uint32_t max_pair) {
+    __shared__ UnitShared S;
+    const uint32_t nU = b.ctr[CTR_UNITS];
+    uint32_t u0, u1, us;
+    unit_range(nU, u0, u1, us);
+    for (uint32_t u = u0; u < u1; u += us) {
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], 0u, S, b.ctr);
+        if (I.ok)
+            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
+                const uint32_t g = I.own_g0 + q;
+                const uint16_t* row = b.list + (size_t)g * P.stride;
+                const uint32_t cnt = row[P.stride - 1];
+                counts[g] = cnt;
+                for (uint32_t c = 0; c < max_pair; ++c)
+                    out[(size_t)g * max_pair + c] = c < cnt ? tile_to_global(S, I.ncell, row[c]) : 0xFFFFFFFFu;
+            }
+        __syncthreads();
+    }
+}
+
+__global__ void k_cell_ranges(const uint32_t* __restrict__ offset, uint32_t C, uint32_t* __restrict__ start,
+                              uint32_t* __restrict__ end) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t a = offset[c], z = offset[c + 1];
+    start[c] = (z > a) ? a : 0xFFFFFFFFu;
+    end[c] = (z > a) ? z : 0u;
+}
+
+__global__ void k_copy_lambda(const uint32_t* __restrict__ offset, uint32_t C, const float* __restrict__ lam,
+                              float* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= offset[C]) return;
+    out[s] = lam[s];
+}
+
+__global__ void k_selftest(const float* __restrict__ a, const float* __restrict__ bb, float* __restrict__ out,
+                           uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = a[i], y = bb[i];
+    out[4 * i + 0] = x / y;
+    out[4 * i + 1] = __builtin_sqrtf(x);
+    out[4 * i + 2] = __builtin_fmaf(x, y, x);
+    out[4 * i + 3] = floorf(x / y);
+}
+
+inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+size_t smem_ns(const Tuning& T, const Phys& P) {
+    return (size_t)T.tile_cap * sizeof(float4) + (size_t)kBlock * P.stride * sizeof(uint16_t);
+}
+
+}  // namespace
+
+int unit_blocks_per_cu(uint32_t tile_cap) {
+    // Allow the largest tile on every unit kernel, then ask the runtime how
+    // many workgroups of the current tile size fit a CU; the two-tile
+    // kernels bound the occupancy.
+    int best = 1 << 30;
+    struct K {
+        const void* f;
+        size_t smem_max, smem;
+    } ks[] = {
+        {(const void*)k_ns_lambda, (size_t)kTileCapMax * 16 + (size_t)kBlock * 64 * 2,
+         (size_t)tile_cap * 16 + (size_t)kBlock * 64 * 2},
+        {(const void*)k_lambda, (size_t)kTileCapMax * 16, (size_t)tile_cap * 16},
+        {(const void*)k_deltap, (size_t)kTileCapMax * 16, (size_t)tile_cap * 16},
+        {(const void*)k_vort, (size_t)kTileCapMax * 32, (size_t)tile_cap * 32},
+        {(const void*)k_confine, (size_t)kTileCapMax * 16, (size_t)tile_cap * 16},
+        {(const void*)k_xsph_out, (size_t)kTileCapMax * 32, (size_t)tile_cap * 32},
+    };
+    for (auto& k : ks) {
+        (void)hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.smem_max);
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.f, kBlock, k.smem) != hipSuccess || nb < 1) nb = 1;
+        best = nb < best ? nb : best;
+    }
+    return best;
+}
+
+hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bufs& b, const Tuning& T, float* x,
+                       float* v, uint32_t n, const float lo[3], const float hi[3], int iters, int xsph_on,
+                       int vort_on, hipStream_t s) {
+    const uint32_t nb = ceil_div(n > 0 ? n : 1, 256);
+    const uint32_t GU = T.grid_units;
+    const uint32_t uw = T.unit_width < G.gx ? T.unit_width : G.gx;
+    const uint32_t budget = (T.tile_cap * 3u) / 4u > 0 ? (T.tile_cap * 3u) / 4u : 1u;
+    if (n > 0)
+        hipLaunchKernelGGL(k_predict_hash, dim3(nb), dim3(256), 0, s, x, v, n, G, P.dt, P.g[0], P.g[1], P.g[2], lo[0],
+                           lo[1], lo[2], hi[0], hi[1], hi[2], b.key, b.rank, b.count, b.tsum, b.lgT);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.tsum, b.tprefix, b.nT + 1, b.ctr);
+    hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.offset, b.tsum, b.tprefix, b.nT,
+                       b.lgT, G.C);
+    if (n == 0) return hipGetLastError();
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, b.key, b.rank, b.offset, b.tmp, n);
+    hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, x, v, G, P.dt, P.g[0], P.g[1], P.g[2], b.key, b.tmp,
+                       b.offset, b.sidx, b.PA, b.X, b.cap, n, uw, budget, b.units, b.ctr);
+    const size_t s1 = (size_t)T.tile_cap * sizeof(float4);
+    hipLaunchKernelGGL(k_ns_lambda, dim3(GU), dim3(kBlock), smem_ns(T, P), s, G, P, b, T.tile_cap,
+                       (const float*)b.PA);
+    float* cur = b.PA;
+    float* nxt = b.PB;
+    for (int it = 0; it < iters; ++it) {
+        if (it > 0)
+            hipLaunchKernelGGL(k_lambda, dim3(GU), dim3(kBlock), s1, s, G, P, b, T.tile_cap, (const float*)cur);
+        hipLaunchKernelGGL(k_deltap, dim3(GU), dim3(kBlock), s1, s, G, P, B, b, T.tile_cap, (const float*)cur, nxt);
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    if (vort_on) {
+        hipLaunchKernelGGL(k_vort, dim3(GU), dim3(kBlock), 2 * s1, s, G, P, b, T.tile_cap, (const float*)cur);
+        hipLaunchKernelGGL(k_confine, dim3(GU), dim3(kBlock), s1, s, G, P, b, T.tile_cap, (const float*)cur);
+    }
+    const float* vs = vort_on ? b.V : nullptr;
+    if (xsph_on)
+        hipLaunchKernelGGL(k_xsph_out, dim3(GU), dim3(kBlock), 2 * s1, s, G, P, b, T.tile_cap, (const float*)cur, vs,
+                           x, v);
+    else
+        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, vs, G.C, x, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, const Tuning& T, uint32_t* out,
+                                 uint32_t* counts, uint32_t max_pair, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbors_view, dim3(T.grid_units), dim3(kBlock), 0, s, G, P, b, out, counts, max_pair);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uint32_t* end, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_ranges, dim3(ceil_div(G.C, 256)), dim3(256), 0, s, b.offset, G.C, start, end);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_lambda, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b.offset, G.C,
+                       b.lam, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_selftest, dim3(ceil_div(n, 256)), dim3(256), 0, s, a, bb, out, n);
+    return hipGetLastError();
+}
+
+}  // namespace pbfx

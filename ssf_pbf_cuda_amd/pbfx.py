@@ -1,0 +1,312 @@
+"""ctypes binding of libpbfx.so, the HIP PBF solver step (include/pbfx.h).
+
+Device buffers are passed as raw device pointers; the helpers accept torch
+tensors on a ROCm device (torch is only used for device memory and streams).
+Loading fails loudly when the HIP library has not been built: there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .scenes import Scene, StepParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpbfx.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "pbfx.h")
+
+PBFX_OK = 0
+STATUS = {
+    0: "ok",
+    1: "invalid argument",
+    2: "capacity",
+    3: "grid too large",
+    4: "HIP error",
+    5: "not converged",
+    6: "neighbour list truncated",
+    7: "tile overflow",
+    8: "not supported",
+}
+
+
+class PbfxError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: pbfx status {status} ({STATUS.get(status, '?')})")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("stable_density", ctypes.c_float),
+        ("stable_distance", ctypes.c_float),
+        ("particle_mass", ctypes.c_float),
+        ("smoothing_length", ctypes.c_float),
+        ("time_step", ctypes.c_float),
+        ("relaxation", ctypes.c_float),
+        ("xsph_parameter", ctypes.c_float),
+        ("vc_parameter", ctypes.c_float),
+        ("body_force", ctypes.c_float * 3),
+        ("scorr_mode", ctypes.c_int32),
+        ("scorr_k", ctypes.c_float),
+        ("scorr_n", ctypes.c_int32),
+        ("scorr_dq_over_h", ctypes.c_float),
+        ("max_pair", ctypes.c_int32),
+        ("xsph_on", ctypes.c_int32),
+        ("vort_on", ctypes.c_int32),
+        ("vort_h_as_inv_h", ctypes.c_int32),
+        ("outer_as_inner", ctypes.c_int32),
+        ("max_sweeps", ctypes.c_int32),
+    ]
+
+
+class NsView(ctypes.Structure):
+    _fields_ = [
+        ("grid", ctypes.c_uint32 * 3),
+        ("num_cells", ctypes.c_uint32),
+        ("cell_width", ctypes.c_float),
+        ("max_pair", ctypes.c_uint32),
+        ("sorted_index", ctypes.c_void_p),
+        ("cell_offset", ctypes.c_void_p),
+        ("hash", ctypes.c_void_p),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "num_in", "num_live", "truncated", "sweep_cap_hits", "max_sweeps_used", "xsph_nan", "units", "slow_units",
+        "slow_particles", "tile_overflow", "max_tile", "steps")]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libpbfx.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, f32p = ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        sig = {
+            "pbfx_params_default": ([ctypes.POINTER(Params)], None),
+            "pbfx_create": ([ctypes.POINTER(Params), u32, f32p, ctypes.POINTER(vp)], ctypes.c_int),
+            "pbfx_destroy": ([vp], None),
+            "pbfx_set_params": ([vp, ctypes.POINTER(Params)], ctypes.c_int),
+            "pbfx_set_boundary": ([vp, f32p, u32, f32p, u32, f32p, f32p, u32], ctypes.c_int),
+            "pbfx_step": ([vp, vp, vp, u32p, f32p, f32p, ctypes.c_int, vp], ctypes.c_int),
+            "pbfx_step_async": ([vp, vp, vp, u32, f32p, f32p, ctypes.c_int, vp], ctypes.c_int),
+            "pbfx_step_no_boundary": ([vp, vp, vp, u32p, f32p, f32p, ctypes.c_int, vp], ctypes.c_int),
+            "pbfx_get_num": ([vp, vp, u32p], ctypes.c_int),
+            "pbfx_get_stats": ([vp, vp, ctypes.POINTER(Stats)], ctypes.c_int),
+            "pbfx_ns_view_get": ([vp, ctypes.POINTER(NsView)], ctypes.c_int),
+            "pbfx_ns_neighbors": ([vp, vp, vp, vp], ctypes.c_int),
+            "pbfx_ns_cell_ranges": ([vp, vp, vp, vp], ctypes.c_int),
+            "pbfx_scaling_factor": ([vp, vp, vp], ctypes.c_int),
+            "pbfx_set_tuning": ([vp, u32, u32], ctypes.c_int),
+            "pbfx_set_report_info": ([vp, ctypes.c_int], None),
+            "pbfx_get_grid": ([vp, u32p], ctypes.c_int),
+            "pbfx_calc_grid_size": ([ctypes.c_double, f32p, u32p], ctypes.c_int),
+            "pbfx_last_hip_error": ([vp], ctypes.c_int),
+            "pbfx_status_string": ([ctypes.c_int], ctypes.c_char_p),
+            "pbfx_selftest_arith": ([vp, vp, vp, u32, vp], ctypes.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def header_functions() -> list:
+    """Names of the functions include/pbfx.h declares."""
+    import re
+
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"\b(pbfx_[a-z_]+)\s*\(", text)))
+
+
+def _f3(a) -> ctypes.Array:
+    arr = (ctypes.c_float * 3)()
+    arr[:] = [float(x) for x in a]
+    return arr
+
+
+def _check(st: int, what: str) -> None:
+    if st != PBFX_OK:
+        raise PbfxError(st, what)
+
+
+def make_params(p: StepParams) -> Params:
+    o = Params()
+    for name in ("stable_density", "stable_distance", "particle_mass", "smoothing_length", "time_step", "relaxation",
+                 "xsph_parameter", "vc_parameter", "scorr_k", "scorr_dq_over_h"):
+        setattr(o, name, float(getattr(p, name)))
+    for name in ("scorr_mode", "scorr_n", "max_pair", "xsph_on", "vort_on", "vort_h_as_inv_h", "outer_as_inner",
+                 "max_sweeps"):
+        setattr(o, name, int(getattr(p, name)))
+    o.body_force[:] = [float(c) for c in p.body_force]
+    return o
+
+
+def calc_grid_size(cell_size: float, end: Sequence[float]):
+    g = (ctypes.c_uint32 * 3)()
+    st = lib().pbfx_calc_grid_size(float(cell_size), _f3(end), g)
+    return None if st else tuple(g)
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return 0
+    if isinstance(t, int):
+        return t
+    return int(t.data_ptr())
+
+
+def _stream(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+class Solver:
+    """One pbfx context (the reference's pbf_particle + pbf_buffer + neighbor_search)."""
+
+    def __init__(self, params: StepParams, capacity: int, domain_end: Sequence[float]):
+        self._lib = lib()
+        self.params = params
+        self.capacity = int(capacity)
+        self._ctx = ctypes.c_void_p()
+        _check(self._lib.pbfx_create(ctypes.byref(make_params(params)), self.capacity, _f3(domain_end),
+                                     ctypes.byref(self._ctx)), "pbfx_create")
+
+    @classmethod
+    def for_scene(cls, scene: Scene, capacity: Optional[int] = None, boundary: bool = True) -> "Solver":
+        s = cls(scene.params, capacity or scene.n, scene.box_hi)
+        if boundary:
+            s.set_boundary(scene.inner_spheres, scene.outer_spheres, scene.plane_points, scene.plane_normals)
+        return s
+
+    def close(self) -> None:
+        if self._ctx:
+            self._lib.pbfx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: StepParams) -> None:
+        _check(self._lib.pbfx_set_params(self._ctx, ctypes.byref(make_params(params))), "pbfx_set_params")
+        self.params = params
+
+    def set_boundary(self, inner=None, outer=None, plane_points=None, plane_normals=None) -> None:
+        def arr(a, w):
+            if a is None:
+                return np.zeros((0, w), np.float32)
+            return np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, w))
+
+        ai, ao, pp, pn = arr(inner, 4), arr(outer, 4), arr(plane_points, 3), arr(plane_normals, 3)
+        fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+        self._keep = (ai, ao, pp, pn)
+        _check(self._lib.pbfx_set_boundary(self._ctx, fp(ai), ai.shape[0], fp(ao), ao.shape[0], fp(pp), fp(pn),
+                                           pp.shape[0]), "pbfx_set_boundary")
+
+    def set_tuning(self, tile_cap: int = 2048, unit_width: int = 16) -> None:
+        _check(self._lib.pbfx_set_tuning(self._ctx, int(tile_cap), int(unit_width)), "pbfx_set_tuning")
+
+    def set_report_info(self, enable: bool) -> None:
+        self._lib.pbfx_set_report_info(self._ctx, int(bool(enable)))
+
+    def grid(self):
+        g = (ctypes.c_uint32 * 3)()
+        _check(self._lib.pbfx_get_grid(self._ctx, g), "pbfx_get_grid")
+        return tuple(g)
+
+    def step(self, x, v, num: int, lo, hi, iters: int, stream=None, boundary: bool = True) -> int:
+        """pbf::one_step: returns the live count after deletion (synchronises)."""
+        n = ctypes.c_uint32(int(num))
+        fn = self._lib.pbfx_step if boundary else self._lib.pbfx_step_no_boundary
+        st = fn(self._ctx, _ptr(x), _ptr(v), ctypes.byref(n), _f3(lo), _f3(hi), int(iters), _stream(stream))
+        _check(st, "pbfx_step")
+        return int(n.value)
+
+    def step_status(self, x, v, num: int, lo, hi, iters: int, stream=None):
+        n = ctypes.c_uint32(int(num))
+        st = self._lib.pbfx_step(self._ctx, _ptr(x), _ptr(v), ctypes.byref(n), _f3(lo), _f3(hi), int(iters),
+                                 _stream(stream))
+        return int(st), int(n.value)
+
+    def step_async(self, x, v, num: int, lo, hi, iters: int, stream=None) -> None:
+        _check(self._lib.pbfx_step_async(self._ctx, _ptr(x), _ptr(v), int(num), _f3(lo), _f3(hi), int(iters),
+                                         _stream(stream)), "pbfx_step_async")
+
+    def num(self, stream=None) -> int:
+        n = ctypes.c_uint32()
+        _check(self._lib.pbfx_get_num(self._ctx, _stream(stream), ctypes.byref(n)), "pbfx_get_num")
+        return int(n.value)
+
+    def stats(self, stream=None) -> dict:
+        s = Stats()
+        _check(self._lib.pbfx_get_stats(self._ctx, _stream(stream), ctypes.byref(s)), "pbfx_get_stats")
+        return {k: int(getattr(s, k)) for k, _ in Stats._fields_}
+
+    def ns_view(self) -> NsView:
+        v = NsView()
+        _check(self._lib.pbfx_ns_view_get(self._ctx, ctypes.byref(v)), "pbfx_ns_view_get")
+        return v
+
+    def ns_neighbors(self, out, counts, stream=None) -> None:
+        _check(self._lib.pbfx_ns_neighbors(self._ctx, _ptr(out), _ptr(counts), _stream(stream)), "pbfx_ns_neighbors")
+
+    def ns_cell_ranges(self, start, end, stream=None) -> None:
+        _check(self._lib.pbfx_ns_cell_ranges(self._ctx, _ptr(start), _ptr(end), _stream(stream)),
+               "pbfx_ns_cell_ranges")
+
+    def scaling_factor(self, out, stream=None) -> None:
+        _check(self._lib.pbfx_scaling_factor(self._ctx, _ptr(out), _stream(stream)), "pbfx_scaling_factor")
+
+
+def selftest_arith(a, b, out, stream=None) -> None:
+    n = int(a.numel())
+    _check(lib().pbfx_selftest_arith(_ptr(a), _ptr(b), _ptr(out), n, _stream(stream)), "pbfx_selftest_arith")
+
+
+_hip = None
+
+
+def _hiprt():
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise RuntimeError("libamdhip64.so not found")
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemcpy.restype = ctypes.c_int
+    return _hip
+
+
+def d2h(ptr: int, count: int, dtype=np.uint32) -> np.ndarray:
+    """Copies `count` elements from a device pointer owned by a context."""
+    out = np.empty(int(count), dtype=dtype)
+    if count:
+        rc = _hiprt().hipMemcpy(out.ctypes.data, ctypes.c_void_p(int(ptr)), out.nbytes, 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return out

@@ -123,10 +123,11 @@ pbfx_status pbfx_set_boundary(pbfx_ctx* ctx, const float* inner_spheres4, uint32
 pbfx_status pbfx_step(pbfx_ctx* ctx, float* x, float* v, uint32_t* num_inout, const float lo[3],
                       const float hi[3], int iters, void* stream);
 
-/* Same step without any host synchronisation: `num` is an upper bound of
- * the live count (normally the previous step's result), the new live count
- * stays on the device (pbfx_get_num reads it).  Use this form for streams of
- * steps and for graph capture. */
+/* Same step without any host synchronisation: `num` is the live count
+ * entering the step; the count after deletion stays on the device
+ * (pbfx_get_num reads it), so a caller chaining steps whose simulation area
+ * can delete particles reads it before passing the next count.  Use this
+ * form for streams of steps and for graph capture. */
 pbfx_status pbfx_step_async(pbfx_ctx* ctx, float* x, float* v, uint32_t num, const float lo[3],
                             const float hi[3], int iters, void* stream);
 
@@ -208,6 +209,21 @@ pbfx_status pbfx_calc_grid_size(double cell_size, const float domain_end[3], uin
 int pbfx_last_hip_error(pbfx_ctx* ctx);
 
 const char* pbfx_status_string(pbfx_status s);
+
+/* Per-kernel timing with HIP events recorded on the step's stream around
+ * every launch (off by default; at most 64 launches between reads, i.e. read
+ * after each step).  No counterpart in the reference, which has no timers;
+ * used by the benchmark to report each kernel's duration. */
+typedef struct pbfx_kernel_time {
+    char name[32];
+    double total_ms;           /* accumulated since timing was enabled */
+    uint32_t launches;
+} pbfx_kernel_time;
+pbfx_status pbfx_set_kernel_timing(pbfx_ctx* ctx, int enable);
+/* Synchronises `stream`, folds the recorded launches into the totals and
+ * writes up to `max` entries; *count = number of kernels with launches. */
+pbfx_status pbfx_kernel_timing_read(pbfx_ctx* ctx, void* stream, pbfx_kernel_time* out, uint32_t max,
+                                    uint32_t* count);
 
 /* Arithmetic self-test on the device: for n inputs computes
  * out[4*i+0] = a/b, out[4*i+1] = sqrtf(a), out[4*i+2] = fmaf(a,b,a),

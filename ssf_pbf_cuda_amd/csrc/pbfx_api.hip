@@ -33,6 +33,9 @@ struct pbfx_ctx {
     int report_info = 0;
     uint32_t steps = 0;
     uint32_t list_stride = 0;
+    Timer* timer = nullptr;          // per-launch events when kernel timing is on
+    double kt_ms[K_NUM] = {};
+    uint32_t kt_n[K_NUM] = {};
 };
 
 namespace {
@@ -123,11 +126,21 @@ hipError_t dalloc(T** p, size_t n) {
     return hipMalloc((void**)p, (n ? n : 1) * sizeof(T));
 }
 
+void free_timer(pbfx_ctx* c) {
+    if (!c->timer) return;
+    for (int i = 0; i < 2 * kTimerSlots; ++i) (void)hipEventDestroy(c->timer->ev[i]);
+    delete c->timer;
+    c->timer = nullptr;
+}
+
 void free_bufs(pbfx_ctx* c) {
+    free_timer(c);
     Bufs& b = c->b;
-    void* ptrs[] = {b.key, b.rank, b.tmp, b.sidx, b.PA, b.PB, b.X, b.V, b.OM, b.lam, b.list, b.count, b.offset,
-                    b.tsum, b.tprefix, b.units, b.ctr, c->d_inner, c->d_outer, c->d_pts, c->d_nrm,
-                    c->d_default_sphere};
+    void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.PA,     b.PB,    b.X,      b.V,
+                    b.OM,    b.lam,     b.list,    b.count, b.offset, b.tsum,  b.tprefix, b.units,
+                    b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
+                    b.tflag, b.ctflag,
+                    b.ctr,   c->d_inner, c->d_outer, c->d_pts, c->d_nrm, c->d_default_sphere};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
@@ -166,7 +179,7 @@ pbfx_status do_step(pbfx_ctx* c, float* x, float* v, uint32_t n, const float lo[
     Phys P = make_phys(c->params);
     if ((uint32_t)P.stride > c->list_stride) return PBFX_INVALID_ARG;
     hipError_t e = launch_step(c->G, P, B, c->b, c->T, x, v, n, lo, hi, iters, c->params.xsph_on,
-                               c->params.vort_on, s);
+                               c->params.vort_on, s, c->timer);
     if (e != hipSuccess) return hip_fail(c, e);
     c->steps++;
     return PBFX_OK;
@@ -285,7 +298,18 @@ pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const floa
     PBFX_ALLOC(b.offset, (size_t)G.C + 2);
     PBFX_ALLOC(b.tsum, (size_t)b.nT + 1);
     PBFX_ALLOC(b.tprefix, (size_t)b.nT + 1);
-    PBFX_ALLOC(b.units, cap);
+    const size_t items = cap + cap / kChunk + 64;  // a unit per own chunk of kChunk particles
+    PBFX_ALLOC(b.units, items);
+    PBFX_ALLOC(b.ckey, cap);
+    PBFX_ALLOC(b.crank, cap);
+    PBFX_ALLOC(b.cperm, cap);
+    PBFX_ALLOC(b.ccount, (size_t)G.C + 1);
+    PBFX_ALLOC(b.coffset, (size_t)G.C + 2);
+    PBFX_ALLOC(b.ctsum, (size_t)b.nT + 1);
+    PBFX_ALLOC(b.ctprefix, (size_t)b.nT + 1);
+    PBFX_ALLOC(b.cunits, items);
+    PBFX_ALLOC(b.tflag, (size_t)b.nT + 1);
+    PBFX_ALLOC(b.ctflag, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctr, CTR_NUM);
     PBFX_ALLOC(c->d_default_sphere, 1);
 #undef PBFX_ALLOC
@@ -293,6 +317,11 @@ pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const floa
     if (e == hipSuccess) e = hipMemset(b.count, 0, ((size_t)G.C + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b.offset, 0, ((size_t)G.C + 2) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b.tsum, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.ccount, 0, ((size_t)G.C + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.coffset, 0, ((size_t)G.C + 2) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.ctsum, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.tflag, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.ctflag, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b.ctr, 0, CTR_NUM * sizeof(uint32_t));
     if (e == hipSuccess) {
         const float4 sph = make_float4(3.f, 3.f, 3.f, 3.f);
@@ -509,6 +538,60 @@ const char* pbfx_status_string(pbfx_status s) {
         case PBFX_NOT_SUPPORTED: return "not supported";
     }
     return "unknown status";
+}
+
+pbfx_status pbfx_set_kernel_timing(pbfx_ctx* c, int enable) {
+    if (!c) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    (void)hipDeviceSynchronize();
+    free_timer(c);
+    std::memset(c->kt_ms, 0, sizeof(c->kt_ms));
+    std::memset(c->kt_n, 0, sizeof(c->kt_n));
+    if (!enable) return PBFX_OK;
+    Timer* t = new (std::nothrow) Timer();
+    if (!t) return PBFX_HIP_ERROR;
+    for (int i = 0; i < 2 * kTimerSlots; ++i) {
+        hipError_t e = hipEventCreate(&t->ev[i]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; ++j) (void)hipEventDestroy(t->ev[j]);
+            delete t;
+            return hip_fail(c, e);
+        }
+    }
+    t->n = 0;
+    c->timer = t;
+    return PBFX_OK;
+}
+
+pbfx_status pbfx_kernel_timing_read(pbfx_ctx* c, void* stream, pbfx_kernel_time* out, uint32_t max,
+                                    uint32_t* count) {
+    if (!c || !count || (max && !out)) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    if (c->timer) {
+        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(c, e);
+        for (int i = 0; i < c->timer->n; ++i) {
+            float ms = 0.f;
+            e = hipEventElapsedTime(&ms, c->timer->ev[2 * i], c->timer->ev[2 * i + 1]);
+            if (e != hipSuccess) return hip_fail(c, e);
+            c->kt_ms[c->timer->id[i]] += ms;
+            c->kt_n[c->timer->id[i]] += 1;
+        }
+        c->timer->n = 0;
+    }
+    uint32_t k = 0;
+    for (int id = 0; id < K_NUM; ++id) {
+        if (!c->kt_n[id]) continue;
+        if (k < max) {
+            std::memset(out[k].name, 0, sizeof(out[k].name));
+            std::strncpy(out[k].name, kernel_name(id), sizeof(out[k].name) - 1);
+            out[k].total_ms = c->kt_ms[id];
+            out[k].launches = c->kt_n[id];
+        }
+        ++k;
+    }
+    *count = k;
+    return PBFX_OK;
 }
 
 pbfx_status pbfx_selftest_arith(const float* a, const float* b, float* out, uint32_t n, void* stream) {

@@ -9,8 +9,10 @@
 //                                                 count in the row's last entry
 //   count                          u32[C+1]      cell histogram (cell C = deleted)
 //   offset                         u32[C+2]      exclusive scan: cell c owns [off[c], off[c+1])
-//   tsum, tprefix                  u32[nT+1]     per-4096-cell-tile totals and their scan
-//   units                          uint2[cap]    work units (first cell key, width in cells)
+//   tsum, tflag, tprefix           u32[nT+1]     per-4096-cell-tile totals, occupied flags, scan
+//   units                          uint2[cap+]   work items (first cell key, width | chunk << 5)
+//   ckey, crank, cperm, ccount, coffset, ctsum, ctprefix, cunits
+//                                                 the same sort keyed by each particle's final cell
 //   ctr                            u32[16]       counters of the last step
 #pragma once
 
@@ -26,6 +28,8 @@ constexpr uint32_t kMaxCells = 9 * (kMaxW + 2);
 constexpr uint32_t kTileCapMax = 4096;
 constexpr uint32_t kTileIndexLimit = 65535;  // tile indices are stored as u16
 constexpr uint32_t kScanTile = 4096;      // cells per scan tile
+constexpr uint32_t kChunk = kBlock;       // own particles per work item
+constexpr uint32_t kSparse = 64;          // final-cell runs below this go one thread per particle
 
 enum Ctr : int {
     CTR_UNITS = 0,
@@ -37,6 +41,7 @@ enum Ctr : int {
     CTR_SLOWPART,
     CTR_OVERFLOW,
     CTR_MAXTILE,
+    CTR_CUNITS,
     CTR_NUM = 16
 };
 
@@ -91,11 +96,48 @@ struct Bufs {
     uint32_t* count;
     uint32_t* offset;
     uint32_t* tsum;
+    uint32_t* tflag;
     uint32_t* tprefix;
     uint2* units;
+    // regrouping by final cell for the vorticity / confinement / XSPH passes
+    uint32_t* ckey;
+    uint32_t* crank;
+    uint32_t* cperm;
+    uint32_t* ccount;
+    uint32_t* coffset;
+    uint32_t* ctsum;
+    uint32_t* ctflag;
+    uint32_t* ctprefix;
+    uint2* cunits;
     uint32_t* ctr;
     uint32_t lgT;
     uint32_t nT;
+};
+
+// Kernels of one step, for the optional per-launch HIP-event timing.
+enum KernelId : int {
+    K_PREDICT_HASH = 0,
+    K_SCAN_TILES,
+    K_SCAN_CELLS,
+    K_SCATTER,
+    K_REORDER,
+    K_NS_LAMBDA,
+    K_LAMBDA,
+    K_DELTAP,
+    K_VORT,
+    K_CONFINE,
+    K_XSPH_OUT,
+    K_OUT,
+    K_CUR_SORT,
+    K_NUM
+};
+constexpr int kTimerSlots = 64;
+
+// Event pairs recorded on the step's stream around each launch.
+struct Timer {
+    hipEvent_t ev[2 * kTimerSlots];
+    int id[kTimerSlots];
+    int n;
 };
 
 struct Tuning {
@@ -107,7 +149,8 @@ struct Tuning {
 // Launchers (pbfx_kernels.hip).  All enqueue on `s` and never synchronise.
 hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bufs& b, const Tuning& T,
                        float* x, float* v, uint32_t n_in, const float lo[3], const float hi[3], int iters,
-                       int xsph_on, int vort_on, hipStream_t s);
+                       int xsph_on, int vort_on, hipStream_t s, Timer* timer);
+const char* kernel_name(int id);
 hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, const Tuning& T, uint32_t* out,
                                  uint32_t* counts, uint32_t max_pair, hipStream_t s);
 hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uint32_t* end, hipStream_t s);

@@ -266,18 +266,24 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t* ctr, uint32_t k, bool ac
     return mine;
 }
 
+// Marks a scan tile as occupied: a read first, so that the common case (the
+// tile is already marked) issues no store.
+__device__ __forceinline__ void mark_tile(uint32_t* __restrict__ tflag, uint32_t t) {
+    if (tflag[t] == 0u) tflag[t] = 1u;
+}
+
 // ------------------------------------------------- K1: predict + hash + histogram
 // pbf_predict.cu:15-45 (v* = fmaf(g,dt,v), x* = fmaf(v*,dt,x)), calcHashCUDA
 // (pbf_neighbor_search_util.cu:20-37), checkDeleteParticleCUDA
 // (pbf_delete.cu:9-28, sentinel key C) and the histogram of insertAndCountCUDA
-// (pbf_counting_sort.cu:21-32) with wave-aggregated atomics; the per-tile
-// totals feed the two-level scan.
+// (pbf_counting_sort.cu:21-32) with wave-aggregated atomics; each particle
+// marks its 4096-cell scan tile as occupied (a plain store, no contention).
 __global__ __launch_bounds__(256) void k_predict_hash(const float* __restrict__ x, const float* __restrict__ v,
                                                       uint32_t n, Grid G, float dt, float g0, float g1, float g2,
                                                       float lo0, float lo1, float lo2, float hi0, float hi1,
                                                       float hi2, uint32_t* __restrict__ key,
                                                       uint32_t* __restrict__ rank, uint32_t* __restrict__ count,
-                                                      uint32_t* __restrict__ tsum, uint32_t lgT) {
+                                                      uint32_t* __restrict__ tflag, uint32_t lgT) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = i < n;
     uint32_t k = 0;
@@ -295,11 +301,35 @@ __global__ __launch_bounds__(256) void k_predict_hash(const float* __restrict__ 
         if (q[0] < lo0 || hi0 < q[0] || q[1] < lo1 || hi1 < q[1] || q[2] < lo2 || hi2 < q[2]) k = G.C;
     }
     const uint32_t r = wave_rank(count, k, active);
-    wave_rank(tsum, k >> lgT, active);
     if (active) {
+        mark_tile(tflag, k >> lgT);
         key[i] = k;
         rank[i] = r;
     }
+}
+
+// ------------------------------------------------- K2a: per-tile totals
+// Totals of the cell counts of each occupied scan tile (unoccupied tiles are
+// all zero); tile nT is the deleted bucket (cell C).
+__global__ __launch_bounds__(256) void k_tile_reduce(const uint32_t* __restrict__ count,
+                                                     const uint32_t* __restrict__ tflag, uint32_t* __restrict__ tsum,
+                                                     uint32_t nT, uint32_t lgT, uint32_t C) {
+    __shared__ uint32_t wsum[kWaves];
+    const uint32_t t = blockIdx.x;
+    if (t == nT) {
+        if (threadIdx.x == 0) tsum[nT] = count[C];
+        return;
+    }
+    if (tflag[t] == 0u) {
+        if (threadIdx.x == 0) tsum[t] = 0u;
+        return;
+    }
+    const uint32_t T = 1u << lgT, c0 = t << lgT;
+    uint32_t s = 0;
+    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) s += count[c0 + j];
+    uint32_t total;
+    block_excl_scan(s, wsum, total);
+    if (threadIdx.x == 0) tsum[t] = total;
 }
 
 // ------------------------------------------------- K2: scan of the tile totals
@@ -307,7 +337,7 @@ __global__ __launch_bounds__(256) void k_predict_hash(const float* __restrict__ 
 // bucket), and reset of the step counters.
 __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict__ tsum,
                                                      uint32_t* __restrict__ tprefix, uint32_t nT1,
-                                                     uint32_t* __restrict__ ctr) {
+                                                     uint32_t* __restrict__ ctr, uint32_t reset) {
     __shared__ uint32_t wsum[16];
     const uint32_t per = (nT1 + blockDim.x - 1) / blockDim.x;
     const uint32_t b0 = threadIdx.x * per;
@@ -331,16 +361,16 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict_
             tprefix[b0 + j] = run;
             run += tsum[b0 + j];
         }
-    if (threadIdx.x < CTR_NUM) ctr[threadIdx.x] = 0;
+    if (reset && threadIdx.x < CTR_NUM) ctr[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------- K3: scan of the cell counts
 // Workgroup t scans the counts of cell tile t (empty tiles only write their
-// prefix), clears the counts it read and its tile total; the last workgroup
-// writes the deleted bucket: off[C] = live count, off[C+1] = n.
+// prefix), clears the counts it read and the tile's occupied flag; the last
+// workgroup writes the deleted bucket: off[C] = live count, off[C+1] = n.
 // (CUB DeviceScan::ExclusiveSum in pbf_counting_sort.cu:62-71.)
 __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
-                                                    uint32_t* __restrict__ tsum,
+                                                    const uint32_t* __restrict__ tsum, uint32_t* __restrict__ tflag,
                                                     const uint32_t* __restrict__ tprefix, uint32_t nT,
                                                     uint32_t lgT, uint32_t C) {
     __shared__ uint32_t wsum[kWaves];
@@ -350,7 +380,7 @@ __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count
             offset[C] = tprefix[nT];
             offset[C + 1] = tprefix[nT] + tsum[nT];
             count[C] = 0;
-            tsum[nT] = 0;
+            tflag[nT] = 0;
         }
         return;
     }
@@ -375,8 +405,7 @@ __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count
         run += c;
         count[my0 + j] = 0;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) tsum[t] = 0;
+    if (threadIdx.x == 0) tflag[t] = 0;
 }
 
 // ------------------------------------------------- K4: scatter
@@ -387,6 +416,32 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ke
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     tmp[offset[key[i]] + rank[i]] = i;
+}
+
+// Work items of one sort: the particle at the first slot f of a run of `uw`
+// cells along x (cells kb .. kb+uw-1) appends the run's items.  A run whose
+// estimated 27-cell neighbourhood (9 rows of W+2 cells at the run's mean
+// density) exceeds `budget` particles is split into narrower units, and a
+// unit with more than kChunk own particles into several items that share its
+// neighbourhood (item = (first cell, W | chunk << 5)).
+__device__ __forceinline__ void emit_units(const uint32_t* __restrict__ offset, uint32_t k, uint32_t f, uint32_t uw,
+                                           uint32_t budget, uint32_t min_own, uint2* __restrict__ units,
+                                           uint32_t* __restrict__ nunits) {
+    const uint32_t kb = k & ~(uw - 1u);
+    if (f != offset[kb]) return;
+    const uint32_t nseg = offset[kb + uw] - offset[kb];
+    if (nseg < min_own) return;  // left to the per-particle path
+    uint32_t W = uw;
+    while (W > 1u && 9ull * (W + 2u) * nseg > (unsigned long long)budget * uw) W >>= 1;
+    for (uint32_t q0 = 0; q0 < uw; q0 += W) {
+        const uint32_t kq = kb + q0;
+        const uint32_t own = offset[kq + W] - offset[kq];
+        if (own) {
+            const uint32_t nch = (own + kChunk - 1u) / kChunk;
+            const uint32_t u = atomicAdd(nunits, nch);
+            for (uint32_t c = 0; c < nch; ++c) units[u + c] = make_uint2(kq, W | (c << 5));
+        }
+    }
 }
 
 // ------------------------------------------------- K5: stable reorder
@@ -422,34 +477,57 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
     }
     st3(PA, cap, f, mk3(q[0], q[1], q[2]));
     st3(X, cap, f, mk3(xo[0], xo[1], xo[2]));
-    if (k < G.C) {
-        const uint32_t kb = k & ~(uw - 1u);
-        if (f == offset[kb]) {
-            const uint32_t nseg = offset[kb + uw] - offset[kb];
-            uint32_t W = uw;
-            while (W > 1u && 9ull * (W + 2u) * nseg > (unsigned long long)budget * uw) W >>= 1;
-            for (uint32_t q0 = 0; q0 < uw; q0 += W) {
-                const uint32_t kq = kb + q0;
-                if (offset[kq + W] > offset[kq]) {
-                    const uint32_t u = atomicAdd(&ctr[CTR_UNITS], 1u);
-                    units[u] = make_uint2(kq, W);
-                }
-            }
-        }
-    }
+    if (k < G.C) emit_units(offset, k, f, uw, budget, 0u, units, &ctr[CTR_UNITS]);
 }
+
 
 // ------------------------------------------------- work units
 struct UnitShared {
     uint32_t tstart[kMaxCells + 1];  // tile start of each neighbourhood cell (row-major 9 x (W+2))
     uint32_t gstart[kMaxCells];      // sorted slot of each neighbourhood cell's first particle
+    uint32_t cnt[kMaxCells];
+    uint32_t run_e[kMaxCells + 1];   // first cell of each run of cells contiguous in memory
     uint32_t wsum[kWaves];
 };
 
 struct UnitInfo {
-    uint32_t W, ncell, xb, y, z, total, own_t0, own_g0, n_own;
+    uint32_t W, ncell, xb, y, z, total, own_t0, own_g0, n_own, nruns;
     bool staged, ok;
 };
+
+// Per-thread event tallies, summed per wave and added once at kernel end.
+struct Tally {
+    uint32_t trunc = 0, slowpart = 0, slowunits = 0, overflow = 0, maxtile = 0, sweepcap = 0, maxsweep = 0,
+             nan = 0;
+};
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
+    return v;
+}
+
+// Called by every thread of the block at kernel end.
+__device__ void flush_tally(uint32_t* __restrict__ ctr, const Tally& t) {
+    const uint32_t tr = wave_sum(t.trunc), sp = wave_sum(t.slowpart), su = wave_sum(t.slowunits),
+                   ov = wave_sum(t.overflow), mt = wave_max(t.maxtile), sc = wave_sum(t.sweepcap),
+                   ms = wave_max(t.maxsweep), nn = wave_sum(t.nan);
+    if (lane_id() == 0) {
+        if (tr) atomicAdd(&ctr[CTR_TRUNC], tr);
+        if (sp) atomicAdd(&ctr[CTR_SLOWPART], sp);
+        if (su) atomicAdd(&ctr[CTR_SLOWUNITS], su);
+        if (ov) atomicAdd(&ctr[CTR_OVERFLOW], ov);
+        if (mt) atomicMax(&ctr[CTR_MAXTILE], mt);
+        if (sc) atomicAdd(&ctr[CTR_SWEEPCAP], sc);
+        if (ms) atomicMax(&ctr[CTR_MAXSWEEP], ms);
+        if (nn) atomicAdd(&ctr[CTR_XSPHNAN], nn);
+    }
+}
 
 // XCD-aware split: workgroup b runs on XCD b % 8; each XCD takes a contiguous
 // share of the (roughly key-ordered) unit list so neighbouring units share an L2.
@@ -471,11 +549,14 @@ __device__ __forceinline__ void unit_range(uint32_t nU, uint32_t& u0, uint32_t& 
 
 // Builds the neighbourhood table of a unit: 9 rows (dz outer, dy inner -- the
 // reference's z,y order) of W+2 cells (x0-1 .. x0+W, wrapped), their tile
-// starts and sorted slots.  Contains barriers: call uniformly.
+// starts and sorted slots, and the runs of cells whose particles are
+// contiguous in memory (one per row unless the row wraps).  Contains
+// barriers: call uniformly.
 __device__ UnitInfo unit_setup(const Grid& G, const uint32_t* __restrict__ offset, uint2 unit, uint32_t tile_cap,
-                               UnitShared& S, uint32_t* __restrict__ ctr) {
+                               UnitShared& S, Tally& tl) {
     UnitInfo I;
-    I.W = unit.y;
+    I.W = unit.y & 31u;
+    const uint32_t chunk = unit.y >> 5;
     I.ncell = 9u * (I.W + 2u);
     const uint32_t kb = unit.x;
     I.xb = kb & (G.gx - 1u);
@@ -492,25 +573,47 @@ __device__ UnitInfo unit_setup(const Grid& G, const uint32_t* __restrict__ offse
         const uint32_t gs = offset[key];
         cnt = offset[key + 1] - gs;
         S.gstart[e] = gs;
+        S.cnt[e] = cnt;
     }
     uint32_t total;
     const uint32_t ex = block_excl_scan(cnt, S.wsum, total);
     if (e < I.ncell) S.tstart[e] = ex;
     if (e == 0) S.tstart[I.ncell] = total;
+    uint32_t flag = 0;
+    if (e < I.ncell) flag = (e == 0 || S.gstart[e] != S.gstart[e - 1] + S.cnt[e - 1]) ? 1u : 0u;
+    uint32_t nruns;
+    const uint32_t ri = block_excl_scan(flag, S.wsum, nruns);
+    if (flag) S.run_e[ri] = e;
+    if (e == 0) S.run_e[nruns] = I.ncell;
     __syncthreads();
     const uint32_t base = 4u * (I.W + 2u);
     I.total = total;
-    I.own_t0 = S.tstart[base + 1];
-    I.own_g0 = S.gstart[base + 1];
-    I.n_own = S.tstart[base + I.W + 1] - I.own_t0;
+    I.nruns = nruns;
+    const uint32_t skip = chunk * kChunk;
+    I.own_t0 = S.tstart[base + 1] + skip;
+    I.own_g0 = S.gstart[base + 1] + skip;
+    I.n_own = min(kChunk, S.tstart[base + I.W + 1] - I.own_t0);
     I.ok = total <= kTileIndexLimit;
     I.staged = I.ok && total <= tile_cap;
     if (threadIdx.x == 0) {
-        atomicMax(&ctr[CTR_MAXTILE], total);
-        if (!I.ok) atomicAdd(&ctr[CTR_OVERFLOW], 1u);
-        else if (!I.staged) atomicAdd(&ctr[CTR_SLOWUNITS], 1u);
+        tl.maxtile = max(tl.maxtile, total);
+        if (!I.ok) tl.overflow++;
+        else if (!I.staged) tl.slowunits++;
     }
     return I;
+}
+
+// Copies the unit's neighbourhood into the tile, one run per wave at a time
+// (coalesced): put(tile index, sorted slot).
+template <class F>
+__device__ __forceinline__ void load_runs(const UnitShared& S, const UnitInfo& I, F&& put) {
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    for (uint32_t ri = wid; ri < I.nruns; ri += kWaves) {
+        const uint32_t e0 = S.run_e[ri], e1 = S.run_e[ri + 1];
+        const uint32_t t0 = S.tstart[e0], len = S.tstart[e1] - t0, g0 = S.gstart[e0];
+#pragma unroll 4
+        for (uint32_t i = lane; i < len; i += 64) put(t0 + i, g0 + i);
+    }
 }
 
 // tile index -> sorted slot (largest cell e with tstart[e] <= t)
@@ -538,12 +641,46 @@ __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, 
 
 __device__ __forceinline__ F3 f3of(float4 f) { return mk3(f.x, f.y, f.z); }
 
+// Copies a particle's frozen list row into its LDS slot (vector loads).
+__device__ __forceinline__ void fetch_row(const uint16_t* __restrict__ list, uint32_t g, int stride, uint16_t* my) {
+    const uint4* src = reinterpret_cast<const uint4*>(list + (size_t)g * stride);
+    uint4* dst = reinterpret_cast<uint4*>(my);
+    if (stride == 32) {
+        const uint4 a = src[0], b = src[1], c = src[2], d = src[3];
+        dst[0] = a;
+        dst[1] = b;
+        dst[2] = c;
+        dst[3] = d;
+    } else {
+        for (int c = 0; c < stride / 8; ++c) dst[c] = src[c];
+    }
+}
+
+// lambda of one particle from its list (calcScalingFactorCUDA memo form,
+// pbf_constraint.cu:168-209); pos(t) gives the neighbour's position.
+template <class PosF>
+__device__ __forceinline__ float lambda_of(F3 self, const uint16_t* my, int cnt, const Phys& P, PosF&& pos) {
+    float ks = 0.f, kg2s = 0.f;
+    F3 kgs = mk3(0.f, 0.f, 0.f);
+    for (int c = 0; c < cnt; ++c) {
+        float W;
+        F3 kg;
+        pair_terms(self, pos(my[c]), P, W, kg);
+        const float kg2 = fdot(kg, kg);
+        ks += W;
+        acc(kgs, kg);
+        kg2s += kg2;
+    }
+    const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
+    const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
+    return Cc / __builtin_fmaf(P.m2r2, Gs, P.relax);
+}
+
 // ------------------------------------------------- K6: neighbour lists + lambda_0
 // detectNeighborsCUDA (pbf_neighbor_search_util.cu:361-482): per particle the
 // 27 cells in z,y,x order, slots ascending, keep r2 < h^2 while fewer than
 // cap are kept; the list is frozen for all Jacobi iterations.  Entries are
-// tile indices of the unit's neighbourhood.  Then calcScalingFactorCUDA
-// (memo form, pbf_constraint.cu:168-209) for iteration 0.
+// tile indices of the unit's neighbourhood.  Then lambda for iteration 0.
 __global__ __launch_bounds__(kBlock) void k_ns_lambda(Grid G, Phys P, Bufs b, uint32_t tile_cap,
                                                       const float* __restrict__ Pc) {
     __shared__ UnitShared S;
@@ -554,14 +691,14 @@ __global__ __launch_bounds__(kBlock) void k_ns_lambda(Grid G, Phys P, Bufs b, ui
     const uint32_t nU = b.ctr[CTR_UNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
+    Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, tl);
         if (I.ok) {
             if (I.staged)
-                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
-                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                load_runs(S, I, [&](uint32_t t, uint32_t g) {
                     tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], 0.f);
-                }
+                });
             __syncthreads();
             uint16_t* my = lst + threadIdx.x * P.stride;
             for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
@@ -590,29 +727,17 @@ __global__ __launch_bounds__(kBlock) void k_ns_lambda(Grid G, Phys P, Bufs b, ui
                         }
                     }
                 }
-                float ks = 0.f, kg2s = 0.f;
-                F3 kgs = mk3(0.f, 0.f, 0.f);
-                for (int c = 0; c < cnt; ++c) {
-                    float W;
-                    F3 kg;
-                    pair_terms(self, pos(my[c]), P, W, kg);
-                    const float kg2 = fdot(kg, kg);
-                    ks += W;
-                    acc(kgs, kg);
-                    kg2s += kg2;
-                }
-                const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
-                const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
-                b.lam[g] = Cc / __builtin_fmaf(P.m2r2, Gs, P.relax);
+                b.lam[g] = lambda_of(self, my, cnt, P, pos);
                 my[P.stride - 1] = (uint16_t)cnt;
                 uint4* dst = reinterpret_cast<uint4*>(b.list + (size_t)g * P.stride);
                 const uint4* src = reinterpret_cast<const uint4*>(my);
                 for (int c = 0; c < P.stride / 8; ++c) dst[c] = src[c];
-                if (trunc) atomicAdd(&b.ctr[CTR_TRUNC], 1u);
+                tl.trunc += trunc ? 1u : 0u;
             }
         }
         __syncthreads();
     }
+    flush_tally(b.ctr, tl);
 }
 
 // ------------------------------------------------- K8: lambda of iterations >= 1
@@ -621,46 +746,34 @@ __global__ __launch_bounds__(kBlock) void k_lambda(Grid G, Phys P, Bufs b, uint3
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tile = smem;
+    uint16_t* lst = reinterpret_cast<uint16_t*>(smem + tile_cap);
     const uint32_t cap = b.cap;
     const uint32_t nU = b.ctr[CTR_UNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
+    Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, tl);
         if (I.ok) {
             if (I.staged)
-                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
-                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                load_runs(S, I, [&](uint32_t t, uint32_t g) {
                     tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], 0.f);
-                }
+                });
             __syncthreads();
+            uint16_t* my = lst + threadIdx.x * P.stride;
             for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
                 const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
+                fetch_row(b.list, g, P.stride, my);
                 auto pos = [&](uint32_t t) -> F3 {
                     if (I.staged) return f3of(tile[t]);
                     return ld3(Pc, cap, tile_to_global(S, I.ncell, t));
                 };
-                const F3 self = pos(ts);
-                const uint16_t* row = b.list + (size_t)g * P.stride;
-                const int cnt = row[P.stride - 1];
-                float ks = 0.f, kg2s = 0.f;
-                F3 kgs = mk3(0.f, 0.f, 0.f);
-                for (int c = 0; c < cnt; ++c) {
-                    float W;
-                    F3 kg;
-                    pair_terms(self, pos(row[c]), P, W, kg);
-                    const float kg2 = fdot(kg, kg);
-                    ks += W;
-                    acc(kgs, kg);
-                    kg2s += kg2;
-                }
-                const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
-                const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
-                b.lam[g] = Cc / __builtin_fmaf(P.m2r2, Gs, P.relax);
+                b.lam[g] = lambda_of(pos(ts), my, my[P.stride - 1], P, pos);
             }
         }
         __syncthreads();
     }
+    flush_tally(b.ctr, tl);
 }
 
 // ------------------------------------------------- K7: delta-p + collision + apply
@@ -673,22 +786,24 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tile = smem;
+    uint16_t* lst = reinterpret_cast<uint16_t*>(smem + tile_cap);
     const uint32_t cap = b.cap;
     const uint32_t nU = b.ctr[CTR_UNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
-    uint32_t capped_count = 0, max_sw = 0;
+    Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, tl);
         if (I.ok) {
             if (I.staged)
-                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
-                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                load_runs(S, I, [&](uint32_t t, uint32_t g) {
                     tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], b.lam[g]);
-                }
+                });
             __syncthreads();
+            uint16_t* my = lst + threadIdx.x * P.stride;
             for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
                 const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
+                fetch_row(b.list, g, P.stride, my);
                 auto elem = [&](uint32_t t) -> float4 {
                     if (I.staged) return tile[t];
                     const uint32_t gg = tile_to_global(S, I.ncell, t);
@@ -698,11 +813,10 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
                 const F3 self = f3of(se);
                 const float si = se.w;
                 const float self_cl = (si < 0.f) ? -si : 0.f;
-                const uint16_t* row = b.list + (size_t)g * P.stride;
-                const int cnt = row[P.stride - 1];
+                const int cnt = my[P.stride - 1];
                 F3 sum = mk3(0.f, 0.f, 0.f);
                 for (int c = 0; c < cnt; ++c) {
-                    const float4 pe = elem(row[c]);
+                    const float4 pe = elem(my[c]);
                     float W;
                     F3 kg;
                     pair_terms(self, f3of(pe), P, W, kg);
@@ -724,25 +838,66 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
                 F3 dp = mk3(sum.x * P.m * P.inv_rho0, sum.y * P.m * P.inv_rho0, sum.z * P.m * P.inv_rho0);
                 bool capped = false;
                 const int sw = collide(P, B, dp, self, capped);
-                capped_count += capped ? 1u : 0u;
-                max_sw = max(max_sw, (uint32_t)sw);
+                tl.sweepcap += capped ? 1u : 0u;
+                tl.maxsweep = max(tl.maxsweep, (uint32_t)sw);
                 st3(Pn, cap, g, self + dp);
             }
         }
         __syncthreads();
     }
-    if (capped_count) atomicAdd(&b.ctr[CTR_SWEEPCAP], capped_count);
-    if (max_sw) atomicMax(&b.ctr[CTR_MAXSWEEP], max_sw);
+    flush_tally(b.ctr, tl);
 }
 
-// Sum over the 27 cells around the particle's CURRENT cell, cell ranges from
-// the search (calcContribution, pbf/interaction/cuda/detail/pbf_contribution.inl:7-57),
-// per-cell partial sums added in z,y,x order.  Fast path: the current cell is
-// one of the unit's own cells, so all 27 cells are in the staged tile.
+// ------------------------------------------------- final-cell sort
+// The vorticity, confinement and XSPH sums visit the 27 cells around each
+// particle's FINAL position, bucketed by the search's cell ranges
+// (pbf/interaction/cuda/detail/pbf_contribution.inl:7-57).  Particles are
+// regrouped by final cell (order inside a cell is irrelevant: every sum is
+// per particle) so that each unit's own particles have their 27 cells in the
+// unit's tile.
+__global__ __launch_bounds__(256) void k_cur_hash(const float* __restrict__ Pc, uint32_t cap,
+                                                  const uint32_t* __restrict__ offset, Grid G,
+                                                  uint32_t* __restrict__ ckey, uint32_t* __restrict__ crank,
+                                                  uint32_t* __restrict__ ccount, uint32_t* __restrict__ ctflag,
+                                                  uint32_t lgT) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nl = offset[G.C];
+    const bool active = s < nl;
+    uint32_t k = 0;
+    if (active) {
+        int gp[3];
+        grid_pos(ld3(Pc, cap, s), G.cw, gp);
+        k = grid_hash(gp[0], gp[1], gp[2], G);
+    }
+    const uint32_t r = wave_rank(ccount, k, active);
+    if (active) {
+        mark_tile(ctflag, k >> lgT);
+        ckey[s] = k;
+        crank[s] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict__ offset, Grid G,
+                                                     const uint32_t* __restrict__ ckey,
+                                                     const uint32_t* __restrict__ crank,
+                                                     const uint32_t* __restrict__ coffset,
+                                                     uint32_t* __restrict__ cperm, uint32_t uw, uint32_t budget,
+                                                     uint2* __restrict__ cunits, uint32_t* __restrict__ ctr) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= offset[G.C]) return;
+    const uint32_t k = ckey[s];
+    const uint32_t slot = coffset[k] + crank[s];
+    cperm[slot] = s;
+    emit_units(coffset, k, slot, uw, budget, kSparse, cunits, &ctr[CTR_CUNITS]);
+}
+
+// Sum over the 27 cells around the particle's final cell, per-cell partial
+// sums added in z,y,x order.  Fast path: the final cell is one of the unit's
+// own cells (always, for final-cell units) and the tile is staged.
 template <class PairF, class GlobF>
 __device__ __forceinline__ F3 sum27(const Grid& G, const UnitInfo& I, const UnitShared& S,
                                     const uint32_t* __restrict__ offset, F3 self, PairF&& pair_tile,
-                                    GlobF&& pair_glob, uint32_t* __restrict__ ctr) {
+                                    GlobF&& pair_glob, Tally& tl) {
     int gp[3];
     grid_pos(self, G.cw, gp);
     const uint32_t cx = (uint32_t)gp[0] & (G.gx - 1u);
@@ -759,7 +914,7 @@ __device__ __forceinline__ F3 sum27(const Grid& G, const UnitInfo& I, const Unit
                 acc(sum, cs);
             }
     } else {
-        atomicAdd(&ctr[CTR_SLOWPART], 1u);
+        tl.slowpart++;
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
@@ -770,6 +925,55 @@ __device__ __forceinline__ F3 sum27(const Grid& G, const UnitInfo& I, const Unit
                 }
     }
     return sum;
+}
+
+// The same sum straight from global memory: per z plane the 9 cell ranges are
+// loaded together, then visited in y, x order.
+template <class GlobF>
+__device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self,
+                                           GlobF&& pair_glob) {
+    int gp[3];
+    grid_pos(self, G.cw, gp);
+    F3 sum = mk3(0.f, 0.f, 0.f);
+    for (int dz = -1; dz <= 1; ++dz) {
+        uint32_t st[9], en[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const uint32_t c = grid_hash(gp[0] + (q % 3) - 1, gp[1] + (q / 3) - 1, gp[2] + dz, G);
+            st[q] = offset[c];
+            en[q] = offset[c + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            F3 cs = mk3(0.f, 0.f, 0.f);
+            for (uint32_t j = st[q]; j < en[q]; ++j) acc(cs, pair_glob(j));
+            acc(sum, cs);
+        }
+    }
+    return sum;
+}
+
+// Particles of final-cell runs too sparse for a unit (fewer than kSparse
+// particles in their run of uw cells): one thread each, grid-strided.
+template <class F>
+__device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_t uw, F&& body) {
+    const uint32_t nl = b.offset[G.C];
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nl; s += gridDim.x * blockDim.x) {
+        const uint32_t kb = b.ckey[s] & ~(uw - 1u);
+        if (b.coffset[kb + uw] - b.coffset[kb] < kSparse) body(s);
+    }
+}
+
+// Own particles of a final-cell unit: slots cperm[coffset[kb] ..).
+struct OwnCur {
+    uint32_t c0, n;
+};
+__device__ __forceinline__ OwnCur own_cur(const Bufs& b, uint2 unit) {
+    OwnCur o;
+    const uint32_t W = unit.y & 31u, skip = (unit.y >> 5) * kChunk;
+    o.c0 = b.coffset[unit.x] + skip;
+    o.n = min(kChunk, b.coffset[unit.x + W] - o.c0);
+    return o;
 }
 
 // vorticity pair term (pbf_vorticity_confinement.cu:15-31)
@@ -786,31 +990,33 @@ __device__ __forceinline__ F3 vort_pair(F3 self, F3 selfv, F3 pj, F3 vj, float w
 // updateVelocityCUDA (pbf_update.cu:25-39): v = inv_dt*(x* - x_sorted), then
 // calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
 // spiky derivative (or h, when reproducing the reference's argument).
-__global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_t tile_cap,
+__global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw,
                                                  const float* __restrict__ Pc) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
     float4* tv = smem + tile_cap;
     const uint32_t cap = b.cap;
-    const uint32_t nU = b.ctr[CTR_UNITS];
+    const uint32_t nU = b.ctr[CTR_CUNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
     auto vel = [&](uint32_t g, F3 p) -> F3 { return mulsv(P.inv_dt, p - ld3(b.X, cap, g)); };
+    Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        const uint2 unit = b.cunits[u];
+        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
             if (I.staged)
-                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
-                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                load_runs(S, I, [&](uint32_t t, uint32_t g) {
                     const F3 p = ld3(Pc, cap, g);
                     const F3 vv = vel(g, p);
                     tp[t] = make_float4(p.x, p.y, p.z, 0.f);
                     tv[t] = make_float4(vv.x, vv.y, vv.z, 0.f);
-                }
+                });
             __syncthreads();
-            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
-                const uint32_t g = I.own_g0 + q;
+            const OwnCur o = own_cur(b, unit);
+            for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
+                const uint32_t g = b.cperm[o.c0 + q];
                 const F3 self = ld3(Pc, cap, g);
                 const F3 selfv = vel(g, self);
                 const F3 w = sum27(
@@ -820,13 +1026,25 @@ __global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_
                         const F3 pj = ld3(Pc, cap, j);
                         return vort_pair(self, selfv, pj, vel(j, pj), P.warg);
                     },
-                    b.ctr);
+                    tl);
                 st3(b.OM, cap, g, mulsv(P.m * P.inv_rho0, w));
                 st3(b.V, cap, g, selfv);
             }
         }
         __syncthreads();
     }
+    for_sparse(G, b, uw, [&](uint32_t g) {
+        tl.slowpart++;
+        const F3 self = ld3(Pc, cap, g);
+        const F3 selfv = vel(g, self);
+        const F3 w = sum27_global(G, b.offset, self, [&](uint32_t j) {
+            const F3 pj = ld3(Pc, cap, j);
+            return vort_pair(self, selfv, pj, vel(j, pj), P.warg);
+        });
+        st3(b.OM, cap, g, mulsv(P.m * P.inv_rho0, w));
+        st3(b.V, cap, g, selfv);
+    });
+    flush_tally(b.ctr, tl);
 }
 
 // confinement location-vector pair term (pbf_vorticity_confinement.cu:121-138)
@@ -840,27 +1058,29 @@ __device__ __forceinline__ F3 loc_pair(F3 self, F3 pj, float wj, float warg) {
 
 // ------------------------------------------------- K10: vorticity confinement
 // addVorticityConfinementCUDA (pbf_vorticity_confinement.cu:192-222), in place on V.
-__global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint32_t tile_cap,
+__global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw,
                                                     const float* __restrict__ Pc) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
     const uint32_t cap = b.cap;
-    const uint32_t nU = b.ctr[CTR_UNITS];
+    const uint32_t nU = b.ctr[CTR_CUNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
+    Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        const uint2 unit = b.cunits[u];
+        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
             if (I.staged)
-                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
-                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                load_runs(S, I, [&](uint32_t t, uint32_t g) {
                     const F3 p = ld3(Pc, cap, g);
                     tp[t] = make_float4(p.x, p.y, p.z, glm_length(ld3(b.OM, cap, g)));
-                }
+                });
             __syncthreads();
-            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
-                const uint32_t g = I.own_g0 + q;
+            const OwnCur o = own_cur(b, unit);
+            for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
+                const uint32_t g = b.cperm[o.c0 + q];
                 const F3 self = ld3(Pc, cap, g);
                 const F3 selfw = ld3(b.OM, cap, g);
                 F3 N = sum27(
@@ -870,7 +1090,7 @@ __global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint
                         return loc_pair(self, f3of(e), e.w, P.warg);
                     },
                     [&](uint32_t j) { return loc_pair(self, ld3(Pc, cap, j), glm_length(ld3(b.OM, cap, j)), P.warg); },
-                    b.ctr);
+                    tl);
                 if (glm_length(N) > 0.f) {
                     N = mulvs(mulvs(N, P.m), P.inv_rho0);
                     N = divvs(N, glm_length(N));
@@ -881,6 +1101,21 @@ __global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint
         }
         __syncthreads();
     }
+    for_sparse(G, b, uw, [&](uint32_t g) {
+        tl.slowpart++;
+        const F3 self = ld3(Pc, cap, g);
+        const F3 selfw = ld3(b.OM, cap, g);
+        F3 N = sum27_global(G, b.offset, self, [&](uint32_t j) {
+            return loc_pair(self, ld3(Pc, cap, j), glm_length(ld3(b.OM, cap, j)), P.warg);
+        });
+        if (glm_length(N) > 0.f) {
+            N = mulvs(mulvs(N, P.m), P.inv_rho0);
+            N = divvs(N, glm_length(N));
+            const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, selfw)), P.inv_rho0), P.dtc);
+            st3(b.V, cap, g, ld3(b.V, cap, g) + add);
+        }
+    });
+    flush_tally(b.ctr, tl);
 }
 
 // XSPH pair term (pbf_xsph.cu:15-29)
@@ -895,7 +1130,7 @@ __device__ __forceinline__ F3 xsph_pair(F3 self, F3 selfv, F3 pj, F3 vj, float i
 // calcXSPHCUDA (pbf_xsph.cu:84-113) and updatePosition (pbf_update.cu:16-23):
 // x_out = x*, v_out = v + c*m*inv_rho0*sum unless the sum is NaN.
 // Vs: velocities after confinement, or null to derive them from x* and x.
-__global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uint32_t tile_cap,
+__global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw,
                                                      const float* __restrict__ Pc, const float* __restrict__ Vs,
                                                      float* __restrict__ xo, float* __restrict__ vo) {
     __shared__ UnitShared S;
@@ -903,28 +1138,29 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uin
     float4* tp = smem;
     float4* tv = smem + tile_cap;
     const uint32_t cap = b.cap;
-    const uint32_t nU = b.ctr[CTR_UNITS];
+    const uint32_t nU = b.ctr[CTR_CUNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
     auto vel = [&](uint32_t g, F3 p) -> F3 {
         if (Vs) return ld3(Vs, cap, g);
         return mulsv(P.inv_dt, p - ld3(b.X, cap, g));
     };
-    uint32_t nan_count = 0;
+    Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, b.ctr);
+        const uint2 unit = b.cunits[u];
+        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
             if (I.staged)
-                for (uint32_t t = threadIdx.x; t < I.total; t += kBlock) {
-                    const uint32_t g = tile_to_global(S, I.ncell, t);
+                load_runs(S, I, [&](uint32_t t, uint32_t g) {
                     const F3 p = ld3(Pc, cap, g);
                     const F3 vv = vel(g, p);
                     tp[t] = make_float4(p.x, p.y, p.z, 0.f);
                     tv[t] = make_float4(vv.x, vv.y, vv.z, 0.f);
-                }
+                });
             __syncthreads();
-            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
-                const uint32_t g = I.own_g0 + q;
+            const OwnCur o = own_cur(b, unit);
+            for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
+                const uint32_t g = b.cperm[o.c0 + q];
                 const F3 self = ld3(Pc, cap, g);
                 const F3 selfv = vel(g, self);
                 const F3 a = sum27(
@@ -934,11 +1170,11 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uin
                         const F3 pj = ld3(Pc, cap, j);
                         return xsph_pair(self, selfv, pj, vel(j, pj), P.inv_h);
                     },
-                    b.ctr);
+                    tl);
                 F3 out;
                 if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
                     out = selfv;
-                    ++nan_count;
+                    ++tl.nan;
                 } else {
                     out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
                 }
@@ -952,7 +1188,29 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uin
         }
         __syncthreads();
     }
-    if (nan_count) atomicAdd(&b.ctr[CTR_XSPHNAN], nan_count);
+    for_sparse(G, b, uw, [&](uint32_t g) {
+        tl.slowpart++;
+        const F3 self = ld3(Pc, cap, g);
+        const F3 selfv = vel(g, self);
+        const F3 a = sum27_global(G, b.offset, self, [&](uint32_t j) {
+            const F3 pj = ld3(Pc, cap, j);
+            return xsph_pair(self, selfv, pj, vel(j, pj), P.inv_h);
+        });
+        F3 out;
+        if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
+            out = selfv;
+            ++tl.nan;
+        } else {
+            out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
+        }
+        xo[3 * g + 0] = self.x;
+        xo[3 * g + 1] = self.y;
+        xo[3 * g + 2] = self.z;
+        vo[3 * g + 0] = out.x;
+        vo[3 * g + 1] = out.y;
+        vo[3 * g + 2] = out.z;
+    });
+    flush_tally(b.ctr, tl);
 }
 
 // ------------------------------------------------- K12: write-back without XSPH
@@ -982,8 +1240,9 @@ __global__ __launch_bounds__(kBlock) void k_neighbors_view(Grid G, Phys P, Bufs 
     const uint32_t nU = b.ctr[CTR_UNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
+    Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], 0u, S, b.ctr);
+        const UnitInfo I = unit_setup(G, b.offset, b.units[u], 0u, S, tl);
         if (I.ok)
             for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
                 const uint32_t g = I.own_g0 + q;
@@ -1026,9 +1285,7 @@ __global__ void k_selftest(const float* __restrict__ a, const float* __restrict_
 
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-size_t smem_ns(const Tuning& T, const Phys& P) {
-    return (size_t)T.tile_cap * sizeof(float4) + (size_t)kBlock * P.stride * sizeof(uint16_t);
-}
+size_t smem_list(const Phys& P) { return (size_t)kBlock * P.stride * sizeof(uint16_t); }
 
 }  // namespace
 
@@ -1037,14 +1294,14 @@ int unit_blocks_per_cu(uint32_t tile_cap) {
     // many workgroups of the current tile size fit a CU; the two-tile
     // kernels bound the occupancy.
     int best = 1 << 30;
+    const size_t lst = (size_t)kBlock * 64 * 2;
     struct K {
         const void* f;
         size_t smem_max, smem;
     } ks[] = {
-        {(const void*)k_ns_lambda, (size_t)kTileCapMax * 16 + (size_t)kBlock * 64 * 2,
-         (size_t)tile_cap * 16 + (size_t)kBlock * 64 * 2},
-        {(const void*)k_lambda, (size_t)kTileCapMax * 16, (size_t)tile_cap * 16},
-        {(const void*)k_deltap, (size_t)kTileCapMax * 16, (size_t)tile_cap * 16},
+        {(const void*)k_ns_lambda, (size_t)kTileCapMax * 16 + lst, (size_t)tile_cap * 16 + lst},
+        {(const void*)k_lambda, (size_t)kTileCapMax * 16 + lst, (size_t)tile_cap * 16 + lst},
+        {(const void*)k_deltap, (size_t)kTileCapMax * 16 + lst, (size_t)tile_cap * 16 + lst},
         {(const void*)k_vort, (size_t)kTileCapMax * 32, (size_t)tile_cap * 32},
         {(const void*)k_confine, (size_t)kTileCapMax * 16, (size_t)tile_cap * 16},
         {(const void*)k_xsph_out, (size_t)kTileCapMax * 32, (size_t)tile_cap * 32},
@@ -1058,46 +1315,123 @@ int unit_blocks_per_cu(uint32_t tile_cap) {
     return best;
 }
 
+const char* kernel_name(int id) {
+    static const char* names[K_NUM] = {"k_predict_hash", "k_scan_tiles", "k_scan_cells", "k_scatter",
+                                       "k_reorder",      "k_ns_lambda",  "k_lambda",     "k_deltap",
+                                       "k_vort",         "k_confine",    "k_xsph_out",   "k_out",
+                                       "k_cur_sort"};
+    return (id >= 0 && id < K_NUM) ? names[id] : "?";
+}
+
+namespace {
+struct Mark {
+    Timer* t;
+    hipStream_t s;
+    int slot;
+    Mark(Timer* tm, hipStream_t st, int id) : t(tm), s(st), slot(-1) {
+        if (t && t->n < kTimerSlots) {
+            slot = t->n++;
+            t->id[slot] = id;
+            (void)hipEventRecord(t->ev[2 * slot], s);
+        }
+    }
+    ~Mark() {
+        if (slot >= 0) (void)hipEventRecord(t->ev[2 * slot + 1], s);
+    }
+};
+}  // namespace
+
 hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bufs& b, const Tuning& T, float* x,
                        float* v, uint32_t n, const float lo[3], const float hi[3], int iters, int xsph_on,
-                       int vort_on, hipStream_t s) {
+                       int vort_on, hipStream_t s, Timer* tm) {
     const uint32_t nb = ceil_div(n > 0 ? n : 1, 256);
     const uint32_t GU = T.grid_units;
     const uint32_t uw = T.unit_width < G.gx ? T.unit_width : G.gx;
     const uint32_t budget = (T.tile_cap * 3u) / 4u > 0 ? (T.tile_cap * 3u) / 4u : 1u;
-    if (n > 0)
+    if (n > 0) {
+        Mark m(tm, s, K_PREDICT_HASH);
         hipLaunchKernelGGL(k_predict_hash, dim3(nb), dim3(256), 0, s, x, v, n, G, P.dt, P.g[0], P.g[1], P.g[2], lo[0],
-                           lo[1], lo[2], hi[0], hi[1], hi[2], b.key, b.rank, b.count, b.tsum, b.lgT);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.tsum, b.tprefix, b.nT + 1, b.ctr);
-    hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.offset, b.tsum, b.tprefix, b.nT,
-                       b.lgT, G.C);
+                           lo[1], lo[2], hi[0], hi[1], hi[2], b.key, b.rank, b.count, b.tflag, b.lgT);
+    }
+    {
+        Mark m(tm, s, K_SCAN_TILES);
+        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.tflag, b.tsum, b.nT, b.lgT,
+                           G.C);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.tsum, b.tprefix, b.nT + 1, b.ctr, 1u);
+    }
+    {
+        Mark m(tm, s, K_SCAN_CELLS);
+        hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.offset, b.tsum, b.tflag,
+                           b.tprefix, b.nT, b.lgT, G.C);
+    }
     if (n == 0) return hipGetLastError();
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, b.key, b.rank, b.offset, b.tmp, n);
-    hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, x, v, G, P.dt, P.g[0], P.g[1], P.g[2], b.key, b.tmp,
-                       b.offset, b.sidx, b.PA, b.X, b.cap, n, uw, budget, b.units, b.ctr);
+    {
+        Mark m(tm, s, K_SCATTER);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, b.key, b.rank, b.offset, b.tmp, n);
+    }
+    {
+        Mark m(tm, s, K_REORDER);
+        hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, x, v, G, P.dt, P.g[0], P.g[1], P.g[2], b.key, b.tmp,
+                           b.offset, b.sidx, b.PA, b.X, b.cap, n, uw, budget, b.units, b.ctr);
+    }
     const size_t s1 = (size_t)T.tile_cap * sizeof(float4);
-    hipLaunchKernelGGL(k_ns_lambda, dim3(GU), dim3(kBlock), smem_ns(T, P), s, G, P, b, T.tile_cap,
-                       (const float*)b.PA);
+    {
+        Mark m(tm, s, K_NS_LAMBDA);
+        hipLaunchKernelGGL(k_ns_lambda, dim3(GU), dim3(kBlock), s1 + smem_list(P), s, G, P, b, T.tile_cap,
+                           (const float*)b.PA);
+    }
     float* cur = b.PA;
     float* nxt = b.PB;
     for (int it = 0; it < iters; ++it) {
-        if (it > 0)
-            hipLaunchKernelGGL(k_lambda, dim3(GU), dim3(kBlock), s1, s, G, P, b, T.tile_cap, (const float*)cur);
-        hipLaunchKernelGGL(k_deltap, dim3(GU), dim3(kBlock), s1, s, G, P, B, b, T.tile_cap, (const float*)cur, nxt);
+        if (it > 0) {
+            Mark m(tm, s, K_LAMBDA);
+            hipLaunchKernelGGL(k_lambda, dim3(GU), dim3(kBlock), s1 + smem_list(P), s, G, P, b, T.tile_cap,
+                               (const float*)cur);
+        }
+        {
+            Mark m(tm, s, K_DELTAP);
+            hipLaunchKernelGGL(k_deltap, dim3(GU), dim3(kBlock), s1 + smem_list(P), s, G, P, B, b, T.tile_cap,
+                               (const float*)cur, nxt);
+        }
         float* t = cur;
         cur = nxt;
         nxt = t;
     }
+    if (!vort_on && !xsph_on) {
+        Mark m(tm, s, K_OUT);
+        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, (const float*)nullptr, G.C, x,
+                           v);
+        return hipGetLastError();
+    }
+    {
+        Mark m(tm, s, K_CUR_SORT);
+        hipLaunchKernelGGL(k_cur_hash, dim3(nb), dim3(256), 0, s, (const float*)cur, b.cap, b.offset, G, b.ckey,
+                           b.crank, b.ccount, b.ctflag, b.lgT);
+        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, b.nT, b.lgT,
+                           G.C);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.ctsum, b.ctprefix, b.nT + 1, b.ctr, 0u);
+        hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(256), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
+                           b.ctprefix, b.nT, b.lgT, G.C);
+        hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, G, b.ckey, b.crank, b.coffset,
+                           b.cperm, uw, budget, b.cunits, b.ctr);
+    }
     if (vort_on) {
-        hipLaunchKernelGGL(k_vort, dim3(GU), dim3(kBlock), 2 * s1, s, G, P, b, T.tile_cap, (const float*)cur);
-        hipLaunchKernelGGL(k_confine, dim3(GU), dim3(kBlock), s1, s, G, P, b, T.tile_cap, (const float*)cur);
+        {
+            Mark m(tm, s, K_VORT);
+            hipLaunchKernelGGL(k_vort, dim3(GU), dim3(kBlock), 2 * s1, s, G, P, b, T.tile_cap, uw, (const float*)cur);
+        }
+        Mark m(tm, s, K_CONFINE);
+        hipLaunchKernelGGL(k_confine, dim3(GU), dim3(kBlock), s1, s, G, P, b, T.tile_cap, uw, (const float*)cur);
     }
     const float* vs = vort_on ? b.V : nullptr;
-    if (xsph_on)
-        hipLaunchKernelGGL(k_xsph_out, dim3(GU), dim3(kBlock), 2 * s1, s, G, P, b, T.tile_cap, (const float*)cur, vs,
-                           x, v);
-    else
+    if (xsph_on) {
+        Mark m(tm, s, K_XSPH_OUT);
+        hipLaunchKernelGGL(k_xsph_out, dim3(GU), dim3(kBlock), 2 * s1, s, G, P, b, T.tile_cap, uw, (const float*)cur,
+                           vs, x, v);
+    } else {
+        Mark m(tm, s, K_OUT);
         hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, vs, G.C, x, v);
+    }
     return hipGetLastError();
 }
 

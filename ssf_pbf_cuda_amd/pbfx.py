@@ -75,6 +75,10 @@ class NsView(ctypes.Structure):
     ]
 
 
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("total_ms", ctypes.c_double), ("launches", ctypes.c_uint32)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "num_in", "num_live", "truncated", "sweep_cap_hits", "max_sweeps_used", "xsph_nan", "units", "slow_units",
@@ -116,6 +120,8 @@ def lib():
             "pbfx_last_hip_error": ([vp], ctypes.c_int),
             "pbfx_status_string": ([ctypes.c_int], ctypes.c_char_p),
             "pbfx_selftest_arith": ([vp, vp, vp, u32, vp], ctypes.c_int),
+            "pbfx_set_kernel_timing": ([vp, ctypes.c_int], ctypes.c_int),
+            "pbfx_kernel_timing_read": ([vp, vp, ctypes.POINTER(KernelTime), u32, u32p], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -273,6 +279,17 @@ class Solver:
     def ns_cell_ranges(self, start, end, stream=None) -> None:
         _check(self._lib.pbfx_ns_cell_ranges(self._ctx, _ptr(start), _ptr(end), _stream(stream)),
                "pbfx_ns_cell_ranges")
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        _check(self._lib.pbfx_set_kernel_timing(self._ctx, int(bool(enable))), "pbfx_set_kernel_timing")
+
+    def kernel_times(self, stream=None) -> dict:
+        """{kernel: (total_ms, launches)} accumulated since timing was enabled."""
+        buf = (KernelTime * 32)()
+        n = ctypes.c_uint32()
+        _check(self._lib.pbfx_kernel_timing_read(self._ctx, _stream(stream), buf, 32, ctypes.byref(n)),
+               "pbfx_kernel_timing_read")
+        return {buf[i].name.decode(): (float(buf[i].total_ms), int(buf[i].launches)) for i in range(min(n.value, 32))}
 
     def scaling_factor(self, out, stream=None) -> None:
         _check(self._lib.pbfx_scaling_factor(self._ctx, _ptr(out), _stream(stream)), "pbfx_scaling_factor")

@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--tile-cap", type=int, default=2048)
     ap.add_argument("--unit-width", type=int, default=16)
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="finish with a plain device copy of this many MiB (a known-size kernel for "
+                         "scaling hardware byte counters)")
     return ap.parse_args()
 
 
@@ -248,6 +251,12 @@ def main():
     if a.cpu_seconds > 0:
         log(f"cpu baseline ({a.cpu_seconds:.0f} s budget)")
         out["cpu_baseline"] = cpu_baseline(sc, a.cpu_seconds)
+    if a.calibrate_copy_mb:
+        src = torch.ones(a.calibrate_copy_mb * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = src.numel() * 4
     print(json.dumps(out), flush=True)
 
 

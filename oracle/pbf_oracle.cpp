@@ -444,12 +444,26 @@ inline void store3(std::vector<R>& a, uint32_t s, V3<R> v) {
     a[3 * (size_t)s + 2] = v.z;
 }
 
+// Squared pair distance beyond which a kernel evaluated with argument `arg`
+// (q = r * arg) is zero: 1/arg^2 widened by 1e-5, so that every skipped pair
+// has q > 1 after rounding.
+inline float support_bound2(float arg) {
+    const double L = 1.0 / (double)arg;
+    return (float)(L * L * (1.0 + 1e-5));
+}
+
 // calcContribution (pbf/interaction/cuda/detail/pbf_contribution.inl:7-57 and
 // its copies in pbf_xsph.cu:51-82, pbf_vorticity_confinement.cu:51-82,156-186):
 // the 27 cells around the cell of self_pos (its CURRENT position, hashed in
-// float), cell ranges from the search, per-cell partial sums added in z,y,x order.
+// float), cell ranges from the search, per-cell partial sums added in z,y,x
+// order.  Pairs with dot(d, d) > cut2 lie outside the kernel support: their
+// terms are exactly zero for finite states and are not evaluated (the sums
+// are unchanged; only a non-finite velocity of a far particle, which the
+// reference would add as NaN, is not propagated).  pair(j, d, r2) gets
+// d = x_j - x_i and r2 = dot(d, d) (glm_length(d) = sqrt(r2)).
 template <class R, class F>
-inline V3<R> sum27(const NSData& ns, const float selfp[3], F&& pair) {
+inline V3<R> sum27(const NSData& ns, const std::vector<R>& P, V3<R> self, R cut2, F&& pair) {
+    float selfp[3] = {(float)self.x, (float)self.y, (float)self.z};
     int g[3];
     grid_pos(selfp[0], selfp[1], selfp[2], ns.cw, g);
     V3<R> sum = v3<R>(R(0), R(0), R(0));
@@ -458,17 +472,15 @@ inline V3<R> sum27(const NSData& ns, const float selfp[3], F&& pair) {
             for (int dx = -1; dx <= 1; ++dx) {
                 uint32_t c = grid_hash(g[0] + dx, g[1] + dy, g[2] + dz, ns.G);
                 V3<R> cs = v3<R>(R(0), R(0), R(0));
-                for (uint32_t j = ns.off[c]; j < ns.off[c + 1]; ++j) cs += pair(j);
+                for (uint32_t j = ns.off[c]; j < ns.off[c + 1]; ++j) {
+                    const V3<R> d = load3(P, j) - self;
+                    const R r2 = glm_dot(d, d);
+                    if (r2 > cut2) continue;
+                    cs += pair(j, d, r2);
+                }
                 sum += cs;
             }
     return sum;
-}
-
-template <class R>
-inline void hash_pos(V3<R> p, float out[3]) {
-    out[0] = (float)p.x;
-    out[1] = (float)p.y;
-    out[2] = (float)p.z;
 }
 
 struct SolveStats {
@@ -565,6 +577,8 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
     }
 
     const R warg = p->vort_h_as_inv_h ? k.h : k.inv_h;
+    const R cut_w = R(support_bound2((float)k.inv_h));
+    const R cut_wd = R(support_bound2(p->vort_h_as_inv_h ? p->smoothing_length : 1.f / p->smoothing_length));
     if (p->vort_on) {
         // calcVorticityCUDA pbf/solver/cuda/pbf_vorticity_confinement.cu:15-115
         std::vector<R> Wv(3 * (size_t)nl);
@@ -572,11 +586,8 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
         for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
             uint32_t i = (uint32_t)ii;
             V3<R> self = load3(P, i), selfv = load3(V, i);
-            float hp[3];
-            hash_pos(self, hp);
-            V3<R> w = sum27<R>(ns, hp, [&](uint32_t j) {
-                V3<R> pd = load3(P, j) - self;
-                R r = glm_length(pd);
+            V3<R> w = sum27<R>(ns, P, self, cut_wd, [&](uint32_t j, V3<R> pd, R r2) {
+                R r = std::sqrt(r2);
                 V3<R> vd = load3(V, j) - selfv;
                 V3<R> vort = v3<R>(R(0), R(0), R(0));
                 if (r > R(0)) {
@@ -593,11 +604,8 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
         for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
             uint32_t i = (uint32_t)ii;
             V3<R> self = load3(P, i), selfw = load3(Wv, i);
-            float hp[3];
-            hash_pos(self, hp);
-            V3<R> N = sum27<R>(ns, hp, [&](uint32_t j) {
-                V3<R> pd = load3(P, j) - self;
-                R r = glm_length(pd);
+            V3<R> N = sum27<R>(ns, P, self, cut_wd, [&](uint32_t j, V3<R> pd, R r2) {
+                R r = std::sqrt(r2);
                 R w = glm_length(load3(Wv, j));
                 V3<R> loc = v3<R>(R(0), R(0), R(0));
                 if (r > R(0)) {
@@ -625,11 +633,8 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
         for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
             uint32_t i = (uint32_t)ii;
             V3<R> self = load3(P, i), selfv = load3(V, i);
-            float hp[3];
-            hash_pos(self, hp);
-            V3<R> acc = sum27<R>(ns, hp, [&](uint32_t j) {
-                V3<R> pd = load3(P, j) - self;
-                R r = glm_length(pd);
+            V3<R> acc = sum27<R>(ns, P, self, cut_w, [&](uint32_t j, V3<R>, R r2) {
+                R r = std::sqrt(r2);
                 V3<R> vd = load3(V, j) - selfv;
                 R kr = wpoly6<R>(r, k.inv_h);
                 return mulvs(vd, kr);

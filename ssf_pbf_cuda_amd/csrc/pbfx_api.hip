@@ -53,6 +53,14 @@ struct DeviceGuard {
 };
 
 // pbf/kernel/cuda/detail/pbf_kernel.inl:34-48 on the host (IEEE float)
+// Squared distance beyond which a kernel evaluated with argument `arg`
+// (q = r * arg) is exactly zero: (1/arg)^2 with a 1e-5 relative margin, far
+// above the rounding of r and q, so the pairs skipped add exactly +-0.
+float support_bound2(float arg) {
+    const double L = 1.0 / (double)arg;
+    return (float)(L * L * (1.0 + 1e-5));
+}
+
 float wpoly6_host(float r, float inv_h) {
     float q = r * inv_h;
     float alpha = 1.56668147065f * inv_h * inv_h * inv_h;
@@ -118,6 +126,8 @@ Phys make_phys(const pbfx_params& p) {
     P.stride = p.max_pair <= 32 ? 32 : 64;
     P.max_sweeps = p.max_sweeps;
     P.outer_as_inner = p.outer_as_inner;
+    P.cut2_w = support_bound2(P.inv_h);
+    P.cut2_wd = support_bound2(P.warg);
     return P;
 }
 
@@ -167,9 +177,8 @@ pbfx_status alloc_list(pbfx_ctx* c, uint32_t stride) {
 }
 
 void set_grid_units(pbfx_ctx* c) {
-    c->blocks_per_cu = unit_blocks_per_cu(c->T.tile_cap);
-    c->T.grid_units = (uint32_t)(c->num_cu > 0 ? c->num_cu : 1) * (uint32_t)(c->blocks_per_cu > 0 ? c->blocks_per_cu : 1);
-    if (c->T.grid_units < 8) c->T.grid_units = 8;
+    size_unit_grids(c->T, c->num_cu, make_phys(c->params).stride);
+    c->blocks_per_cu = (int)(c->T.grid_units / (uint32_t)(c->num_cu > 0 ? c->num_cu : 1));
 }
 
 pbfx_status do_step(pbfx_ctx* c, float* x, float* v, uint32_t n, const float lo[3], const float hi[3], int iters,
@@ -369,6 +378,7 @@ pbfx_status pbfx_set_params(pbfx_ctx* c, const pbfx_params* p) {
     st = alloc_list(c, make_phys(*p).stride);
     if (st != PBFX_OK) return st;
     c->params = *p;
+    set_grid_units(c);
     return PBFX_OK;
 }
 

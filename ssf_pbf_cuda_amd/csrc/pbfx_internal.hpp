@@ -30,6 +30,7 @@ constexpr uint32_t kTileIndexLimit = 65535;  // tile indices are stored as u16
 constexpr uint32_t kScanTile = 4096;      // cells per scan tile
 constexpr uint32_t kChunk = kBlock;       // own particles per work item
 constexpr uint32_t kSparse = 64;          // final-cell runs below this go one thread per particle
+constexpr uint32_t kCompact = 16;         // in-range candidates buffered per lane in the final passes
 
 enum Ctr : int {
     CTR_UNITS = 0,
@@ -68,6 +69,7 @@ struct Phys {
     int32_t stride;         // u16 per list row
     int32_t max_sweeps;
     int32_t outer_as_inner;
+    float cut2_w, cut2_wd;  // squared support bounds of W (arg inv_h) and of W' (arg warg): pairs beyond add 0
 };
 
 struct Boundary {
@@ -143,7 +145,8 @@ struct Timer {
 struct Tuning {
     uint32_t tile_cap;      // particles staged on chip per unit
     uint32_t unit_width;    // max cells per unit along x
-    uint32_t grid_units;    // persistent workgroups for unit kernels
+    uint32_t grid_units;    // persistent workgroups for the unit kernels (the smallest of grid[])
+    uint32_t grid[K_NUM];   // persistent workgroups per unit kernel, from its own occupancy
 };
 
 // Launchers (pbfx_kernels.hip).  All enqueue on `s` and never synchronise.
@@ -156,6 +159,7 @@ hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, co
 hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uint32_t* end, hipStream_t s);
 hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s);
 hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t n, hipStream_t s);
-int unit_blocks_per_cu(uint32_t tile_cap);
+// Fills T.grid[] (and T.grid_units) for num_cu compute units and list rows of `stride` entries.
+void size_unit_grids(Tuning& T, int num_cu, int stride);
 
 }  // namespace pbfx

@@ -641,36 +641,36 @@ __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, 
 
 __device__ __forceinline__ F3 f3of(float4 f) { return mk3(f.x, f.y, f.z); }
 
-// Copies a particle's frozen list row into its LDS slot (vector loads).
-__device__ __forceinline__ void fetch_row(const uint16_t* __restrict__ list, uint32_t g, int stride, uint16_t* my) {
-    const uint4* src = reinterpret_cast<const uint4*>(list + (size_t)g * stride);
-    uint4* dst = reinterpret_cast<uint4*>(my);
-    if (stride == 32) {
-        const uint4 a = src[0], b = src[1], c = src[2], d = src[3];
-        dst[0] = a;
-        dst[1] = b;
-        dst[2] = c;
-        dst[3] = d;
-    } else {
-        for (int c = 0; c < stride / 8; ++c) dst[c] = src[c];
+// Visits a particle's frozen list in order: the row is read as uint4 chunks
+// of 8 entries held in registers (the count is the row's last entry).
+template <class F>
+__device__ __forceinline__ void for_list(const uint16_t* __restrict__ list, uint32_t g, int stride, F&& f) {
+    const uint4* row = reinterpret_cast<const uint4*>(list + (size_t)g * stride);
+    const int cnt = (int)(row[stride / 8 - 1].w >> 16);
+    for (int c0 = 0; c0 < cnt; c0 += 8) {
+        const uint4 w = row[c0 >> 3];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (c0 + j < cnt) f((ws[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu);
     }
 }
 
-// lambda of one particle from its list (calcScalingFactorCUDA memo form,
-// pbf_constraint.cu:168-209); pos(t) gives the neighbour's position.
-template <class PosF>
-__device__ __forceinline__ float lambda_of(F3 self, const uint16_t* my, int cnt, const Phys& P, PosF&& pos) {
+// lambda of one particle (calcScalingFactorCUDA memo form, pbf_constraint.cu:168-209):
+// `visit(fn)` calls fn(neighbour position) for the listed neighbours in order.
+template <class VisitF>
+__device__ __forceinline__ float lambda_of(F3 self, const Phys& P, VisitF&& visit) {
     float ks = 0.f, kg2s = 0.f;
     F3 kgs = mk3(0.f, 0.f, 0.f);
-    for (int c = 0; c < cnt; ++c) {
+    visit([&](F3 pj) {
         float W;
         F3 kg;
-        pair_terms(self, pos(my[c]), P, W, kg);
+        pair_terms(self, pj, P, W, kg);
         const float kg2 = fdot(kg, kg);
         ks += W;
         acc(kgs, kg);
         kg2s += kg2;
-    }
+    });
     const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
     const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
     return Cc / __builtin_fmaf(P.m2r2, Gs, P.relax);
@@ -727,7 +727,9 @@ __global__ __launch_bounds__(kBlock) void k_ns_lambda(Grid G, Phys P, Bufs b, ui
                         }
                     }
                 }
-                b.lam[g] = lambda_of(self, my, cnt, P, pos);
+                b.lam[g] = lambda_of(self, P, [&](auto&& fn) {
+                    for (int c = 0; c < cnt; ++c) fn(pos(my[c]));
+                });
                 my[P.stride - 1] = (uint16_t)cnt;
                 uint4* dst = reinterpret_cast<uint4*>(b.list + (size_t)g * P.stride);
                 const uint4* src = reinterpret_cast<const uint4*>(my);
@@ -746,7 +748,6 @@ __global__ __launch_bounds__(kBlock) void k_lambda(Grid G, Phys P, Bufs b, uint3
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tile = smem;
-    uint16_t* lst = reinterpret_cast<uint16_t*>(smem + tile_cap);
     const uint32_t cap = b.cap;
     const uint32_t nU = b.ctr[CTR_UNITS];
     uint32_t u0, u1, us;
@@ -760,15 +761,15 @@ __global__ __launch_bounds__(kBlock) void k_lambda(Grid G, Phys P, Bufs b, uint3
                     tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], 0.f);
                 });
             __syncthreads();
-            uint16_t* my = lst + threadIdx.x * P.stride;
             for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
                 const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
-                fetch_row(b.list, g, P.stride, my);
                 auto pos = [&](uint32_t t) -> F3 {
                     if (I.staged) return f3of(tile[t]);
                     return ld3(Pc, cap, tile_to_global(S, I.ncell, t));
                 };
-                b.lam[g] = lambda_of(pos(ts), my, my[P.stride - 1], P, pos);
+                b.lam[g] = lambda_of(pos(ts), P, [&](auto&& fn) {
+                    for_list(b.list, g, P.stride, [&](uint32_t t) { fn(pos(t)); });
+                });
             }
         }
         __syncthreads();
@@ -786,7 +787,6 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tile = smem;
-    uint16_t* lst = reinterpret_cast<uint16_t*>(smem + tile_cap);
     const uint32_t cap = b.cap;
     const uint32_t nU = b.ctr[CTR_UNITS];
     uint32_t u0, u1, us;
@@ -800,10 +800,8 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
                     tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], b.lam[g]);
                 });
             __syncthreads();
-            uint16_t* my = lst + threadIdx.x * P.stride;
             for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
                 const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
-                fetch_row(b.list, g, P.stride, my);
                 auto elem = [&](uint32_t t) -> float4 {
                     if (I.staged) return tile[t];
                     const uint32_t gg = tile_to_global(S, I.ncell, t);
@@ -813,10 +811,9 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
                 const F3 self = f3of(se);
                 const float si = se.w;
                 const float self_cl = (si < 0.f) ? -si : 0.f;
-                const int cnt = my[P.stride - 1];
                 F3 sum = mk3(0.f, 0.f, 0.f);
-                for (int c = 0; c < cnt; ++c) {
-                    const float4 pe = elem(my[c]);
+                for_list(b.list, g, P.stride, [&](uint32_t t) {
+                    const float4 pe = elem(t);
                     float W;
                     F3 kg;
                     pair_terms(self, f3of(pe), P, W, kg);
@@ -834,7 +831,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
                     }
                     const float a = -(si + sj + corr);
                     acc(sum, mulsv(a, kg));
-                }
+                });
                 F3 dp = mk3(sum.x * P.m * P.inv_rho0, sum.y * P.m * P.inv_rho0, sum.z * P.m * P.inv_rho0);
                 bool capped = false;
                 const int sw = collide(P, B, dp, self, capped);
@@ -891,47 +888,61 @@ __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict_
     emit_units(coffset, k, slot, uw, budget, kSparse, cunits, &ctr[CTR_CUNITS]);
 }
 
-// Sum over the 27 cells around the particle's final cell, per-cell partial
-// sums added in z,y,x order.  Fast path: the final cell is one of the unit's
-// own cells (always, for final-cell units) and the tile is staged.
-template <class PairF, class GlobF>
-__device__ __forceinline__ F3 sum27(const Grid& G, const UnitInfo& I, const UnitShared& S,
-                                    const uint32_t* __restrict__ offset, F3 self, PairF&& pair_tile,
-                                    GlobF&& pair_glob, Tally& tl) {
-    int gp[3];
-    grid_pos(self, G.cw, gp);
-    const uint32_t cx = (uint32_t)gp[0] & (G.gx - 1u);
-    const uint32_t cy = (uint32_t)gp[1] & (G.gy - 1u);
-    const uint32_t cz = (uint32_t)gp[2] & (G.gz - 1u);
-    F3 sum = mk3(0.f, 0.f, 0.f);
-    if (I.staged && cy == I.y && cz == I.z && (cx - I.xb) < I.W) {
-        const uint32_t kc = cx - I.xb + 1u;
-        for (uint32_t r = 0; r < 9u; ++r)
-            for (uint32_t dx = 0; dx < 3u; ++dx) {
-                const uint32_t e = r * (I.W + 2u) + kc - 1u + dx;
-                F3 cs = mk3(0.f, 0.f, 0.f);
-                for (uint32_t t = S.tstart[e]; t < S.tstart[e + 1]; ++t) acc(cs, pair_tile(t));
-                acc(sum, cs);
+
+// Sum over the 27 cells of the staged tile around own column kc (the
+// particle's final cell), per-cell partial sums added in z,y,x order
+// (calcContribution, pbf/interaction/cuda/detail/pbf_contribution.inl:7-57).
+// Pairs with dot(d, d) > cut2 lie outside the kernel support; their terms are
+// exactly zero for finite states and are skipped.  Candidates are filtered in
+// rounds: each lane collects up to kCompact in-range tile indices (tagged
+// with their cell ordinal) in its LDS slot, then the wave evaluates them, so
+// the expensive terms run only on in-range pairs.
+template <class TermF>
+__device__ __forceinline__ F3 sum27_tile(const UnitInfo& I, const UnitShared& S, const float4* __restrict__ tp,
+                                         uint32_t kc, F3 self, float cut2, uint32_t* __restrict__ buf,
+                                         TermF&& term) {
+    F3 sum = mk3(0.f, 0.f, 0.f), cs = mk3(0.f, 0.f, 0.f);
+    uint32_t cur = 0xFFFFFFFFu;
+    uint32_t q = 0;
+    uint32_t e = kc - 1u;
+    uint32_t t = S.tstart[e], tend = S.tstart[e + 1];
+    bool more = true;
+    while (__any(more)) {
+        uint32_t n = 0;
+        while (more && n < kCompact) {
+            if (t < tend) {
+                const F3 d = f3of(tp[t]) - self;
+                if (!(glm_dot(d, d) > cut2)) buf[n++] = t | (q << 16);
+                ++t;
+            } else if (++q < 27u) {
+                e = (q / 3u) * (I.W + 2u) + kc - 1u + (q % 3u);
+                t = S.tstart[e];
+                tend = S.tstart[e + 1];
+            } else {
+                more = false;
             }
-    } else {
-        tl.slowpart++;
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const uint32_t c = grid_hash(gp[0] + dx, gp[1] + dy, gp[2] + dz, G);
-                    F3 cs = mk3(0.f, 0.f, 0.f);
-                    for (uint32_t j = offset[c]; j < offset[c + 1]; ++j) acc(cs, pair_glob(j));
-                    acc(sum, cs);
-                }
+        }
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t ent = buf[k];
+            const uint32_t cq = ent >> 16;
+            if (cq != cur) {
+                acc(sum, cs);
+                cs = mk3(0.f, 0.f, 0.f);
+                cur = cq;
+            }
+            acc(cs, term(tp[ent & 0xFFFFu]));
+        }
     }
+    acc(sum, cs);
     return sum;
 }
 
 // The same sum straight from global memory: per z plane the 9 cell ranges are
-// loaded together, then visited in y, x order.
-template <class GlobF>
-__device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self,
-                                           GlobF&& pair_glob) {
+// loaded together, then visited in y, x order; elem(j) builds the tile-format
+// element of slot j.
+template <class ElemF, class TermF>
+__device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self, float cut2,
+                                           ElemF&& elem, TermF&& term) {
     int gp[3];
     grid_pos(self, G.cw, gp);
     F3 sum = mk3(0.f, 0.f, 0.f);
@@ -946,7 +957,12 @@ __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __rest
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             F3 cs = mk3(0.f, 0.f, 0.f);
-            for (uint32_t j = st[q]; j < en[q]; ++j) acc(cs, pair_glob(j));
+            for (uint32_t j = st[q]; j < en[q]; ++j) {
+                const float4 el = elem(j);
+                const F3 d = f3of(el) - self;
+                if (glm_dot(d, d) > cut2) continue;
+                acc(cs, term(el));
+            }
             acc(sum, cs);
         }
     }
@@ -964,7 +980,7 @@ __device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_
     }
 }
 
-// Own particles of a final-cell unit: slots cperm[coffset[kb] ..).
+// Own particles of a final-cell work item: slots cperm[coffset[kb] + chunk*kChunk ..).
 struct OwnCur {
     uint32_t c0, n;
 };
@@ -974,6 +990,13 @@ __device__ __forceinline__ OwnCur own_cur(const Bufs& b, uint2 unit) {
     o.c0 = b.coffset[unit.x] + skip;
     o.n = min(kChunk, b.coffset[unit.x + W] - o.c0);
     return o;
+}
+
+// Own column (1..W) of a final-cell unit's particle at position p.
+__device__ __forceinline__ uint32_t final_column(const Grid& G, const UnitInfo& I, F3 p) {
+    int gp[3];
+    grid_pos(p, G.cw, gp);
+    return (((uint32_t)gp[0] & (G.gx - 1u)) - I.xb) + 1u;
 }
 
 // vorticity pair term (pbf_vorticity_confinement.cu:15-31)
@@ -986,6 +1009,29 @@ __device__ __forceinline__ F3 vort_pair(F3 self, F3 selfv, F3 pj, F3 vj, float w
     return w;
 }
 
+// confinement location-vector pair term (pbf_vorticity_confinement.cu:121-138)
+__device__ __forceinline__ F3 loc_pair(F3 self, F3 pj, float wj, float warg) {
+    const F3 pd = pj - self;
+    const float r = glm_length(pd);
+    F3 l = mk3(0.f, 0.f, 0.f);
+    if (r > 0.f) l = mulvs(mulsv(wj, divvs(pd, r)), wspiky_deriv(r, warg));
+    return l;
+}
+
+// XSPH pair term (pbf_xsph.cu:15-29)
+__device__ __forceinline__ F3 xsph_pair(F3 self, F3 selfv, F3 pj, F3 vj, float inv_h) {
+    const F3 pd = pj - self;
+    const float r = glm_length(pd);
+    const F3 vd = vj - selfv;
+    return mulvs(vd, wpoly6(r, inv_h));
+}
+
+// Tile elements of the final passes: position and the particle's sorted slot
+// (as float bits) or, for the confinement, |omega|.
+__device__ __forceinline__ float4 pos_slot(const float* __restrict__ Pc, uint32_t cap, uint32_t g) {
+    return make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], __uint_as_float(g));
+}
+
 // ------------------------------------------------- K9: velocity + vorticity
 // updateVelocityCUDA (pbf_update.cu:25-39): v = inv_dt*(x* - x_sorted), then
 // calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
@@ -995,7 +1041,7 @@ __global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
-    float4* tv = smem + tile_cap;
+    uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
     const uint32_t cap = b.cap;
     const uint32_t nU = b.ctr[CTR_CUNITS];
     uint32_t u0, u1, us;
@@ -1006,27 +1052,25 @@ __global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_
         const uint2 unit = b.cunits[u];
         const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
-            if (I.staged)
-                load_runs(S, I, [&](uint32_t t, uint32_t g) {
-                    const F3 p = ld3(Pc, cap, g);
-                    const F3 vv = vel(g, p);
-                    tp[t] = make_float4(p.x, p.y, p.z, 0.f);
-                    tv[t] = make_float4(vv.x, vv.y, vv.z, 0.f);
-                });
+            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tp[t] = pos_slot(Pc, cap, g); });
             __syncthreads();
             const OwnCur o = own_cur(b, unit);
             for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
                 const uint32_t g = b.cperm[o.c0 + q];
                 const F3 self = ld3(Pc, cap, g);
                 const F3 selfv = vel(g, self);
-                const F3 w = sum27(
-                    G, I, S, b.offset, self,
-                    [&](uint32_t t) { return vort_pair(self, selfv, f3of(tp[t]), f3of(tv[t]), P.warg); },
-                    [&](uint32_t j) {
-                        const F3 pj = ld3(Pc, cap, j);
-                        return vort_pair(self, selfv, pj, vel(j, pj), P.warg);
-                    },
-                    tl);
+                auto term = [&](float4 el) {
+                    const F3 pj = f3of(el);
+                    return vort_pair(self, selfv, pj, vel(__float_as_uint(el.w), pj), P.warg);
+                };
+                F3 w;
+                if (I.staged) {
+                    w = sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_wd, buf, term);
+                } else {
+                    tl.slowpart++;
+                    w = sum27_global(G, b.offset, self, P.cut2_wd, [&](uint32_t j) { return pos_slot(Pc, cap, j); },
+                                     term);
+                }
                 st3(b.OM, cap, g, mulsv(P.m * P.inv_rho0, w));
                 st3(b.V, cap, g, selfv);
             }
@@ -1037,23 +1081,16 @@ __global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_
         tl.slowpart++;
         const F3 self = ld3(Pc, cap, g);
         const F3 selfv = vel(g, self);
-        const F3 w = sum27_global(G, b.offset, self, [&](uint32_t j) {
-            const F3 pj = ld3(Pc, cap, j);
-            return vort_pair(self, selfv, pj, vel(j, pj), P.warg);
-        });
+        const F3 w = sum27_global(
+            G, b.offset, self, P.cut2_wd, [&](uint32_t j) { return pos_slot(Pc, cap, j); },
+            [&](float4 el) {
+                const F3 pj = f3of(el);
+                return vort_pair(self, selfv, pj, vel(__float_as_uint(el.w), pj), P.warg);
+            });
         st3(b.OM, cap, g, mulsv(P.m * P.inv_rho0, w));
         st3(b.V, cap, g, selfv);
     });
     flush_tally(b.ctr, tl);
-}
-
-// confinement location-vector pair term (pbf_vorticity_confinement.cu:121-138)
-__device__ __forceinline__ F3 loc_pair(F3 self, F3 pj, float wj, float warg) {
-    const F3 pd = pj - self;
-    const float r = glm_length(pd);
-    F3 l = mk3(0.f, 0.f, 0.f);
-    if (r > 0.f) l = mulvs(mulsv(wj, divvs(pd, r)), wspiky_deriv(r, warg));
-    return l;
 }
 
 // ------------------------------------------------- K10: vorticity confinement
@@ -1063,40 +1100,43 @@ __global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
+    uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
     const uint32_t cap = b.cap;
     const uint32_t nU = b.ctr[CTR_CUNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
+    auto pos_w = [&](uint32_t g) {
+        const F3 p = ld3(Pc, cap, g);
+        return make_float4(p.x, p.y, p.z, glm_length(ld3(b.OM, cap, g)));
+    };
+    auto finish = [&](uint32_t g, F3 N, F3 selfw) {
+        if (glm_length(N) > 0.f) {
+            N = mulvs(mulvs(N, P.m), P.inv_rho0);
+            N = divvs(N, glm_length(N));
+            const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, selfw)), P.inv_rho0), P.dtc);
+            st3(b.V, cap, g, ld3(b.V, cap, g) + add);
+        }
+    };
     Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
         const uint2 unit = b.cunits[u];
         const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
-            if (I.staged)
-                load_runs(S, I, [&](uint32_t t, uint32_t g) {
-                    const F3 p = ld3(Pc, cap, g);
-                    tp[t] = make_float4(p.x, p.y, p.z, glm_length(ld3(b.OM, cap, g)));
-                });
+            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tp[t] = pos_w(g); });
             __syncthreads();
             const OwnCur o = own_cur(b, unit);
             for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
                 const uint32_t g = b.cperm[o.c0 + q];
                 const F3 self = ld3(Pc, cap, g);
-                const F3 selfw = ld3(b.OM, cap, g);
-                F3 N = sum27(
-                    G, I, S, b.offset, self,
-                    [&](uint32_t t) {
-                        const float4 e = tp[t];
-                        return loc_pair(self, f3of(e), e.w, P.warg);
-                    },
-                    [&](uint32_t j) { return loc_pair(self, ld3(Pc, cap, j), glm_length(ld3(b.OM, cap, j)), P.warg); },
-                    tl);
-                if (glm_length(N) > 0.f) {
-                    N = mulvs(mulvs(N, P.m), P.inv_rho0);
-                    N = divvs(N, glm_length(N));
-                    const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, selfw)), P.inv_rho0), P.dtc);
-                    st3(b.V, cap, g, ld3(b.V, cap, g) + add);
+                auto term = [&](float4 el) { return loc_pair(self, f3of(el), el.w, P.warg); };
+                F3 N;
+                if (I.staged) {
+                    N = sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_wd, buf, term);
+                } else {
+                    tl.slowpart++;
+                    N = sum27_global(G, b.offset, self, P.cut2_wd, pos_w, term);
                 }
+                finish(g, N, ld3(b.OM, cap, g));
             }
         }
         __syncthreads();
@@ -1104,26 +1144,11 @@ __global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint
     for_sparse(G, b, uw, [&](uint32_t g) {
         tl.slowpart++;
         const F3 self = ld3(Pc, cap, g);
-        const F3 selfw = ld3(b.OM, cap, g);
-        F3 N = sum27_global(G, b.offset, self, [&](uint32_t j) {
-            return loc_pair(self, ld3(Pc, cap, j), glm_length(ld3(b.OM, cap, j)), P.warg);
-        });
-        if (glm_length(N) > 0.f) {
-            N = mulvs(mulvs(N, P.m), P.inv_rho0);
-            N = divvs(N, glm_length(N));
-            const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, selfw)), P.inv_rho0), P.dtc);
-            st3(b.V, cap, g, ld3(b.V, cap, g) + add);
-        }
+        const F3 N = sum27_global(G, b.offset, self, P.cut2_wd, pos_w,
+                                  [&](float4 el) { return loc_pair(self, f3of(el), el.w, P.warg); });
+        finish(g, N, ld3(b.OM, cap, g));
     });
     flush_tally(b.ctr, tl);
-}
-
-// XSPH pair term (pbf_xsph.cu:15-29)
-__device__ __forceinline__ F3 xsph_pair(F3 self, F3 selfv, F3 pj, F3 vj, float inv_h) {
-    const F3 pd = pj - self;
-    const float r = glm_length(pd);
-    const F3 vd = vj - selfv;
-    return mulvs(vd, wpoly6(r, inv_h));
 }
 
 // ------------------------------------------------- K11: XSPH + write-back
@@ -1136,7 +1161,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uin
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
-    float4* tv = smem + tile_cap;
+    uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
     const uint32_t cap = b.cap;
     const uint32_t nU = b.ctr[CTR_CUNITS];
     uint32_t u0, u1, us;
@@ -1146,56 +1171,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uin
         return mulsv(P.inv_dt, p - ld3(b.X, cap, g));
     };
     Tally tl;
-    for (uint32_t u = u0; u < u1; u += us) {
-        const uint2 unit = b.cunits[u];
-        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
-        if (I.ok) {
-            if (I.staged)
-                load_runs(S, I, [&](uint32_t t, uint32_t g) {
-                    const F3 p = ld3(Pc, cap, g);
-                    const F3 vv = vel(g, p);
-                    tp[t] = make_float4(p.x, p.y, p.z, 0.f);
-                    tv[t] = make_float4(vv.x, vv.y, vv.z, 0.f);
-                });
-            __syncthreads();
-            const OwnCur o = own_cur(b, unit);
-            for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
-                const uint32_t g = b.cperm[o.c0 + q];
-                const F3 self = ld3(Pc, cap, g);
-                const F3 selfv = vel(g, self);
-                const F3 a = sum27(
-                    G, I, S, b.offset, self,
-                    [&](uint32_t t) { return xsph_pair(self, selfv, f3of(tp[t]), f3of(tv[t]), P.inv_h); },
-                    [&](uint32_t j) {
-                        const F3 pj = ld3(Pc, cap, j);
-                        return xsph_pair(self, selfv, pj, vel(j, pj), P.inv_h);
-                    },
-                    tl);
-                F3 out;
-                if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
-                    out = selfv;
-                    ++tl.nan;
-                } else {
-                    out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
-                }
-                xo[3 * g + 0] = self.x;
-                xo[3 * g + 1] = self.y;
-                xo[3 * g + 2] = self.z;
-                vo[3 * g + 0] = out.x;
-                vo[3 * g + 1] = out.y;
-                vo[3 * g + 2] = out.z;
-            }
-        }
-        __syncthreads();
-    }
-    for_sparse(G, b, uw, [&](uint32_t g) {
-        tl.slowpart++;
-        const F3 self = ld3(Pc, cap, g);
-        const F3 selfv = vel(g, self);
-        const F3 a = sum27_global(G, b.offset, self, [&](uint32_t j) {
-            const F3 pj = ld3(Pc, cap, j);
-            return xsph_pair(self, selfv, pj, vel(j, pj), P.inv_h);
-        });
+    auto finish = [&](uint32_t g, F3 self, F3 selfv, F3 a) {
         F3 out;
         if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
             out = selfv;
@@ -1209,6 +1185,46 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uin
         vo[3 * g + 0] = out.x;
         vo[3 * g + 1] = out.y;
         vo[3 * g + 2] = out.z;
+    };
+    for (uint32_t u = u0; u < u1; u += us) {
+        const uint2 unit = b.cunits[u];
+        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
+        if (I.ok) {
+            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tp[t] = pos_slot(Pc, cap, g); });
+            __syncthreads();
+            const OwnCur o = own_cur(b, unit);
+            for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
+                const uint32_t g = b.cperm[o.c0 + q];
+                const F3 self = ld3(Pc, cap, g);
+                const F3 selfv = vel(g, self);
+                auto term = [&](float4 el) {
+                    const F3 pj = f3of(el);
+                    return xsph_pair(self, selfv, pj, vel(__float_as_uint(el.w), pj), P.inv_h);
+                };
+                F3 a;
+                if (I.staged) {
+                    a = sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_w, buf, term);
+                } else {
+                    tl.slowpart++;
+                    a = sum27_global(G, b.offset, self, P.cut2_w, [&](uint32_t j) { return pos_slot(Pc, cap, j); },
+                                     term);
+                }
+                finish(g, self, selfv, a);
+            }
+        }
+        __syncthreads();
+    }
+    for_sparse(G, b, uw, [&](uint32_t g) {
+        tl.slowpart++;
+        const F3 self = ld3(Pc, cap, g);
+        const F3 selfv = vel(g, self);
+        const F3 a = sum27_global(
+            G, b.offset, self, P.cut2_w, [&](uint32_t j) { return pos_slot(Pc, cap, j); },
+            [&](float4 el) {
+                const F3 pj = f3of(el);
+                return xsph_pair(self, selfv, pj, vel(__float_as_uint(el.w), pj), P.inv_h);
+            });
+        finish(g, self, selfv, a);
     });
     flush_tally(b.ctr, tl);
 }
@@ -1283,36 +1299,50 @@ __global__ void k_selftest(const float* __restrict__ a, const float* __restrict_
     out[4 * i + 3] = floorf(x / y);
 }
 
+
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-size_t smem_list(const Phys& P) { return (size_t)kBlock * P.stride * sizeof(uint16_t); }
+// Dynamic LDS of each unit kernel: the tile, plus the list rows under
+// construction (neighbour search) or the per-lane candidate buffers (final passes).
+size_t smem_of(int k, uint32_t tile_cap, int stride) {
+    const size_t tile = (size_t)tile_cap * sizeof(float4);
+    switch (k) {
+        case K_NS_LAMBDA: return tile + (size_t)kBlock * stride * sizeof(uint16_t);
+        case K_VORT:
+        case K_CONFINE:
+        case K_XSPH_OUT: return tile + (size_t)kBlock * kCompact * sizeof(uint32_t);
+        default: return tile;
+    }
+}
 
 }  // namespace
 
-int unit_blocks_per_cu(uint32_t tile_cap) {
-    // Allow the largest tile on every unit kernel, then ask the runtime how
-    // many workgroups of the current tile size fit a CU; the two-tile
-    // kernels bound the occupancy.
-    int best = 1 << 30;
-    const size_t lst = (size_t)kBlock * 64 * 2;
+void size_unit_grids(Tuning& T, int num_cu, int stride) {
     struct K {
+        int id;
         const void* f;
-        size_t smem_max, smem;
     } ks[] = {
-        {(const void*)k_ns_lambda, (size_t)kTileCapMax * 16 + lst, (size_t)tile_cap * 16 + lst},
-        {(const void*)k_lambda, (size_t)kTileCapMax * 16 + lst, (size_t)tile_cap * 16 + lst},
-        {(const void*)k_deltap, (size_t)kTileCapMax * 16 + lst, (size_t)tile_cap * 16 + lst},
-        {(const void*)k_vort, (size_t)kTileCapMax * 32, (size_t)tile_cap * 32},
-        {(const void*)k_confine, (size_t)kTileCapMax * 16, (size_t)tile_cap * 16},
-        {(const void*)k_xsph_out, (size_t)kTileCapMax * 32, (size_t)tile_cap * 32},
+        {K_NS_LAMBDA, (const void*)k_ns_lambda}, {K_LAMBDA, (const void*)k_lambda},
+        {K_DELTAP, (const void*)k_deltap},       {K_VORT, (const void*)k_vort},
+        {K_CONFINE, (const void*)k_confine},     {K_XSPH_OUT, (const void*)k_xsph_out},
     };
+    const uint32_t ncu = num_cu > 0 ? (uint32_t)num_cu : 1u;
+    uint32_t lo = 0xFFFFFFFFu;
+    for (int i = 0; i < K_NUM; ++i) T.grid[i] = 0;
     for (auto& k : ks) {
-        (void)hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.smem_max);
+        // allow the largest configuration, then ask how many workgroups of
+        // the current one fit a CU
+        (void)hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)smem_of(k.id, kTileCapMax, 64));
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.f, kBlock, k.smem) != hipSuccess || nb < 1) nb = 1;
-        best = nb < best ? nb : best;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.f, kBlock, smem_of(k.id, T.tile_cap, stride)) !=
+                hipSuccess ||
+            nb < 1)
+            nb = 1;
+        T.grid[k.id] = ncu * (uint32_t)nb < 8u ? 8u : ncu * (uint32_t)nb;
+        lo = T.grid[k.id] < lo ? T.grid[k.id] : lo;
     }
-    return best;
+    T.grid_units = lo;
 }
 
 const char* kernel_name(int id) {
@@ -1345,9 +1375,10 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
                        float* v, uint32_t n, const float lo[3], const float hi[3], int iters, int xsph_on,
                        int vort_on, hipStream_t s, Timer* tm) {
     const uint32_t nb = ceil_div(n > 0 ? n : 1, 256);
-    const uint32_t GU = T.grid_units;
     const uint32_t uw = T.unit_width < G.gx ? T.unit_width : G.gx;
     const uint32_t budget = (T.tile_cap * 3u) / 4u > 0 ? (T.tile_cap * 3u) / 4u : 1u;
+    auto grid = [&](int k) { return dim3(T.grid[k] ? T.grid[k] : T.grid_units); };
+    auto smem = [&](int k) { return smem_of(k, T.tile_cap, P.stride); };
     if (n > 0) {
         Mark m(tm, s, K_PREDICT_HASH);
         hipLaunchKernelGGL(k_predict_hash, dim3(nb), dim3(256), 0, s, x, v, n, G, P.dt, P.g[0], P.g[1], P.g[2], lo[0],
@@ -1374,10 +1405,9 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
         hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, x, v, G, P.dt, P.g[0], P.g[1], P.g[2], b.key, b.tmp,
                            b.offset, b.sidx, b.PA, b.X, b.cap, n, uw, budget, b.units, b.ctr);
     }
-    const size_t s1 = (size_t)T.tile_cap * sizeof(float4);
     {
         Mark m(tm, s, K_NS_LAMBDA);
-        hipLaunchKernelGGL(k_ns_lambda, dim3(GU), dim3(kBlock), s1 + smem_list(P), s, G, P, b, T.tile_cap,
+        hipLaunchKernelGGL(k_ns_lambda, grid(K_NS_LAMBDA), dim3(kBlock), smem(K_NS_LAMBDA), s, G, P, b, T.tile_cap,
                            (const float*)b.PA);
     }
     float* cur = b.PA;
@@ -1385,12 +1415,12 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
     for (int it = 0; it < iters; ++it) {
         if (it > 0) {
             Mark m(tm, s, K_LAMBDA);
-            hipLaunchKernelGGL(k_lambda, dim3(GU), dim3(kBlock), s1 + smem_list(P), s, G, P, b, T.tile_cap,
+            hipLaunchKernelGGL(k_lambda, grid(K_LAMBDA), dim3(kBlock), smem(K_LAMBDA), s, G, P, b, T.tile_cap,
                                (const float*)cur);
         }
         {
             Mark m(tm, s, K_DELTAP);
-            hipLaunchKernelGGL(k_deltap, dim3(GU), dim3(kBlock), s1 + smem_list(P), s, G, P, B, b, T.tile_cap,
+            hipLaunchKernelGGL(k_deltap, grid(K_DELTAP), dim3(kBlock), smem(K_DELTAP), s, G, P, B, b, T.tile_cap,
                                (const float*)cur, nxt);
         }
         float* t = cur;
@@ -1418,16 +1448,18 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
     if (vort_on) {
         {
             Mark m(tm, s, K_VORT);
-            hipLaunchKernelGGL(k_vort, dim3(GU), dim3(kBlock), 2 * s1, s, G, P, b, T.tile_cap, uw, (const float*)cur);
+            hipLaunchKernelGGL(k_vort, grid(K_VORT), dim3(kBlock), smem(K_VORT), s, G, P, b, T.tile_cap, uw,
+                               (const float*)cur);
         }
         Mark m(tm, s, K_CONFINE);
-        hipLaunchKernelGGL(k_confine, dim3(GU), dim3(kBlock), s1, s, G, P, b, T.tile_cap, uw, (const float*)cur);
+        hipLaunchKernelGGL(k_confine, grid(K_CONFINE), dim3(kBlock), smem(K_CONFINE), s, G, P, b, T.tile_cap, uw,
+                           (const float*)cur);
     }
     const float* vs = vort_on ? b.V : nullptr;
     if (xsph_on) {
         Mark m(tm, s, K_XSPH_OUT);
-        hipLaunchKernelGGL(k_xsph_out, dim3(GU), dim3(kBlock), 2 * s1, s, G, P, b, T.tile_cap, uw, (const float*)cur,
-                           vs, x, v);
+        hipLaunchKernelGGL(k_xsph_out, grid(K_XSPH_OUT), dim3(kBlock), smem(K_XSPH_OUT), s, G, P, b, T.tile_cap, uw,
+                           (const float*)cur, vs, x, v);
     } else {
         Mark m(tm, s, K_OUT);
         hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, vs, G.C, x, v);

@@ -52,7 +52,6 @@ struct DeviceGuard {
     }
 };
 
-// pbf/kernel/cuda/detail/pbf_kernel.inl:34-48 on the host (IEEE float)
 // Squared distance beyond which a kernel evaluated with argument `arg`
 // (q = r * arg) is exactly zero: (1/arg)^2 with a 1e-5 relative margin, far
 // above the rounding of r and q, so the pairs skipped add exactly +-0.
@@ -61,6 +60,7 @@ float support_bound2(float arg) {
     return (float)(L * L * (1.0 + 1e-5));
 }
 
+// pbf/kernel/cuda/detail/pbf_kernel.inl:34-48 on the host (IEEE float)
 float wpoly6_host(float r, float inv_h) {
     float q = r * inv_h;
     float alpha = 1.56668147065f * inv_h * inv_h * inv_h;
@@ -146,8 +146,8 @@ void free_timer(pbfx_ctx* c) {
 void free_bufs(pbfx_ctx* c) {
     free_timer(c);
     Bufs& b = c->b;
-    void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.PA,     b.PB,    b.X,      b.V,
-                    b.OM,    b.lam,     b.list,    b.count, b.offset, b.tsum,  b.tprefix, b.units,
+    void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.PA,     b.PB,    b.X,      b.R,
+                    b.OM4,   b.lam,     b.list,    b.count, b.offset, b.tsum,  b.tprefix, b.units,
                     b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
                     b.tflag, b.ctflag,
                     b.ctr,   c->d_inner, c->d_outer, c->d_pts, c->d_nrm, c->d_default_sphere};
@@ -300,8 +300,8 @@ pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const floa
     PBFX_ALLOC(b.PA, 3 * cap);
     PBFX_ALLOC(b.PB, 3 * cap);
     PBFX_ALLOC(b.X, 3 * cap);
-    PBFX_ALLOC(b.V, 3 * cap);
-    PBFX_ALLOC(b.OM, 3 * cap);
+    PBFX_ALLOC(b.R, 2 * cap);
+    PBFX_ALLOC(b.OM4, cap);
     PBFX_ALLOC(b.lam, cap);
     PBFX_ALLOC(b.count, (size_t)G.C + 1);
     PBFX_ALLOC(b.offset, (size_t)G.C + 2);

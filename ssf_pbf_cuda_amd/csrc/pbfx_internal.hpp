@@ -3,7 +3,9 @@
 // Layout in HBM (capacity `cap` particles, C cells):
 //   key, rank, tmp, sorted_index   u32[cap]      hash / histogram rank / scatter / slot->index
 //   PA, PB                         f32[3*cap]    SoA ping-pong of the predicted positions
-//   X, V, OM                       f32[3*cap]    SoA sorted old positions, velocities, vorticity
+//   X                              f32[3*cap]    SoA sorted old positions
+//   R                              f32x4[2*cap]  per slot: (x*, 0), (v, 0) -- the final passes' own record
+//   OM4                            f32x4[cap]    per slot: (omega, |omega|)
 //   lam                            f32[cap]      scaling factor s (= -lambda)
 //   list                           u16[cap*stride] frozen neighbour lists as tile indices,
 //                                                 count in the row's last entry
@@ -91,8 +93,8 @@ struct Bufs {
     float* PA;
     float* PB;
     float* X;
-    float* V;
-    float* OM;
+    float4* R;
+    float4* OM4;
     float* lam;
     uint16_t* list;
     uint32_t* count;

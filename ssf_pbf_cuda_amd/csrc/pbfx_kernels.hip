@@ -851,19 +851,25 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
 // (pbf/interaction/cuda/detail/pbf_contribution.inl:7-57).  Particles are
 // regrouped by final cell (order inside a cell is irrelevant: every sum is
 // per particle) so that each unit's own particles have their 27 cells in the
-// unit's tile.
-__global__ __launch_bounds__(256) void k_cur_hash(const float* __restrict__ Pc, uint32_t cap,
-                                                  const uint32_t* __restrict__ offset, Grid G,
-                                                  uint32_t* __restrict__ ckey, uint32_t* __restrict__ crank,
+// unit's tile.  The same pass packs each slot's record R = (x*, 0), (v, 0)
+// with v = inv_dt*(x* - x_sorted) (updateVelocityCUDA, pbf_update.cu:25-39),
+// so that a particle visited in final-cell order costs one cache line.
+__global__ __launch_bounds__(256) void k_cur_hash(const float* __restrict__ Pc, const float* __restrict__ X,
+                                                  float inv_dt, uint32_t cap, const uint32_t* __restrict__ offset,
+                                                  Grid G, uint32_t* __restrict__ ckey, uint32_t* __restrict__ crank,
                                                   uint32_t* __restrict__ ccount, uint32_t* __restrict__ ctflag,
-                                                  uint32_t lgT) {
+                                                  uint32_t lgT, float4* __restrict__ R) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nl = offset[G.C];
     const bool active = s < nl;
     uint32_t k = 0;
     if (active) {
+        const F3 p = ld3(Pc, cap, s);
+        const F3 vv = mulsv(inv_dt, p - ld3(X, cap, s));
+        R[2 * s] = make_float4(p.x, p.y, p.z, 0.f);
+        R[2 * s + 1] = make_float4(vv.x, vv.y, vv.z, 0.f);
         int gp[3];
-        grid_pos(ld3(Pc, cap, s), G.cw, gp);
+        grid_pos(p, G.cw, gp);
         k = grid_hash(gp[0], gp[1], gp[2], G);
     }
     const uint32_t r = wave_rank(ccount, k, active);
@@ -887,7 +893,6 @@ __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict_
     cperm[slot] = s;
     emit_units(coffset, k, slot, uw, budget, kSparse, cunits, &ctr[CTR_CUNITS]);
 }
-
 
 // Sum over the 27 cells of the staged tile around own column kc (the
 // particle's final cell), per-cell partial sums added in z,y,x order
@@ -970,13 +975,21 @@ __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __rest
 }
 
 // Particles of final-cell runs too sparse for a unit (fewer than kSparse
-// particles in their run of uw cells): one thread each, grid-strided.
+// particles in their run of uw cells): one thread each, in final-cell order
+// so that the lanes of a wave share their neighbourhoods' cache lines, with
+// a contiguous share of that order per XCD.  Sparse runs are contiguous
+// stretches of the order.
 template <class F>
 __device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_t uw, F&& body) {
     const uint32_t nl = b.offset[G.C];
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nl; s += gridDim.x * blockDim.x) {
-        const uint32_t kb = b.ckey[s] & ~(uw - 1u);
-        if (b.coffset[kb + uw] - b.coffset[kb] < kSparse) body(s);
+    uint32_t c0, c1, cs;
+    unit_range((nl + kBlock - 1) / kBlock, c0, c1, cs);
+    for (uint32_t c = c0; c < c1; c += cs) {
+        const uint32_t q = c * kBlock + threadIdx.x;
+        if (q >= nl) continue;
+        const uint32_t g = b.cperm[q];
+        const uint32_t kb = b.ckey[g] & ~(uw - 1u);
+        if (b.coffset[kb + uw] - b.coffset[kb] < kSparse) body(g);
     }
 }
 
@@ -1026,221 +1039,164 @@ __device__ __forceinline__ F3 xsph_pair(F3 self, F3 selfv, F3 pj, F3 vj, float i
     return mulvs(vd, wpoly6(r, inv_h));
 }
 
-// Tile elements of the final passes: position and the particle's sorted slot
-// (as float bits) or, for the confinement, |omega|.
-__device__ __forceinline__ float4 pos_slot(const float* __restrict__ Pc, uint32_t cap, uint32_t g) {
-    return make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], __uint_as_float(g));
+// Tile element of the vorticity and XSPH passes: x*_j and slot j (as float bits).
+__device__ __forceinline__ float4 pos_slot(const float4* __restrict__ R, uint32_t j) {
+    const float4 r = R[2 * j];
+    return make_float4(r.x, r.y, r.z, __uint_as_float(j));
 }
 
-// ------------------------------------------------- K9: velocity + vorticity
-// updateVelocityCUDA (pbf_update.cu:25-39): v = inv_dt*(x* - x_sorted), then
-// calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
-// spiky derivative (or h, when reproducing the reference's argument).
-__global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw,
-                                                 const float* __restrict__ Pc) {
-    __shared__ UnitShared S;
-    extern __shared__ float4 smem[];
-    float4* tp = smem;
-    uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
-    const uint32_t cap = b.cap;
+// Runs body(g) for every own particle of the final-cell work items of this
+// workgroup (tile staged with put(t, j) when it fits), then for the sparse
+// particles.  body(g, tiled) gets tiled = true when the tile is usable.
+template <class PutF, class BodyF>
+__device__ __forceinline__ void final_pass(const Grid& G, const Bufs& b, uint32_t tile_cap, uint32_t uw,
+                                           UnitShared& S, Tally& tl, UnitInfo& I, PutF&& put, BodyF&& body) {
     const uint32_t nU = b.ctr[CTR_CUNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
-    auto vel = [&](uint32_t g, F3 p) -> F3 { return mulsv(P.inv_dt, p - ld3(b.X, cap, g)); };
-    Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
         const uint2 unit = b.cunits[u];
-        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
+        I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
-            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tp[t] = pos_slot(Pc, cap, g); });
+            if (I.staged) load_runs(S, I, put);
             __syncthreads();
             const OwnCur o = own_cur(b, unit);
             for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
-                const uint32_t g = b.cperm[o.c0 + q];
-                const F3 self = ld3(Pc, cap, g);
-                const F3 selfv = vel(g, self);
-                auto term = [&](float4 el) {
-                    const F3 pj = f3of(el);
-                    return vort_pair(self, selfv, pj, vel(__float_as_uint(el.w), pj), P.warg);
-                };
-                F3 w;
-                if (I.staged) {
-                    w = sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_wd, buf, term);
-                } else {
-                    tl.slowpart++;
-                    w = sum27_global(G, b.offset, self, P.cut2_wd, [&](uint32_t j) { return pos_slot(Pc, cap, j); },
-                                     term);
-                }
-                st3(b.OM, cap, g, mulsv(P.m * P.inv_rho0, w));
-                st3(b.V, cap, g, selfv);
+                if (!I.staged) tl.slowpart++;
+                body(b.cperm[o.c0 + q], I.staged);
             }
         }
         __syncthreads();
     }
     for_sparse(G, b, uw, [&](uint32_t g) {
         tl.slowpart++;
-        const F3 self = ld3(Pc, cap, g);
-        const F3 selfv = vel(g, self);
-        const F3 w = sum27_global(
-            G, b.offset, self, P.cut2_wd, [&](uint32_t j) { return pos_slot(Pc, cap, j); },
-            [&](float4 el) {
-                const F3 pj = f3of(el);
-                return vort_pair(self, selfv, pj, vel(__float_as_uint(el.w), pj), P.warg);
-            });
-        st3(b.OM, cap, g, mulsv(P.m * P.inv_rho0, w));
-        st3(b.V, cap, g, selfv);
+        body(g, false);
     });
+}
+
+// ------------------------------------------------- K9: vorticity
+// calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
+// spiky derivative (or h, when reproducing the reference's argument);
+// writes (omega, |omega|).
+__global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw) {
+    __shared__ UnitShared S;
+    extern __shared__ float4 smem[];
+    float4* tp = smem;
+    uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
+    const float4* __restrict__ R = b.R;
+    const float mr = P.m * P.inv_rho0;
+    Tally tl;
+    UnitInfo I;
+    final_pass(
+        G, b, tile_cap, uw, S, tl, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_slot(R, j); },
+        [&](uint32_t g, bool tiled) {
+            const F3 self = f3of(R[2 * g]);
+            const F3 selfv = f3of(R[2 * g + 1]);
+            auto term = [&](float4 el) {
+                return vort_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.warg);
+            };
+            const F3 w = tiled ? sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_wd, buf, term)
+                               : sum27_global(G, b.offset, self, P.cut2_wd, [&](uint32_t j) { return pos_slot(R, j); },
+                                              term);
+            const F3 om = mulsv(mr, w);
+            b.OM4[g] = make_float4(om.x, om.y, om.z, glm_length(om));
+        });
     flush_tally(b.ctr, tl);
 }
 
 // ------------------------------------------------- K10: vorticity confinement
-// addVorticityConfinementCUDA (pbf_vorticity_confinement.cu:192-222), in place on V.
-__global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw,
-                                                    const float* __restrict__ Pc) {
+// addVorticityConfinementCUDA (pbf_vorticity_confinement.cu:192-222), in place
+// on the record's velocity.
+__global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
     uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
-    const uint32_t cap = b.cap;
-    const uint32_t nU = b.ctr[CTR_CUNITS];
-    uint32_t u0, u1, us;
-    unit_range(nU, u0, u1, us);
-    auto pos_w = [&](uint32_t g) {
-        const F3 p = ld3(Pc, cap, g);
-        return make_float4(p.x, p.y, p.z, glm_length(ld3(b.OM, cap, g)));
-    };
-    auto finish = [&](uint32_t g, F3 N, F3 selfw) {
-        if (glm_length(N) > 0.f) {
-            N = mulvs(mulvs(N, P.m), P.inv_rho0);
-            N = divvs(N, glm_length(N));
-            const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, selfw)), P.inv_rho0), P.dtc);
-            st3(b.V, cap, g, ld3(b.V, cap, g) + add);
-        }
+    float4* __restrict__ R = b.R;
+    const float4* __restrict__ OM4 = b.OM4;
+    auto pos_w = [&](uint32_t j) {
+        const float4 r = R[2 * j];
+        return make_float4(r.x, r.y, r.z, OM4[j].w);
     };
     Tally tl;
-    for (uint32_t u = u0; u < u1; u += us) {
-        const uint2 unit = b.cunits[u];
-        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
-        if (I.ok) {
-            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tp[t] = pos_w(g); });
-            __syncthreads();
-            const OwnCur o = own_cur(b, unit);
-            for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
-                const uint32_t g = b.cperm[o.c0 + q];
-                const F3 self = ld3(Pc, cap, g);
-                auto term = [&](float4 el) { return loc_pair(self, f3of(el), el.w, P.warg); };
-                F3 N;
-                if (I.staged) {
-                    N = sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_wd, buf, term);
-                } else {
-                    tl.slowpart++;
-                    N = sum27_global(G, b.offset, self, P.cut2_wd, pos_w, term);
-                }
-                finish(g, N, ld3(b.OM, cap, g));
+    UnitInfo I;
+    final_pass(
+        G, b, tile_cap, uw, S, tl, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_w(j); },
+        [&](uint32_t g, bool tiled) {
+            const F3 self = f3of(R[2 * g]);
+            auto term = [&](float4 el) { return loc_pair(self, f3of(el), el.w, P.warg); };
+            F3 N = tiled ? sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_wd, buf, term)
+                         : sum27_global(G, b.offset, self, P.cut2_wd, pos_w, term);
+            if (glm_length(N) > 0.f) {
+                N = mulvs(mulvs(N, P.m), P.inv_rho0);
+                N = divvs(N, glm_length(N));
+                const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, f3of(OM4[g]))), P.inv_rho0), P.dtc);
+                const F3 nv = f3of(R[2 * g + 1]) + add;
+                R[2 * g + 1] = make_float4(nv.x, nv.y, nv.z, 0.f);
             }
-        }
-        __syncthreads();
-    }
-    for_sparse(G, b, uw, [&](uint32_t g) {
-        tl.slowpart++;
-        const F3 self = ld3(Pc, cap, g);
-        const F3 N = sum27_global(G, b.offset, self, P.cut2_wd, pos_w,
-                                  [&](float4 el) { return loc_pair(self, f3of(el), el.w, P.warg); });
-        finish(g, N, ld3(b.OM, cap, g));
-    });
+        });
     flush_tally(b.ctr, tl);
 }
 
 // ------------------------------------------------- K11: XSPH + write-back
 // calcXSPHCUDA (pbf_xsph.cu:84-113) and updatePosition (pbf_update.cu:16-23):
-// x_out = x*, v_out = v + c*m*inv_rho0*sum unless the sum is NaN.
-// Vs: velocities after confinement, or null to derive them from x* and x.
+// x_out = x*, v_out = v + c*m*inv_rho0*sum unless the sum is NaN, with v the
+// record's velocity (after confinement when that ran).
 __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw,
-                                                     const float* __restrict__ Pc, const float* __restrict__ Vs,
                                                      float* __restrict__ xo, float* __restrict__ vo) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
     uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
-    const uint32_t cap = b.cap;
-    const uint32_t nU = b.ctr[CTR_CUNITS];
-    uint32_t u0, u1, us;
-    unit_range(nU, u0, u1, us);
-    auto vel = [&](uint32_t g, F3 p) -> F3 {
-        if (Vs) return ld3(Vs, cap, g);
-        return mulsv(P.inv_dt, p - ld3(b.X, cap, g));
-    };
+    const float4* __restrict__ R = b.R;
     Tally tl;
-    auto finish = [&](uint32_t g, F3 self, F3 selfv, F3 a) {
-        F3 out;
-        if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
-            out = selfv;
-            ++tl.nan;
-        } else {
-            out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
-        }
-        xo[3 * g + 0] = self.x;
-        xo[3 * g + 1] = self.y;
-        xo[3 * g + 2] = self.z;
-        vo[3 * g + 0] = out.x;
-        vo[3 * g + 1] = out.y;
-        vo[3 * g + 2] = out.z;
-    };
-    for (uint32_t u = u0; u < u1; u += us) {
-        const uint2 unit = b.cunits[u];
-        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
-        if (I.ok) {
-            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tp[t] = pos_slot(Pc, cap, g); });
-            __syncthreads();
-            const OwnCur o = own_cur(b, unit);
-            for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
-                const uint32_t g = b.cperm[o.c0 + q];
-                const F3 self = ld3(Pc, cap, g);
-                const F3 selfv = vel(g, self);
-                auto term = [&](float4 el) {
-                    const F3 pj = f3of(el);
-                    return xsph_pair(self, selfv, pj, vel(__float_as_uint(el.w), pj), P.inv_h);
-                };
-                F3 a;
-                if (I.staged) {
-                    a = sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_w, buf, term);
-                } else {
-                    tl.slowpart++;
-                    a = sum27_global(G, b.offset, self, P.cut2_w, [&](uint32_t j) { return pos_slot(Pc, cap, j); },
-                                     term);
-                }
-                finish(g, self, selfv, a);
+    UnitInfo I;
+    final_pass(
+        G, b, tile_cap, uw, S, tl, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_slot(R, j); },
+        [&](uint32_t g, bool tiled) {
+            const F3 self = f3of(R[2 * g]);
+            const F3 selfv = f3of(R[2 * g + 1]);
+            auto term = [&](float4 el) {
+                return xsph_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.inv_h);
+            };
+            const F3 a = tiled ? sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_w, buf, term)
+                               : sum27_global(G, b.offset, self, P.cut2_w, [&](uint32_t j) { return pos_slot(R, j); },
+                                              term);
+            F3 out;
+            if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
+                out = selfv;
+                ++tl.nan;
+            } else {
+                out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
             }
-        }
-        __syncthreads();
-    }
-    for_sparse(G, b, uw, [&](uint32_t g) {
-        tl.slowpart++;
-        const F3 self = ld3(Pc, cap, g);
-        const F3 selfv = vel(g, self);
-        const F3 a = sum27_global(
-            G, b.offset, self, P.cut2_w, [&](uint32_t j) { return pos_slot(Pc, cap, j); },
-            [&](float4 el) {
-                const F3 pj = f3of(el);
-                return xsph_pair(self, selfv, pj, vel(__float_as_uint(el.w), pj), P.inv_h);
-            });
-        finish(g, self, selfv, a);
-    });
+            xo[3 * g + 0] = self.x;
+            xo[3 * g + 1] = self.y;
+            xo[3 * g + 2] = self.z;
+            vo[3 * g + 0] = out.x;
+            vo[3 * g + 1] = out.y;
+            vo[3 * g + 2] = out.z;
+        });
     flush_tally(b.ctr, tl);
 }
 
 // ------------------------------------------------- K12: write-back without XSPH
-// pbf_solve.cpp:125-145 as shipped: v = inv_dt*(x* - x_sorted), x = x*
-// (or v = the confined velocity when vorticity ran without XSPH).
+// pbf_solve.cpp:125-145 as shipped: v = inv_dt*(x* - x_sorted), x = x*; with
+// R (vorticity ran without XSPH) the record's position and confined velocity.
 __global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float* __restrict__ Pc,
-                                             const float* __restrict__ Vs, uint32_t C, float* __restrict__ xo,
+                                             const float4* __restrict__ R, uint32_t C, float* __restrict__ xo,
                                              float* __restrict__ vo) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nl = b.offset[C];
     if (s >= nl) return;
     const uint32_t cap = b.cap;
-    const F3 p = ld3(Pc, cap, s);
-    const F3 vv = Vs ? ld3(Vs, cap, s) : mulsv(P.inv_dt, p - ld3(b.X, cap, s));
+    F3 p, vv;
+    if (R) {
+        p = f3of(R[2 * s]);
+        vv = f3of(R[2 * s + 1]);
+    } else {
+        p = ld3(Pc, cap, s);
+        vv = mulsv(P.inv_dt, p - ld3(b.X, cap, s));
+    }
     xo[3 * s + 0] = p.x;
     xo[3 * s + 1] = p.y;
     xo[3 * s + 2] = p.z;
@@ -1429,14 +1385,14 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
     }
     if (!vort_on && !xsph_on) {
         Mark m(tm, s, K_OUT);
-        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, (const float*)nullptr, G.C, x,
-                           v);
+        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, (const float4*)nullptr, G.C,
+                           x, v);
         return hipGetLastError();
     }
     {
         Mark m(tm, s, K_CUR_SORT);
-        hipLaunchKernelGGL(k_cur_hash, dim3(nb), dim3(256), 0, s, (const float*)cur, b.cap, b.offset, G, b.ckey,
-                           b.crank, b.ccount, b.ctflag, b.lgT);
+        hipLaunchKernelGGL(k_cur_hash, dim3(nb), dim3(256), 0, s, (const float*)cur, (const float*)b.X, P.inv_dt,
+                           b.cap, b.offset, G, b.ckey, b.crank, b.ccount, b.ctflag, b.lgT, b.R);
         hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, b.nT, b.lgT,
                            G.C);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.ctsum, b.ctprefix, b.nT + 1, b.ctr, 0u);
@@ -1448,21 +1404,19 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
     if (vort_on) {
         {
             Mark m(tm, s, K_VORT);
-            hipLaunchKernelGGL(k_vort, grid(K_VORT), dim3(kBlock), smem(K_VORT), s, G, P, b, T.tile_cap, uw,
-                               (const float*)cur);
+            hipLaunchKernelGGL(k_vort, grid(K_VORT), dim3(kBlock), smem(K_VORT), s, G, P, b, T.tile_cap, uw);
         }
         Mark m(tm, s, K_CONFINE);
-        hipLaunchKernelGGL(k_confine, grid(K_CONFINE), dim3(kBlock), smem(K_CONFINE), s, G, P, b, T.tile_cap, uw,
-                           (const float*)cur);
+        hipLaunchKernelGGL(k_confine, grid(K_CONFINE), dim3(kBlock), smem(K_CONFINE), s, G, P, b, T.tile_cap, uw);
     }
-    const float* vs = vort_on ? b.V : nullptr;
     if (xsph_on) {
         Mark m(tm, s, K_XSPH_OUT);
         hipLaunchKernelGGL(k_xsph_out, grid(K_XSPH_OUT), dim3(kBlock), smem(K_XSPH_OUT), s, G, P, b, T.tile_cap, uw,
-                           (const float*)cur, vs, x, v);
+                           x, v);
     } else {
         Mark m(tm, s, K_OUT);
-        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, vs, G.C, x, v);
+        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, (const float4*)b.R, G.C, x,
+                           v);
     }
     return hipGetLastError();
 }

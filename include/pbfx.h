@@ -179,10 +179,12 @@ typedef struct pbfx_stats {
     uint32_t xsph_nan;         /* particles whose XSPH sum was NaN (velocity kept) */
     uint32_t units;            /* work units (cell blocks) of the last step */
     uint32_t slow_units;       /* units whose neighbourhood exceeded the on-chip tile */
-    uint32_t slow_particles;   /* particles whose final cell left their unit's rows */
+    uint32_t slow_particles;   /* final-pass particles filtered from global memory (sparse runs, large tiles) */
     uint32_t tile_overflow;    /* units beyond 65535 neighbourhood particles (error) */
     uint32_t max_tile;         /* largest neighbourhood tile seen */
     uint32_t steps;            /* steps run by this context */
+    uint32_t final_units;      /* work units of the final passes (final-cell runs) */
+    uint32_t list_overflow;    /* particles with more than 64 in-range final pairs (summed directly) */
 } pbfx_stats;
 
 /* Statistics of the last step (synchronises `stream`). */

@@ -149,7 +149,7 @@ void free_bufs(pbfx_ctx* c) {
     void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.PA,     b.PB,    b.X,      b.R,
                     b.OM4,   b.lam,     b.list,    b.count, b.offset, b.tsum,  b.tprefix, b.units,
                     b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
-                    b.tflag, b.ctflag,
+                    b.tflag, b.ctflag, b.flist, b.fcnt,
                     b.ctr,   c->d_inner, c->d_outer, c->d_pts, c->d_nrm, c->d_default_sphere};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -317,6 +317,8 @@ pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const floa
     PBFX_ALLOC(b.ctsum, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctprefix, (size_t)b.nT + 1);
     PBFX_ALLOC(b.cunits, items);
+    PBFX_ALLOC(b.flist, (size_t)kFinalK * cap);
+    PBFX_ALLOC(b.fcnt, cap);
     PBFX_ALLOC(b.tflag, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctflag, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctr, CTR_NUM);
@@ -472,6 +474,8 @@ pbfx_status pbfx_get_stats(pbfx_ctx* c, void* stream, pbfx_stats* out) {
     out->tile_overflow = h[CTR_OVERFLOW];
     out->max_tile = h[CTR_MAXTILE];
     out->steps = c->steps;
+    out->final_units = h[CTR_CUNITS];
+    out->list_overflow = h[CTR_LISTOVF];
     return PBFX_OK;
 }
 

@@ -15,6 +15,8 @@
 //   units                          uint2[cap+]   work items (first cell key, width | chunk << 5)
 //   ckey, crank, cperm, ccount, coffset, ctsum, ctprefix, cunits
 //                                                 the same sort keyed by each particle's final cell
+//   flist, fcnt                    u32[64*cap], u32[cap]  in-range pairs of the final passes by
+//                                                 final-order position (column-major), counts
 //   ctr                            u32[16]       counters of the last step
 #pragma once
 
@@ -32,7 +34,7 @@ constexpr uint32_t kTileIndexLimit = 65535;  // tile indices are stored as u16
 constexpr uint32_t kScanTile = 4096;      // cells per scan tile
 constexpr uint32_t kChunk = kBlock;       // own particles per work item
 constexpr uint32_t kSparse = 64;          // final-cell runs below this go one thread per particle
-constexpr uint32_t kCompact = 16;         // in-range candidates buffered per lane in the final passes
+constexpr uint32_t kFinalK = 64;          // stored in-range pairs per particle for the final passes
 
 enum Ctr : int {
     CTR_UNITS = 0,
@@ -45,6 +47,7 @@ enum Ctr : int {
     CTR_OVERFLOW,
     CTR_MAXTILE,
     CTR_CUNITS,
+    CTR_LISTOVF,
     CTR_NUM = 16
 };
 
@@ -113,6 +116,8 @@ struct Bufs {
     uint32_t* ctflag;
     uint32_t* ctprefix;
     uint2* cunits;
+    uint32_t* flist;   // final pair lists, column-major [kFinalK][cap]
+    uint32_t* fcnt;    // pairs per final-order position
     uint32_t* ctr;
     uint32_t lgT;
     uint32_t nT;
@@ -133,6 +138,7 @@ enum KernelId : int {
     K_XSPH_OUT,
     K_OUT,
     K_CUR_SORT,
+    K_FINAL_LIST,
     K_NUM
 };
 constexpr int kTimerSlots = 64;

@@ -498,7 +498,7 @@ struct UnitInfo {
 // Per-thread event tallies, summed per wave and added once at kernel end.
 struct Tally {
     uint32_t trunc = 0, slowpart = 0, slowunits = 0, overflow = 0, maxtile = 0, sweepcap = 0, maxsweep = 0,
-             nan = 0;
+             nan = 0, listovf = 0;
 };
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
@@ -516,7 +516,7 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 __device__ void flush_tally(uint32_t* __restrict__ ctr, const Tally& t) {
     const uint32_t tr = wave_sum(t.trunc), sp = wave_sum(t.slowpart), su = wave_sum(t.slowunits),
                    ov = wave_sum(t.overflow), mt = wave_max(t.maxtile), sc = wave_sum(t.sweepcap),
-                   ms = wave_max(t.maxsweep), nn = wave_sum(t.nan);
+                   ms = wave_max(t.maxsweep), nn = wave_sum(t.nan), lo = wave_sum(t.listovf);
     if (lane_id() == 0) {
         if (tr) atomicAdd(&ctr[CTR_TRUNC], tr);
         if (sp) atomicAdd(&ctr[CTR_SLOWPART], sp);
@@ -526,6 +526,7 @@ __device__ void flush_tally(uint32_t* __restrict__ ctr, const Tally& t) {
         if (sc) atomicAdd(&ctr[CTR_SWEEPCAP], sc);
         if (ms) atomicMax(&ctr[CTR_MAXSWEEP], ms);
         if (nn) atomicAdd(&ctr[CTR_XSPHNAN], nn);
+        if (lo) atomicAdd(&ctr[CTR_LISTOVF], lo);
     }
 }
 
@@ -894,57 +895,91 @@ __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict_
     emit_units(coffset, k, slot, uw, budget, kSparse, cunits, &ctr[CTR_CUNITS]);
 }
 
-// Sum over the 27 cells of the staged tile around own column kc (the
-// particle's final cell), per-cell partial sums added in z,y,x order
-// (calcContribution, pbf/interaction/cuda/detail/pbf_contribution.inl:7-57).
-// Pairs with dot(d, d) > cut2 lie outside the kernel support; their terms are
-// exactly zero for finite states and are skipped.  Candidates are filtered in
-// rounds: each lane collects up to kCompact in-range tile indices (tagged
-// with their cell ordinal) in its LDS slot, then the wave evaluates them, so
-// the expensive terms run only on in-range pairs.
-template <class TermF>
-__device__ __forceinline__ F3 sum27_tile(const UnitInfo& I, const UnitShared& S, const float4* __restrict__ tp,
-                                         uint32_t kc, F3 self, float cut2, uint32_t* __restrict__ buf,
-                                         TermF&& term) {
-    F3 sum = mk3(0.f, 0.f, 0.f), cs = mk3(0.f, 0.f, 0.f);
-    uint32_t cur = 0xFFFFFFFFu;
-    uint32_t q = 0;
-    uint32_t e = kc - 1u;
-    uint32_t t = S.tstart[e], tend = S.tstart[e + 1];
-    bool more = true;
-    while (__any(more)) {
-        uint32_t n = 0;
-        while (more && n < kCompact) {
-            if (t < tend) {
-                const F3 d = f3of(tp[t]) - self;
-                if (!(glm_dot(d, d) > cut2)) buf[n++] = t | (q << 16);
-                ++t;
-            } else if (++q < 27u) {
-                e = (q / 3u) * (I.W + 2u) + kc - 1u + (q % 3u);
-                t = S.tstart[e];
-                tend = S.tstart[e + 1];
-            } else {
-                more = false;
+// The 27-cell neighbourhood of a particle's final cell, restated from
+// calcContribution (pbf/interaction/cuda/detail/pbf_contribution.inl:7-57):
+// cells in z,y,x order, slots ascending in each, per-cell partial sums added
+// in that order.  Pairs with dot(d, d) > cut2 lie outside the kernel support
+// and add exactly +-0 for finite states; they are skipped.
+//
+// The final passes run in two stages.  k_final_list filters each particle's
+// candidates once and stores its in-range pairs (bit 31 marks the first pair
+// of each cell); the vorticity, confinement and XSPH passes then walk those
+// lists with every lane busy.  Adding only the cells that have pairs gives
+// the same sums: the skipped cells add +0.
+
+// Emits the in-range tile elements of the staged neighbourhood around own
+// column kc, in order: emit(slot, first pair of its cell).
+template <class EmitF>
+__device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S, const float4* __restrict__ tp,
+                                          uint32_t kc, F3 self, float cut2, EmitF&& emit) {
+    for (uint32_t q = 0; q < 27u; ++q) {
+        const uint32_t e = (q / 3u) * (I.W + 2u) + kc - 1u + (q % 3u);
+        const uint32_t lo = S.tstart[e], hi = S.tstart[e + 1];
+        bool first = true;
+        for (uint32_t t = lo; t < hi; t += 4u) {
+            float4 el[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) el[i] = tp[min(t + i, hi - 1u)];
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) {
+                const F3 d = f3of(el[i]) - self;
+                if (t + i < hi && !(glm_dot(d, d) > cut2)) {
+                    emit(__float_as_uint(el[i].w), first);
+                    first = false;
+                }
             }
-        }
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t ent = buf[k];
-            const uint32_t cq = ent >> 16;
-            if (cq != cur) {
-                acc(sum, cs);
-                cs = mk3(0.f, 0.f, 0.f);
-                cur = cq;
-            }
-            acc(cs, term(tp[ent & 0xFFFFu]));
         }
     }
-    acc(sum, cs);
-    return sum;
 }
 
-// The same sum straight from global memory: per z plane the 9 cell ranges are
-// loaded together, then visited in y, x order; elem(j) builds the tile-format
-// element of slot j.
+// The same from global memory, row by row (dz outer, dy inner), the next
+// row's three cell ranges loaded while the current row is filtered; pos(j)
+// gives x*_j.
+template <class PosF, class EmitF>
+__device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self, float cut2,
+                                            PosF&& pos, EmitF&& emit) {
+    int gp[3];
+    grid_pos(self, G.cw, gp);
+    auto row = [&](int r, uint32_t* st, uint32_t* en) {
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            const uint32_t c = grid_hash(gp[0] + dx - 1, gp[1] + (r % 3) - 1, gp[2] + (r / 3) - 1, G);
+            st[dx] = offset[c];
+            en[dx] = offset[c + 1];
+        }
+    };
+    uint32_t st[3], en[3];
+    row(0, st, en);
+    for (int r = 0; r < 9; ++r) {
+        uint32_t ns[3] = {0, 0, 0}, ne[3] = {0, 0, 0};
+        if (r < 8) row(r + 1, ns, ne);
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            bool first = true;
+            for (uint32_t j = st[dx]; j < en[dx]; j += 4u) {
+                F3 pj[4];
+#pragma unroll
+                for (uint32_t i = 0; i < 4u; ++i) pj[i] = pos(min(j + i, en[dx] - 1u));
+#pragma unroll
+                for (uint32_t i = 0; i < 4u; ++i) {
+                    const F3 d = pj[i] - self;
+                    if (j + i < en[dx] && !(glm_dot(d, d) > cut2)) {
+                        emit(j + i, first);
+                        first = false;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            st[dx] = ns[dx];
+            en[dx] = ne[dx];
+        }
+    }
+}
+
+// Direct sum over the 27 cells from global memory, for particles whose list
+// overflowed: elem(j) builds the pass's element of slot j.
 template <class ElemF, class TermF>
 __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self, float cut2,
                                            ElemF&& elem, TermF&& term) {
@@ -978,7 +1013,7 @@ __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __rest
 // particles in their run of uw cells): one thread each, in final-cell order
 // so that the lanes of a wave share their neighbourhoods' cache lines, with
 // a contiguous share of that order per XCD.  Sparse runs are contiguous
-// stretches of the order.
+// stretches of the order.  body(q, g): position q in that order, slot g.
 template <class F>
 __device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_t uw, F&& body) {
     const uint32_t nl = b.offset[G.C];
@@ -989,11 +1024,11 @@ __device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_
         if (q >= nl) continue;
         const uint32_t g = b.cperm[q];
         const uint32_t kb = b.ckey[g] & ~(uw - 1u);
-        if (b.coffset[kb + uw] - b.coffset[kb] < kSparse) body(g);
+        if (b.coffset[kb + uw] - b.coffset[kb] < kSparse) body(q, g);
     }
 }
 
-// Own particles of a final-cell work item: slots cperm[coffset[kb] + chunk*kChunk ..).
+// Own particles of a final-cell work item: positions coffset[kb] + chunk*kChunk .. of the final-cell order.
 struct OwnCur {
     uint32_t c0, n;
 };
@@ -1010,6 +1045,105 @@ __device__ __forceinline__ uint32_t final_column(const Grid& G, const UnitInfo& 
     int gp[3];
     grid_pos(p, G.cw, gp);
     return (((uint32_t)gp[0] & (G.gx - 1u)) - I.xb) + 1u;
+}
+
+// Tile element of the list build: x*_j and slot j (as float bits).
+__device__ __forceinline__ float4 pos_slot(const float4* __restrict__ R, uint32_t j) {
+    const float4 r = R[2 * j];
+    return make_float4(r.x, r.y, r.z, __uint_as_float(j));
+}
+
+// ------------------------------------------------- K9: final pair lists
+// For each particle (by position q of the final-cell order) the slots of its
+// in-range pairs, column-major: flist[k * cap + q], k < kFinalK; fcnt[q] =
+// the full count (above kFinalK: the passes fall back to the direct sum).
+__global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw) {
+    __shared__ UnitShared S;
+    extern __shared__ float4 smem[];
+    float4* tp = smem;
+    const float4* __restrict__ R = b.R;
+    const uint32_t cap = b.cap;
+    const float cut2 = fmaxf(P.cut2_w, P.cut2_wd);
+    uint32_t* __restrict__ fl = b.flist;
+    Tally tl;
+    UnitInfo I;
+    auto build = [&](uint32_t q, uint32_t g, bool tiled) {
+        const F3 self = f3of(R[2 * g]);
+        uint32_t n = 0;
+        auto emit = [&](uint32_t j, bool first) {
+            if (n < kFinalK) fl[(size_t)n * cap + q] = j | (first ? 0x80000000u : 0u);
+            ++n;
+        };
+        if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, emit);
+        else list_global(G, b.offset, self, cut2, [&](uint32_t j) { return f3of(R[2 * j]); }, emit);
+        b.fcnt[q] = n;
+        tl.listovf += n > kFinalK ? 1u : 0u;
+    };
+    const uint32_t nU = b.ctr[CTR_CUNITS];
+    uint32_t u0, u1, us;
+    unit_range(nU, u0, u1, us);
+    for (uint32_t u = u0; u < u1; u += us) {
+        const uint2 unit = b.cunits[u];
+        I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
+        if (I.ok) {
+            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_slot(R, j); });
+            __syncthreads();
+            const OwnCur o = own_cur(b, unit);
+            for (uint32_t i = threadIdx.x; i < o.n; i += kBlock) {
+                if (!I.staged) tl.slowpart++;
+                build(o.c0 + i, b.cperm[o.c0 + i], I.staged);
+            }
+        }
+        __syncthreads();
+    }
+    for_sparse(G, b, uw, [&](uint32_t q, uint32_t g) {
+        tl.slowpart++;
+        build(q, g, false);
+    });
+    flush_tally(b.ctr, tl);
+}
+
+// Block -> position range of the final-cell order, contiguous per XCD.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x, nx = 8;
+    if (nb < nx) return b;
+    const uint32_t per = nb / nx, rem = nb % nx, x = b % nx;
+    return x * per + min(x, rem) + b / nx;
+}
+
+// Sum over a particle's stored pair list, per-cell partial sums in order;
+// elem(j) builds the pass's element of slot j.
+template <class ElemF, class TermF>
+__device__ __forceinline__ F3 sum_list(const uint32_t* __restrict__ fl, uint32_t cap, uint32_t q, uint32_t n,
+                                       ElemF&& elem, TermF&& term) {
+    F3 sum = mk3(0.f, 0.f, 0.f), cs = mk3(0.f, 0.f, 0.f);
+    for (uint32_t k = 0; k < n; k += 4u) {
+        uint32_t e[4];
+        float4 el[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) e[i] = fl[(size_t)min(k + i, n - 1u) * cap + q];
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(e[i] & 0x7FFFFFFFu);
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i)
+            if (k + i < n) {
+                if (e[i] >> 31) {
+                    acc(sum, cs);
+                    cs = mk3(0.f, 0.f, 0.f);
+                }
+                acc(cs, term(el[i]));
+            }
+    }
+    acc(sum, cs);
+    return sum;
+}
+
+// Visits position q of the final-cell order with its slot and stored pair count.
+template <class F>
+__device__ __forceinline__ void for_final(const Grid& G, const Bufs& b, F&& body) {
+    const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
+    if (q >= b.offset[G.C]) return;
+    body(q, b.cperm[q], b.fcnt[q]);
 }
 
 // vorticity pair term (pbf_vorticity_confinement.cu:15-31)
@@ -1039,147 +1173,87 @@ __device__ __forceinline__ F3 xsph_pair(F3 self, F3 selfv, F3 pj, F3 vj, float i
     return mulvs(vd, wpoly6(r, inv_h));
 }
 
-// Tile element of the vorticity and XSPH passes: x*_j and slot j (as float bits).
-__device__ __forceinline__ float4 pos_slot(const float4* __restrict__ R, uint32_t j) {
-    const float4 r = R[2 * j];
-    return make_float4(r.x, r.y, r.z, __uint_as_float(j));
-}
-
-// Runs body(g) for every own particle of the final-cell work items of this
-// workgroup (tile staged with put(t, j) when it fits), then for the sparse
-// particles.  body(g, tiled) gets tiled = true when the tile is usable.
-template <class PutF, class BodyF>
-__device__ __forceinline__ void final_pass(const Grid& G, const Bufs& b, uint32_t tile_cap, uint32_t uw,
-                                           UnitShared& S, Tally& tl, UnitInfo& I, PutF&& put, BodyF&& body) {
-    const uint32_t nU = b.ctr[CTR_CUNITS];
-    uint32_t u0, u1, us;
-    unit_range(nU, u0, u1, us);
-    for (uint32_t u = u0; u < u1; u += us) {
-        const uint2 unit = b.cunits[u];
-        I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
-        if (I.ok) {
-            if (I.staged) load_runs(S, I, put);
-            __syncthreads();
-            const OwnCur o = own_cur(b, unit);
-            for (uint32_t q = threadIdx.x; q < o.n; q += kBlock) {
-                if (!I.staged) tl.slowpart++;
-                body(b.cperm[o.c0 + q], I.staged);
-            }
-        }
-        __syncthreads();
-    }
-    for_sparse(G, b, uw, [&](uint32_t g) {
-        tl.slowpart++;
-        body(g, false);
+// ------------------------------------------------- K10: vorticity
+// calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
+// spiky derivative (or h, when reproducing the reference's argument);
+// writes (omega, |omega|).  Elements: (x*_j, slot j).
+__global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b) {
+    const float4* __restrict__ R = b.R;
+    const float mr = P.m * P.inv_rho0;
+    for_final(G, b, [&](uint32_t q, uint32_t g, uint32_t n) {
+        const F3 self = f3of(R[2 * g]);
+        const F3 selfv = f3of(R[2 * g + 1]);
+        auto elem = [&](uint32_t j) { return pos_slot(R, j); };
+        auto term = [&](float4 el) {
+            return vort_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.warg);
+        };
+        const F3 w = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
+                                  : sum27_global(G, b.offset, self, P.cut2_wd, elem, term);
+        const F3 om = mulsv(mr, w);
+        b.OM4[g] = make_float4(om.x, om.y, om.z, glm_length(om));
     });
 }
 
-// ------------------------------------------------- K9: vorticity
-// calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
-// spiky derivative (or h, when reproducing the reference's argument);
-// writes (omega, |omega|).
-__global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw) {
-    __shared__ UnitShared S;
-    extern __shared__ float4 smem[];
-    float4* tp = smem;
-    uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
-    const float4* __restrict__ R = b.R;
-    const float mr = P.m * P.inv_rho0;
-    Tally tl;
-    UnitInfo I;
-    final_pass(
-        G, b, tile_cap, uw, S, tl, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_slot(R, j); },
-        [&](uint32_t g, bool tiled) {
-            const F3 self = f3of(R[2 * g]);
-            const F3 selfv = f3of(R[2 * g + 1]);
-            auto term = [&](float4 el) {
-                return vort_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.warg);
-            };
-            const F3 w = tiled ? sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_wd, buf, term)
-                               : sum27_global(G, b.offset, self, P.cut2_wd, [&](uint32_t j) { return pos_slot(R, j); },
-                                              term);
-            const F3 om = mulsv(mr, w);
-            b.OM4[g] = make_float4(om.x, om.y, om.z, glm_length(om));
-        });
-    flush_tally(b.ctr, tl);
-}
-
-// ------------------------------------------------- K10: vorticity confinement
+// ------------------------------------------------- K11: vorticity confinement
 // addVorticityConfinementCUDA (pbf_vorticity_confinement.cu:192-222), in place
-// on the record's velocity.
-__global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw) {
-    __shared__ UnitShared S;
-    extern __shared__ float4 smem[];
-    float4* tp = smem;
-    uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
+// on the record's velocity.  Elements: (x*_j, |omega_j|).
+__global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b) {
     float4* __restrict__ R = b.R;
     const float4* __restrict__ OM4 = b.OM4;
-    auto pos_w = [&](uint32_t j) {
-        const float4 r = R[2 * j];
-        return make_float4(r.x, r.y, r.z, OM4[j].w);
-    };
-    Tally tl;
-    UnitInfo I;
-    final_pass(
-        G, b, tile_cap, uw, S, tl, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_w(j); },
-        [&](uint32_t g, bool tiled) {
-            const F3 self = f3of(R[2 * g]);
-            auto term = [&](float4 el) { return loc_pair(self, f3of(el), el.w, P.warg); };
-            F3 N = tiled ? sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_wd, buf, term)
-                         : sum27_global(G, b.offset, self, P.cut2_wd, pos_w, term);
-            if (glm_length(N) > 0.f) {
-                N = mulvs(mulvs(N, P.m), P.inv_rho0);
-                N = divvs(N, glm_length(N));
-                const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, f3of(OM4[g]))), P.inv_rho0), P.dtc);
-                const F3 nv = f3of(R[2 * g + 1]) + add;
-                R[2 * g + 1] = make_float4(nv.x, nv.y, nv.z, 0.f);
-            }
-        });
-    flush_tally(b.ctr, tl);
+    for_final(G, b, [&](uint32_t q, uint32_t g, uint32_t n) {
+        const F3 self = f3of(R[2 * g]);
+        auto elem = [&](uint32_t j) {
+            const float4 r = R[2 * j];
+            return make_float4(r.x, r.y, r.z, OM4[j].w);
+        };
+        auto term = [&](float4 el) { return loc_pair(self, f3of(el), el.w, P.warg); };
+        F3 N = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
+                            : sum27_global(G, b.offset, self, P.cut2_wd, elem, term);
+        if (glm_length(N) > 0.f) {
+            N = mulvs(mulvs(N, P.m), P.inv_rho0);
+            N = divvs(N, glm_length(N));
+            const F3 add = mulvs(divvs(mulsv(P.vc, glm_cross(N, f3of(OM4[g]))), P.inv_rho0), P.dtc);
+            const F3 nv = f3of(R[2 * g + 1]) + add;
+            R[2 * g + 1] = make_float4(nv.x, nv.y, nv.z, 0.f);
+        }
+    });
 }
 
-// ------------------------------------------------- K11: XSPH + write-back
+// ------------------------------------------------- K12: XSPH + write-back
 // calcXSPHCUDA (pbf_xsph.cu:84-113) and updatePosition (pbf_update.cu:16-23):
 // x_out = x*, v_out = v + c*m*inv_rho0*sum unless the sum is NaN, with v the
 // record's velocity (after confinement when that ran).
-__global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw,
-                                                     float* __restrict__ xo, float* __restrict__ vo) {
-    __shared__ UnitShared S;
-    extern __shared__ float4 smem[];
-    float4* tp = smem;
-    uint32_t* buf = reinterpret_cast<uint32_t*>(smem + tile_cap) + threadIdx.x * kCompact;
+__global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, float* __restrict__ xo,
+                                                     float* __restrict__ vo) {
     const float4* __restrict__ R = b.R;
     Tally tl;
-    UnitInfo I;
-    final_pass(
-        G, b, tile_cap, uw, S, tl, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_slot(R, j); },
-        [&](uint32_t g, bool tiled) {
-            const F3 self = f3of(R[2 * g]);
-            const F3 selfv = f3of(R[2 * g + 1]);
-            auto term = [&](float4 el) {
-                return xsph_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.inv_h);
-            };
-            const F3 a = tiled ? sum27_tile(I, S, tp, final_column(G, I, self), self, P.cut2_w, buf, term)
-                               : sum27_global(G, b.offset, self, P.cut2_w, [&](uint32_t j) { return pos_slot(R, j); },
-                                              term);
-            F3 out;
-            if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
-                out = selfv;
-                ++tl.nan;
-            } else {
-                out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
-            }
-            xo[3 * g + 0] = self.x;
-            xo[3 * g + 1] = self.y;
-            xo[3 * g + 2] = self.z;
-            vo[3 * g + 0] = out.x;
-            vo[3 * g + 1] = out.y;
-            vo[3 * g + 2] = out.z;
-        });
+    for_final(G, b, [&](uint32_t q, uint32_t g, uint32_t n) {
+        const F3 self = f3of(R[2 * g]);
+        const F3 selfv = f3of(R[2 * g + 1]);
+        auto elem = [&](uint32_t j) { return pos_slot(R, j); };
+        auto term = [&](float4 el) {
+            return xsph_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.inv_h);
+        };
+        const F3 a = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
+                                  : sum27_global(G, b.offset, self, P.cut2_w, elem, term);
+        F3 out;
+        if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
+            out = selfv;
+            ++tl.nan;
+        } else {
+            out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
+        }
+        xo[3 * g + 0] = self.x;
+        xo[3 * g + 1] = self.y;
+        xo[3 * g + 2] = self.z;
+        vo[3 * g + 0] = out.x;
+        vo[3 * g + 1] = out.y;
+        vo[3 * g + 2] = out.z;
+    });
     flush_tally(b.ctr, tl);
 }
 
-// ------------------------------------------------- K12: write-back without XSPH
+// ------------------------------------------------- K13: write-back without XSPH
 // pbf_solve.cpp:125-145 as shipped: v = inv_dt*(x* - x_sorted), x = x*; with
 // R (vorticity ran without XSPH) the record's position and confined velocity.
 __global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float* __restrict__ Pc,
@@ -1259,14 +1333,11 @@ __global__ void k_selftest(const float* __restrict__ a, const float* __restrict_
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // Dynamic LDS of each unit kernel: the tile, plus the list rows under
-// construction (neighbour search) or the per-lane candidate buffers (final passes).
+// construction in the neighbour search.
 size_t smem_of(int k, uint32_t tile_cap, int stride) {
     const size_t tile = (size_t)tile_cap * sizeof(float4);
     switch (k) {
         case K_NS_LAMBDA: return tile + (size_t)kBlock * stride * sizeof(uint16_t);
-        case K_VORT:
-        case K_CONFINE:
-        case K_XSPH_OUT: return tile + (size_t)kBlock * kCompact * sizeof(uint32_t);
         default: return tile;
     }
 }
@@ -1279,8 +1350,7 @@ void size_unit_grids(Tuning& T, int num_cu, int stride) {
         const void* f;
     } ks[] = {
         {K_NS_LAMBDA, (const void*)k_ns_lambda}, {K_LAMBDA, (const void*)k_lambda},
-        {K_DELTAP, (const void*)k_deltap},       {K_VORT, (const void*)k_vort},
-        {K_CONFINE, (const void*)k_confine},     {K_XSPH_OUT, (const void*)k_xsph_out},
+        {K_DELTAP, (const void*)k_deltap},       {K_FINAL_LIST, (const void*)k_final_list},
     };
     const uint32_t ncu = num_cu > 0 ? (uint32_t)num_cu : 1u;
     uint32_t lo = 0xFFFFFFFFu;
@@ -1305,7 +1375,7 @@ const char* kernel_name(int id) {
     static const char* names[K_NUM] = {"k_predict_hash", "k_scan_tiles", "k_scan_cells", "k_scatter",
                                        "k_reorder",      "k_ns_lambda",  "k_lambda",     "k_deltap",
                                        "k_vort",         "k_confine",    "k_xsph_out",   "k_out",
-                                       "k_cur_sort"};
+                                       "k_cur_sort",     "k_final_list"};
     return (id >= 0 && id < K_NUM) ? names[id] : "?";
 }
 
@@ -1401,18 +1471,22 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
         hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, G, b.ckey, b.crank, b.coffset,
                            b.cperm, uw, budget, b.cunits, b.ctr);
     }
+    {
+        Mark m(tm, s, K_FINAL_LIST);
+        hipLaunchKernelGGL(k_final_list, grid(K_FINAL_LIST), dim3(kBlock), smem(K_FINAL_LIST), s, G, P, b,
+                           T.tile_cap, uw);
+    }
     if (vort_on) {
         {
             Mark m(tm, s, K_VORT);
-            hipLaunchKernelGGL(k_vort, grid(K_VORT), dim3(kBlock), smem(K_VORT), s, G, P, b, T.tile_cap, uw);
+            hipLaunchKernelGGL(k_vort, dim3(nb), dim3(kBlock), 0, s, G, P, b);
         }
         Mark m(tm, s, K_CONFINE);
-        hipLaunchKernelGGL(k_confine, grid(K_CONFINE), dim3(kBlock), smem(K_CONFINE), s, G, P, b, T.tile_cap, uw);
+        hipLaunchKernelGGL(k_confine, dim3(nb), dim3(kBlock), 0, s, G, P, b);
     }
     if (xsph_on) {
         Mark m(tm, s, K_XSPH_OUT);
-        hipLaunchKernelGGL(k_xsph_out, grid(K_XSPH_OUT), dim3(kBlock), smem(K_XSPH_OUT), s, G, P, b, T.tile_cap, uw,
-                           x, v);
+        hipLaunchKernelGGL(k_xsph_out, dim3(nb), dim3(kBlock), 0, s, G, P, b, x, v);
     } else {
         Mark m(tm, s, K_OUT);
         hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, (const float4*)b.R, G.C, x,

@@ -30,8 +30,8 @@ ALG_BYTES = {
     "k_predict_hash": 48,   # x 12, v 12 -> hash 4 + (x* 12 of the reference's predict) ; rank 4, histogram read 4
     "k_scatter": 16,        # hash 4, rank 4 -> sorted index 4, sorted hash 4
     "k_reorder": 56,        # index 4, x* 12, x 12 -> x*s 12, xs 12 ; cell ranges 4
-    "k_ns_lambda": 16,      # x* 12 -> lambda 4
-    "k_lambda": 16,
+    "k_ns": 12,             # x* 12 (the lists themselves are not compulsory traffic)
+    "k_lambda": 16,         # x* 12 -> lambda 4, once per Jacobi iteration
     "k_deltap": 28,         # x* 12, lambda 4 -> x*' 12
     "k_vort": 48,           # x* 12, xs 12 -> v 12, omega 12
     "k_confine": 48,        # x* 12, omega 12, v 12 -> v 12

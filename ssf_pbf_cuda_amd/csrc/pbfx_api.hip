@@ -123,7 +123,7 @@ Phys make_phys(const pbfx_params& p) {
     P.vc = p.vc_parameter;
     P.warg = p.vort_h_as_inv_h ? P.h : P.inv_h;
     P.cap = p.max_pair - 1;
-    P.stride = p.max_pair <= 32 ? 32 : 64;
+    P.stride = P.cap;
     P.max_sweeps = p.max_sweeps;
     P.outer_as_inner = p.outer_as_inner;
     P.cut2_w = support_bound2(P.inv_h);
@@ -146,8 +146,8 @@ void free_timer(pbfx_ctx* c) {
 void free_bufs(pbfx_ctx* c) {
     free_timer(c);
     Bufs& b = c->b;
-    void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.PA,     b.PB,    b.X,      b.R,
-                    b.OM4,   b.lam,     b.list,    b.count, b.offset, b.tsum,  b.tprefix, b.units,
+    void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.Pf,     b.Q,     b.X,      b.R,
+                    b.OM4,   b.ncnt,    b.nlist,   b.count, b.offset, b.tsum,  b.tprefix, b.units,
                     b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
                     b.tflag, b.ctflag, b.flist, b.fcnt,
                     b.ctr,   c->d_inner, c->d_outer, c->d_pts, c->d_nrm, c->d_default_sphere};
@@ -167,17 +167,17 @@ pbfx_status hip_fail(pbfx_ctx* c, hipError_t e) {
 }
 
 pbfx_status alloc_list(pbfx_ctx* c, uint32_t stride) {
-    if (c->b.list && c->list_stride >= stride) return PBFX_OK;
-    if (c->b.list) (void)hipFree(c->b.list);
-    c->b.list = nullptr;
-    hipError_t e = dalloc(&c->b.list, (size_t)c->capacity * stride);
+    if (c->b.nlist && c->list_stride >= stride) return PBFX_OK;
+    if (c->b.nlist) (void)hipFree(c->b.nlist);
+    c->b.nlist = nullptr;
+    hipError_t e = dalloc(&c->b.nlist, (size_t)c->capacity * stride);
     if (e != hipSuccess) return hip_fail(c, e);
     c->list_stride = stride;
     return PBFX_OK;
 }
 
 void set_grid_units(pbfx_ctx* c) {
-    size_unit_grids(c->T, c->num_cu, make_phys(c->params).stride);
+    size_unit_grids(c->T, c->num_cu);
     c->blocks_per_cu = (int)(c->T.grid_units / (uint32_t)(c->num_cu > 0 ? c->num_cu : 1));
 }
 
@@ -297,12 +297,12 @@ pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const floa
     PBFX_ALLOC(b.rank, cap);
     PBFX_ALLOC(b.tmp, cap);
     PBFX_ALLOC(b.sidx, cap);
-    PBFX_ALLOC(b.PA, 3 * cap);
-    PBFX_ALLOC(b.PB, 3 * cap);
+    PBFX_ALLOC(b.Pf, cap);
+    PBFX_ALLOC(b.Q, cap);
     PBFX_ALLOC(b.X, 3 * cap);
     PBFX_ALLOC(b.R, 2 * cap);
     PBFX_ALLOC(b.OM4, cap);
-    PBFX_ALLOC(b.lam, cap);
+    PBFX_ALLOC(b.ncnt, cap);
     PBFX_ALLOC(b.count, (size_t)G.C + 1);
     PBFX_ALLOC(b.offset, (size_t)G.C + 2);
     PBFX_ALLOC(b.tsum, (size_t)b.nT + 1);

@@ -2,13 +2,13 @@
 //
 // Layout in HBM (capacity `cap` particles, C cells):
 //   key, rank, tmp, sorted_index   u32[cap]      hash / histogram rank / scatter / slot->index
-//   PA, PB                         f32[3*cap]    SoA ping-pong of the predicted positions
+//   Pf                             f32x4[cap]    predicted positions x* (w unused)
+//   Q                              f32x4[cap]    (x*, s) of the current Jacobi iteration (s = -lambda)
 //   X                              f32[3*cap]    SoA sorted old positions
 //   R                              f32x4[2*cap]  per slot: (x*, 0), (v, 0) -- the final passes' own record
 //   OM4                            f32x4[cap]    per slot: (omega, |omega|)
-//   lam                            f32[cap]      scaling factor s (= -lambda)
-//   list                           u16[cap*stride] frozen neighbour lists as tile indices,
-//                                                 count in the row's last entry
+//   nlist, ncnt                    u32[cap*kcap], u32[cap]  frozen neighbour lists as sorted
+//                                                 slots, column-major (entry k of slot i at k*cap + i)
 //   count                          u32[C+1]      cell histogram (cell C = deleted)
 //   offset                         u32[C+2]      exclusive scan: cell c owns [off[c], off[c+1])
 //   tsum, tflag, tprefix           u32[nT+1]     per-4096-cell-tile totals, occupied flags, scan
@@ -71,7 +71,7 @@ struct Phys {
     float g[3];
     float xsph_c, vc, warg; // warg: inv_h, or h to reproduce the vorticity argument mix-up
     int32_t cap;            // neighbours kept per particle (max_pair - 1)
-    int32_t stride;         // u16 per list row
+    int32_t stride;         // list entries allocated per particle (= cap)
     int32_t max_sweeps;
     int32_t outer_as_inner;
     float cut2_w, cut2_wd;  // squared support bounds of W (arg inv_h) and of W' (arg warg): pairs beyond add 0
@@ -93,13 +93,13 @@ struct Bufs {
     uint32_t* rank;
     uint32_t* tmp;
     uint32_t* sidx;
-    float* PA;
-    float* PB;
+    float4* Pf;
+    float4* Q;
     float* X;
     float4* R;
     float4* OM4;
-    float* lam;
-    uint16_t* list;
+    uint32_t* nlist;
+    uint32_t* ncnt;
     uint32_t* count;
     uint32_t* offset;
     uint32_t* tsum;
@@ -130,7 +130,7 @@ enum KernelId : int {
     K_SCAN_CELLS,
     K_SCATTER,
     K_REORDER,
-    K_NS_LAMBDA,
+    K_NS,
     K_LAMBDA,
     K_DELTAP,
     K_VORT,
@@ -167,7 +167,7 @@ hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, co
 hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uint32_t* end, hipStream_t s);
 hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s);
 hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t n, hipStream_t s);
-// Fills T.grid[] (and T.grid_units) for num_cu compute units and list rows of `stride` entries.
-void size_unit_grids(Tuning& T, int num_cu, int stride);
+// Fills T.grid[] (and T.grid_units) for num_cu compute units.
+void size_unit_grids(Tuning& T, int num_cu);
 
 }  // namespace pbfx

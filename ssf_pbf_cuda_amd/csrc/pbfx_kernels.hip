@@ -1,15 +1,17 @@
 // pbfx_kernels.hip -- HIP kernels of the PBF solver step for gfx950 (CDNA4).
 //
-// One step = 5 bookkeeping kernels (predict+hash+histogram, two-level scan,
-// scatter, stable reorder) followed by persistent work-unit kernels that
-// stage the 27-cell neighbourhood of a block of cells in LDS:
-//   k_ns_lambda  neighbour lists (frozen, capped) + lambda of iteration 0
-//   k_deltap     delta-p with tensile correction + boundary response + apply
-//   k_lambda     lambda of iterations 1..I-1
-//   k_vort       velocity + vorticity            (vort_on)
-//   k_confine    vorticity confinement           (vort_on)
-//   k_xsph_out   XSPH + AoS write-back           (xsph_on)
-//   k_out        AoS write-back                  (otherwise)
+// One step = bookkeeping kernels (predict+hash+histogram, two-level scan,
+// scatter, stable reorder), then:
+//   k_ns          neighbour lists (frozen, capped), per work unit of cells
+//                 whose 27-cell neighbourhood is staged in LDS
+//   k_lambda      lambda over the lists, one thread per particle    (x I)
+//   k_deltap      delta-p + tensile correction + boundary response  (x I)
+//   k_cur_*       regrouping by final cell                (vort_on or xsph_on)
+//   k_final_list  in-range pairs around each final cell, per work unit
+//   k_vort        vorticity over the final lists                     (vort_on)
+//   k_confine     vorticity confinement                              (vort_on)
+//   k_xsph_out    XSPH + AoS write-back                              (xsph_on)
+//   k_out         AoS write-back                                     (otherwise)
 // All kernels are HBM/LDS-bound integer or FP32 vector work; nothing is a
 // contraction, so MFMA is not used.  Arithmetic follows the reference's
 // operation order with IEEE division/sqrt and explicit fmaf (the library is
@@ -447,7 +449,7 @@ __device__ __forceinline__ void emit_units(const uint32_t* __restrict__ offset, 
 // ------------------------------------------------- K5: stable reorder
 // Makes the in-cell order ascending by original index (the stable order of
 // the reference's radix-sort paths, pbf_neighbor_search.cpp:112-131), gathers
-// x* and x into SoA (reorderDataCUDA, pbf_neighbor_search_util.cu:143-154)
+// x* (as float4) and x (SoA) (reorderDataCUDA, pbf_neighbor_search_util.cu:143-154)
 // and emits the work units: each run of up to `uw` cells along x is split into
 // narrower units where it is dense, so that a unit's 27-cell neighbourhood
 // fits the on-chip tile.
@@ -455,7 +457,7 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
                                                  float dt, float g0, float g1, float g2,
                                                  const uint32_t* __restrict__ key, const uint32_t* __restrict__ tmp,
                                                  const uint32_t* __restrict__ offset, uint32_t* __restrict__ sidx,
-                                                 float* __restrict__ PA, float* __restrict__ X, uint32_t cap,
+                                                 float4* __restrict__ Pf, float* __restrict__ X, uint32_t cap,
                                                  uint32_t n, uint32_t uw, uint32_t budget, uint2* __restrict__ units,
                                                  uint32_t* __restrict__ ctr) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
         float vs = __builtin_fmaf(gg[c], dt, v[3 * i + c]);
         q[c] = __builtin_fmaf(vs, dt, xo[c]);
     }
-    st3(PA, cap, f, mk3(q[0], q[1], q[2]));
+    Pf[f] = make_float4(q[0], q[1], q[2], 0.f);
     st3(X, cap, f, mk3(xo[0], xo[1], xo[2]));
     if (k < G.C) emit_units(offset, k, f, uw, budget, 0u, units, &ctr[CTR_UNITS]);
 }
@@ -617,17 +619,6 @@ __device__ __forceinline__ void load_runs(const UnitShared& S, const UnitInfo& I
     }
 }
 
-// tile index -> sorted slot (largest cell e with tstart[e] <= t)
-__device__ __forceinline__ uint32_t tile_to_global(const UnitShared& S, uint32_t ncell, uint32_t t) {
-    uint32_t lo = 0, hi = ncell - 1u;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1u) >> 1;
-        if (S.tstart[mid] <= t) lo = mid;
-        else hi = mid - 1u;
-    }
-    return S.gstart[lo] + (t - S.tstart[lo]);
-}
-
 // cell column (1..W) of an own particle's tile index
 __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, uint32_t t) {
     const uint32_t base = 4u * (W + 2u);
@@ -642,53 +633,30 @@ __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, 
 
 __device__ __forceinline__ F3 f3of(float4 f) { return mk3(f.x, f.y, f.z); }
 
-// Visits a particle's frozen list in order: the row is read as uint4 chunks
-// of 8 entries held in registers (the count is the row's last entry).
-template <class F>
-__device__ __forceinline__ void for_list(const uint16_t* __restrict__ list, uint32_t g, int stride, F&& f) {
-    const uint4* row = reinterpret_cast<const uint4*>(list + (size_t)g * stride);
-    const int cnt = (int)(row[stride / 8 - 1].w >> 16);
-    for (int c0 = 0; c0 < cnt; c0 += 8) {
-        const uint4 w = row[c0 >> 3];
-        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (c0 + j < cnt) f((ws[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu);
-    }
+// Block -> index range, contiguous per XCD (workgroup b runs on XCD b % 8),
+// so that neighbouring slots share an L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x, nx = 8;
+    if (nb < nx) return b;
+    const uint32_t per = nb / nx, rem = nb % nx, x = b % nx;
+    return x * per + min(x, rem) + b / nx;
 }
 
-// lambda of one particle (calcScalingFactorCUDA memo form, pbf_constraint.cu:168-209):
-// `visit(fn)` calls fn(neighbour position) for the listed neighbours in order.
-template <class VisitF>
-__device__ __forceinline__ float lambda_of(F3 self, const Phys& P, VisitF&& visit) {
-    float ks = 0.f, kg2s = 0.f;
-    F3 kgs = mk3(0.f, 0.f, 0.f);
-    visit([&](F3 pj) {
-        float W;
-        F3 kg;
-        pair_terms(self, pj, P, W, kg);
-        const float kg2 = fdot(kg, kg);
-        ks += W;
-        acc(kgs, kg);
-        kg2s += kg2;
-    });
-    const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
-    const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
-    return Cc / __builtin_fmaf(P.m2r2, Gs, P.relax);
-}
-
-// ------------------------------------------------- K6: neighbour lists + lambda_0
+// ------------------------------------------------- K6: neighbour lists
 // detectNeighborsCUDA (pbf_neighbor_search_util.cu:361-482): per particle the
 // 27 cells in z,y,x order, slots ascending, keep r2 < h^2 while fewer than
-// cap are kept; the list is frozen for all Jacobi iterations.  Entries are
-// tile indices of the unit's neighbourhood.  Then lambda for iteration 0.
-__global__ __launch_bounds__(kBlock) void k_ns_lambda(Grid G, Phys P, Bufs b, uint32_t tile_cap,
-                                                      const float* __restrict__ Pc) {
+// cap are kept; the list is frozen for all Jacobi iterations.  Stored as
+// sorted slots, column-major: nlist[k * cap_particles + slot], with the
+// count in ncnt[slot].  Candidates come from the unit's staged tile, four
+// at a time.
+__global__ __launch_bounds__(kBlock) void k_ns(Grid G, Phys P, Bufs b, uint32_t tile_cap) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tile = smem;
-    uint16_t* lst = reinterpret_cast<uint16_t*>(smem + tile_cap);
+    const float4* __restrict__ Pf = b.Pf;
+    uint32_t* __restrict__ nl = b.nlist;
     const uint32_t cap = b.cap;
+    const uint32_t ncap = (uint32_t)P.cap;
     const uint32_t nU = b.ctr[CTR_UNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
@@ -696,129 +664,113 @@ __global__ __launch_bounds__(kBlock) void k_ns_lambda(Grid G, Phys P, Bufs b, ui
     for (uint32_t u = u0; u < u1; u += us) {
         const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, tl);
         if (I.ok) {
-            if (I.staged)
-                load_runs(S, I, [&](uint32_t t, uint32_t g) {
-                    tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], 0.f);
-                });
+            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tile[t] = Pf[g]; });
             __syncthreads();
-            uint16_t* my = lst + threadIdx.x * P.stride;
             for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
                 const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
                 const uint32_t kc = own_column(S, I.W, ts);
-                auto pos = [&](uint32_t t) -> F3 {
-                    if (I.staged) return f3of(tile[t]);
-                    return ld3(Pc, cap, tile_to_global(S, I.ncell, t));
-                };
-                const F3 self = pos(ts);
-                int cnt = 0;
+                const F3 self = f3of(I.staged ? tile[ts] : Pf[g]);
+                uint32_t cnt = 0;
                 bool trunc = false;
-                for (uint32_t r = 0; r < 9u && !trunc; ++r) {
-                    const uint32_t e0 = r * (I.W + 2u) + kc - 1u;
-                    const uint32_t lo = S.tstart[e0], hi = S.tstart[e0 + 3u];
-                    for (uint32_t t = lo; t < hi; ++t) {
-                        const F3 d = self - pos(t);
-                        const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
-                        if (r2 < P.h2) {
-                            if (cnt < P.cap) {
-                                my[cnt++] = (uint16_t)t;
-                            } else {
-                                trunc = true;
-                                break;
+                for (uint32_t c = 0; c < 27u && !trunc; ++c) {
+                    const uint32_t e = (c / 3u) * (I.W + 2u) + kc - 1u + (c % 3u);
+                    const uint32_t lo = S.tstart[e], hi = S.tstart[e + 1];
+                    const uint32_t gb = S.gstart[e] - lo;  // slot = gb + tile index
+                    for (uint32_t t = lo; t < hi && !trunc; t += 4u) {
+                        F3 pj[4];
+#pragma unroll
+                        for (uint32_t i = 0; i < 4u; ++i) {
+                            const uint32_t ti = min(t + i, hi - 1u);
+                            pj[i] = f3of(I.staged ? tile[ti] : Pf[gb + ti]);
+                        }
+#pragma unroll
+                        for (uint32_t i = 0; i < 4u; ++i) {
+                            const F3 d = self - pj[i];
+                            const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+                            if (!trunc && t + i < hi && r2 < P.h2) {
+                                if (cnt < ncap) nl[(size_t)cnt++ * cap + g] = gb + t + i;
+                                else trunc = true;
                             }
                         }
                     }
                 }
-                b.lam[g] = lambda_of(self, P, [&](auto&& fn) {
-                    for (int c = 0; c < cnt; ++c) fn(pos(my[c]));
-                });
-                my[P.stride - 1] = (uint16_t)cnt;
-                uint4* dst = reinterpret_cast<uint4*>(b.list + (size_t)g * P.stride);
-                const uint4* src = reinterpret_cast<const uint4*>(my);
-                for (int c = 0; c < P.stride / 8; ++c) dst[c] = src[c];
+                b.ncnt[g] = cnt;
                 tl.trunc += trunc ? 1u : 0u;
             }
+        } else {
+            // neighbourhood beyond the u16 tile index range: reported as
+            // tile_overflow; its particles get empty lists
+            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) b.ncnt[I.own_g0 + q] = 0;
         }
         __syncthreads();
     }
     flush_tally(b.ctr, tl);
 }
 
-// ------------------------------------------------- K8: lambda of iterations >= 1
-__global__ __launch_bounds__(kBlock) void k_lambda(Grid G, Phys P, Bufs b, uint32_t tile_cap,
-                                                   const float* __restrict__ Pc) {
-    __shared__ UnitShared S;
-    extern __shared__ float4 smem[];
-    float4* tile = smem;
+// ------------------------------------------------- K7: lambda
+// calcScalingFactorCUDA, memo form (pbf_constraint.cu:168-209): over the
+// frozen list, s = C / (m^2 inv_rho0^2 G + eps) with the reference's fma
+// order; writes (x*, s).  One thread per slot.
+__global__ __launch_bounds__(256) void k_lambda(Grid G, Phys P, Bufs b) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= b.offset[G.C]) return;
+    const float4* __restrict__ Pf = b.Pf;
+    const uint32_t* __restrict__ nl = b.nlist;
     const uint32_t cap = b.cap;
-    const uint32_t nU = b.ctr[CTR_UNITS];
-    uint32_t u0, u1, us;
-    unit_range(nU, u0, u1, us);
-    Tally tl;
-    for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, tl);
-        if (I.ok) {
-            if (I.staged)
-                load_runs(S, I, [&](uint32_t t, uint32_t g) {
-                    tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], 0.f);
-                });
-            __syncthreads();
-            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
-                const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
-                auto pos = [&](uint32_t t) -> F3 {
-                    if (I.staged) return f3of(tile[t]);
-                    return ld3(Pc, cap, tile_to_global(S, I.ncell, t));
-                };
-                b.lam[g] = lambda_of(pos(ts), P, [&](auto&& fn) {
-                    for_list(b.list, g, P.stride, [&](uint32_t t) { fn(pos(t)); });
-                });
+    const F3 self = f3of(Pf[i]);
+    const uint32_t n = b.ncnt[i];
+    float ks = 0.f, kg2s = 0.f;
+    F3 kgs = mk3(0.f, 0.f, 0.f);
+    for (uint32_t k = 0; k < n; k += 4u) {
+        F3 pj[4];
+#pragma unroll
+        for (uint32_t m = 0; m < 4u; ++m) pj[m] = f3of(Pf[nl[(size_t)min(k + m, n - 1u) * cap + i]]);
+#pragma unroll
+        for (uint32_t m = 0; m < 4u; ++m)
+            if (k + m < n) {
+                float W;
+                F3 kg;
+                pair_terms(self, pj[m], P, W, kg);
+                const float kg2 = fdot(kg, kg);
+                ks += W;
+                acc(kgs, kg);
+                kg2s += kg2;
             }
-        }
-        __syncthreads();
     }
-    flush_tally(b.ctr, tl);
+    const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
+    const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
+    b.Q[i] = make_float4(self.x, self.y, self.z, Cc / __builtin_fmaf(P.m2r2, Gs, P.relax));
 }
 
-// ------------------------------------------------- K7: delta-p + collision + apply
+// ------------------------------------------------- K8: delta-p + collision + apply
 // calcPositionUpdateCUDA (memo form, pbf_position_update.cu:131-188),
 // responseCollisionCUDA (pbf_response_collision.cu:12-66) and
-// updateInterimPositionCUDA (pbf_constraint.cu:335-345), written to the
-// other position buffer.
-__global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, Bufs b, uint32_t tile_cap,
-                                                   const float* __restrict__ Pc, float* __restrict__ Pn) {
-    __shared__ UnitShared S;
-    extern __shared__ float4 smem[];
-    float4* tile = smem;
-    const uint32_t cap = b.cap;
-    const uint32_t nU = b.ctr[CTR_UNITS];
-    uint32_t u0, u1, us;
-    unit_range(nU, u0, u1, us);
+// updateInterimPositionCUDA (pbf_constraint.cu:335-345).  Reads (x*, s),
+// writes the new x*.  One thread per slot.
+__global__ __launch_bounds__(256) void k_deltap(Grid G, Phys P, Boundary B, Bufs b) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     Tally tl;
-    for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, tl);
-        if (I.ok) {
-            if (I.staged)
-                load_runs(S, I, [&](uint32_t t, uint32_t g) {
-                    tile[t] = make_float4(Pc[g], Pc[cap + g], Pc[2 * cap + g], b.lam[g]);
-                });
-            __syncthreads();
-            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
-                const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
-                auto elem = [&](uint32_t t) -> float4 {
-                    if (I.staged) return tile[t];
-                    const uint32_t gg = tile_to_global(S, I.ncell, t);
-                    return make_float4(Pc[gg], Pc[cap + gg], Pc[2 * cap + gg], b.lam[gg]);
-                };
-                const float4 se = elem(ts);
-                const F3 self = f3of(se);
-                const float si = se.w;
-                const float self_cl = (si < 0.f) ? -si : 0.f;
-                F3 sum = mk3(0.f, 0.f, 0.f);
-                for_list(b.list, g, P.stride, [&](uint32_t t) {
-                    const float4 pe = elem(t);
+    if (i < b.offset[G.C]) {
+        const float4* __restrict__ Q = b.Q;
+        const uint32_t* __restrict__ nl = b.nlist;
+        const uint32_t cap = b.cap;
+        const float4 se = Q[i];
+        const F3 self = f3of(se);
+        const float si = se.w;
+        const float self_cl = (si < 0.f) ? -si : 0.f;
+        const uint32_t n = b.ncnt[i];
+        F3 sum = mk3(0.f, 0.f, 0.f);
+        for (uint32_t k = 0; k < n; k += 4u) {
+            float4 pe[4];
+#pragma unroll
+            for (uint32_t m = 0; m < 4u; ++m) pe[m] = Q[nl[(size_t)min(k + m, n - 1u) * cap + i]];
+#pragma unroll
+            for (uint32_t m = 0; m < 4u; ++m)
+                if (k + m < n) {
                     float W;
                     F3 kg;
-                    pair_terms(self, f3of(pe), P, W, kg);
-                    const float sj = pe.w;
+                    pair_terms(self, f3of(pe[m]), P, W, kg);
+                    const float sj = pe[m].w;
                     float corr;
                     if (P.mode == 0) {
                         const float kr2 = W * W;
@@ -832,16 +784,15 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
                     }
                     const float a = -(si + sj + corr);
                     acc(sum, mulsv(a, kg));
-                });
-                F3 dp = mk3(sum.x * P.m * P.inv_rho0, sum.y * P.m * P.inv_rho0, sum.z * P.m * P.inv_rho0);
-                bool capped = false;
-                const int sw = collide(P, B, dp, self, capped);
-                tl.sweepcap += capped ? 1u : 0u;
-                tl.maxsweep = max(tl.maxsweep, (uint32_t)sw);
-                st3(Pn, cap, g, self + dp);
-            }
+                }
         }
-        __syncthreads();
+        F3 dp = mk3(sum.x * P.m * P.inv_rho0, sum.y * P.m * P.inv_rho0, sum.z * P.m * P.inv_rho0);
+        bool capped = false;
+        const int sw = collide(P, B, dp, self, capped);
+        const F3 np = self + dp;
+        b.Pf[i] = make_float4(np.x, np.y, np.z, 0.f);
+        tl.sweepcap = capped ? 1u : 0u;
+        tl.maxsweep = (uint32_t)sw;
     }
     flush_tally(b.ctr, tl);
 }
@@ -855,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Grid G, Phys P, Boundary B, B
 // unit's tile.  The same pass packs each slot's record R = (x*, 0), (v, 0)
 // with v = inv_dt*(x* - x_sorted) (updateVelocityCUDA, pbf_update.cu:25-39),
 // so that a particle visited in final-cell order costs one cache line.
-__global__ __launch_bounds__(256) void k_cur_hash(const float* __restrict__ Pc, const float* __restrict__ X,
+__global__ __launch_bounds__(256) void k_cur_hash(const float4* __restrict__ Pf, const float* __restrict__ X,
                                                   float inv_dt, uint32_t cap, const uint32_t* __restrict__ offset,
                                                   Grid G, uint32_t* __restrict__ ckey, uint32_t* __restrict__ crank,
                                                   uint32_t* __restrict__ ccount, uint32_t* __restrict__ ctflag,
@@ -865,7 +816,7 @@ __global__ __launch_bounds__(256) void k_cur_hash(const float* __restrict__ Pc, 
     const bool active = s < nl;
     uint32_t k = 0;
     if (active) {
-        const F3 p = ld3(Pc, cap, s);
+        const F3 p = f3of(Pf[s]);
         const F3 vv = mulsv(inv_dt, p - ld3(X, cap, s));
         R[2 * s] = make_float4(p.x, p.y, p.z, 0.f);
         R[2 * s + 1] = make_float4(vv.x, vv.y, vv.z, 0.f);
@@ -1093,6 +1044,10 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, u
                 if (!I.staged) tl.slowpart++;
                 build(o.c0 + i, b.cperm[o.c0 + i], I.staged);
             }
+        } else {
+            // no tile index range: the passes sum these particles directly
+            const OwnCur o = own_cur(b, unit);
+            for (uint32_t i = threadIdx.x; i < o.n; i += kBlock) b.fcnt[o.c0 + i] = 0xFFFFFFFFu;
         }
         __syncthreads();
     }
@@ -1101,14 +1056,6 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, u
         build(q, g, false);
     });
     flush_tally(b.ctr, tl);
-}
-
-// Block -> position range of the final-cell order, contiguous per XCD.
-__device__ __forceinline__ uint32_t xcd_block() {
-    const uint32_t nb = gridDim.x, b = blockIdx.x, nx = 8;
-    if (nb < nx) return b;
-    const uint32_t per = nb / nx, rem = nb % nx, x = b % nx;
-    return x * per + min(x, rem) + b / nx;
 }
 
 // Sum over a particle's stored pair list, per-cell partial sums in order;
@@ -1256,7 +1203,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, flo
 // ------------------------------------------------- K13: write-back without XSPH
 // pbf_solve.cpp:125-145 as shipped: v = inv_dt*(x* - x_sorted), x = x*; with
 // R (vorticity ran without XSPH) the record's position and confined velocity.
-__global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float* __restrict__ Pc,
+__global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float4* __restrict__ Pf,
                                              const float4* __restrict__ R, uint32_t C, float* __restrict__ xo,
                                              float* __restrict__ vo) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1268,7 +1215,7 @@ __global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float* __rest
         p = f3of(R[2 * s]);
         vv = f3of(R[2 * s + 1]);
     } else {
-        p = ld3(Pc, cap, s);
+        p = f3of(Pf[s]);
         vv = mulsv(P.inv_dt, p - ld3(b.X, cap, s));
     }
     xo[3 * s + 0] = p.x;
@@ -1280,26 +1227,14 @@ __global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float* __rest
 }
 
 // ------------------------------------------------- views
-__global__ __launch_bounds__(kBlock) void k_neighbors_view(Grid G, Phys P, Bufs b, uint32_t* __restrict__ out,
-                                                           uint32_t* __restrict__ counts, uint32_t max_pair) {
-    __shared__ UnitShared S;
-    const uint32_t nU = b.ctr[CTR_UNITS];
-    uint32_t u0, u1, us;
-    unit_range(nU, u0, u1, us);
-    Tally tl;
-    for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], 0u, S, tl);
-        if (I.ok)
-            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
-                const uint32_t g = I.own_g0 + q;
-                const uint16_t* row = b.list + (size_t)g * P.stride;
-                const uint32_t cnt = row[P.stride - 1];
-                counts[g] = cnt;
-                for (uint32_t c = 0; c < max_pair; ++c)
-                    out[(size_t)g * max_pair + c] = c < cnt ? tile_to_global(S, I.ncell, row[c]) : 0xFFFFFFFFu;
-            }
-        __syncthreads();
-    }
+__global__ __launch_bounds__(256) void k_neighbors_view(Grid G, Bufs b, uint32_t* __restrict__ out,
+                                                        uint32_t* __restrict__ counts, uint32_t max_pair) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.offset[G.C]) return;
+    const uint32_t cnt = b.ncnt[g];
+    counts[g] = cnt;
+    for (uint32_t c = 0; c < max_pair; ++c)
+        out[(size_t)g * max_pair + c] = c < cnt ? b.nlist[(size_t)c * b.cap + g] : 0xFFFFFFFFu;
 }
 
 __global__ void k_cell_ranges(const uint32_t* __restrict__ offset, uint32_t C, uint32_t* __restrict__ start,
@@ -1311,11 +1246,11 @@ __global__ void k_cell_ranges(const uint32_t* __restrict__ offset, uint32_t C, u
     end[c] = (z > a) ? z : 0u;
 }
 
-__global__ void k_copy_lambda(const uint32_t* __restrict__ offset, uint32_t C, const float* __restrict__ lam,
+__global__ void k_copy_lambda(const uint32_t* __restrict__ offset, uint32_t C, const float4* __restrict__ Q,
                               float* __restrict__ out) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= offset[C]) return;
-    out[s] = lam[s];
+    out[s] = Q[s].w;
 }
 
 __global__ void k_selftest(const float* __restrict__ a, const float* __restrict__ bb, float* __restrict__ out,
@@ -1332,37 +1267,25 @@ __global__ void k_selftest(const float* __restrict__ a, const float* __restrict_
 
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-// Dynamic LDS of each unit kernel: the tile, plus the list rows under
-// construction in the neighbour search.
-size_t smem_of(int k, uint32_t tile_cap, int stride) {
-    const size_t tile = (size_t)tile_cap * sizeof(float4);
-    switch (k) {
-        case K_NS_LAMBDA: return tile + (size_t)kBlock * stride * sizeof(uint16_t);
-        default: return tile;
-    }
-}
+// Dynamic LDS of each unit kernel: the tile.
+size_t smem_of(uint32_t tile_cap) { return (size_t)tile_cap * sizeof(float4); }
 
 }  // namespace
 
-void size_unit_grids(Tuning& T, int num_cu, int stride) {
+void size_unit_grids(Tuning& T, int num_cu) {
     struct K {
         int id;
         const void* f;
-    } ks[] = {
-        {K_NS_LAMBDA, (const void*)k_ns_lambda}, {K_LAMBDA, (const void*)k_lambda},
-        {K_DELTAP, (const void*)k_deltap},       {K_FINAL_LIST, (const void*)k_final_list},
-    };
+    } ks[] = {{K_NS, (const void*)k_ns}, {K_FINAL_LIST, (const void*)k_final_list}};
     const uint32_t ncu = num_cu > 0 ? (uint32_t)num_cu : 1u;
     uint32_t lo = 0xFFFFFFFFu;
     for (int i = 0; i < K_NUM; ++i) T.grid[i] = 0;
     for (auto& k : ks) {
-        // allow the largest configuration, then ask how many workgroups of
-        // the current one fit a CU
-        (void)hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)smem_of(k.id, kTileCapMax, 64));
+        // allow the largest tile, then ask how many workgroups of the
+        // current one fit a CU
+        (void)hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_of(kTileCapMax));
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.f, kBlock, smem_of(k.id, T.tile_cap, stride)) !=
-                hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.f, kBlock, smem_of(T.tile_cap)) != hipSuccess ||
             nb < 1)
             nb = 1;
         T.grid[k.id] = ncu * (uint32_t)nb < 8u ? 8u : ncu * (uint32_t)nb;
@@ -1373,7 +1296,7 @@ void size_unit_grids(Tuning& T, int num_cu, int stride) {
 
 const char* kernel_name(int id) {
     static const char* names[K_NUM] = {"k_predict_hash", "k_scan_tiles", "k_scan_cells", "k_scatter",
-                                       "k_reorder",      "k_ns_lambda",  "k_lambda",     "k_deltap",
+                                       "k_reorder",      "k_ns",         "k_lambda",     "k_deltap",
                                        "k_vort",         "k_confine",    "k_xsph_out",   "k_out",
                                        "k_cur_sort",     "k_final_list"};
     return (id >= 0 && id < K_NUM) ? names[id] : "?";
@@ -1404,7 +1327,7 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
     const uint32_t uw = T.unit_width < G.gx ? T.unit_width : G.gx;
     const uint32_t budget = (T.tile_cap * 3u) / 4u > 0 ? (T.tile_cap * 3u) / 4u : 1u;
     auto grid = [&](int k) { return dim3(T.grid[k] ? T.grid[k] : T.grid_units); };
-    auto smem = [&](int k) { return smem_of(k, T.tile_cap, P.stride); };
+    const size_t smem = smem_of(T.tile_cap);
     if (n > 0) {
         Mark m(tm, s, K_PREDICT_HASH);
         hipLaunchKernelGGL(k_predict_hash, dim3(nb), dim3(256), 0, s, x, v, n, G, P.dt, P.g[0], P.g[1], P.g[2], lo[0],
@@ -1429,39 +1352,29 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
     {
         Mark m(tm, s, K_REORDER);
         hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, x, v, G, P.dt, P.g[0], P.g[1], P.g[2], b.key, b.tmp,
-                           b.offset, b.sidx, b.PA, b.X, b.cap, n, uw, budget, b.units, b.ctr);
+                           b.offset, b.sidx, b.Pf, b.X, b.cap, n, uw, budget, b.units, b.ctr);
     }
     {
-        Mark m(tm, s, K_NS_LAMBDA);
-        hipLaunchKernelGGL(k_ns_lambda, grid(K_NS_LAMBDA), dim3(kBlock), smem(K_NS_LAMBDA), s, G, P, b, T.tile_cap,
-                           (const float*)b.PA);
+        Mark m(tm, s, K_NS);
+        hipLaunchKernelGGL(k_ns, grid(K_NS), dim3(kBlock), smem, s, G, P, b, T.tile_cap);
     }
-    float* cur = b.PA;
-    float* nxt = b.PB;
     for (int it = 0; it < iters; ++it) {
-        if (it > 0) {
-            Mark m(tm, s, K_LAMBDA);
-            hipLaunchKernelGGL(k_lambda, grid(K_LAMBDA), dim3(kBlock), smem(K_LAMBDA), s, G, P, b, T.tile_cap,
-                               (const float*)cur);
-        }
         {
-            Mark m(tm, s, K_DELTAP);
-            hipLaunchKernelGGL(k_deltap, grid(K_DELTAP), dim3(kBlock), smem(K_DELTAP), s, G, P, B, b, T.tile_cap,
-                               (const float*)cur, nxt);
+            Mark m(tm, s, K_LAMBDA);
+            hipLaunchKernelGGL(k_lambda, dim3(nb), dim3(256), 0, s, G, P, b);
         }
-        float* t = cur;
-        cur = nxt;
-        nxt = t;
+        Mark m(tm, s, K_DELTAP);
+        hipLaunchKernelGGL(k_deltap, dim3(nb), dim3(256), 0, s, G, P, B, b);
     }
     if (!vort_on && !xsph_on) {
         Mark m(tm, s, K_OUT);
-        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, (const float4*)nullptr, G.C,
+        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float4*)b.Pf, (const float4*)nullptr, G.C,
                            x, v);
         return hipGetLastError();
     }
     {
         Mark m(tm, s, K_CUR_SORT);
-        hipLaunchKernelGGL(k_cur_hash, dim3(nb), dim3(256), 0, s, (const float*)cur, (const float*)b.X, P.inv_dt,
+        hipLaunchKernelGGL(k_cur_hash, dim3(nb), dim3(256), 0, s, (const float4*)b.Pf, (const float*)b.X, P.inv_dt,
                            b.cap, b.offset, G, b.ckey, b.crank, b.ccount, b.ctflag, b.lgT, b.R);
         hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, b.nT, b.lgT,
                            G.C);
@@ -1473,8 +1386,7 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
     }
     {
         Mark m(tm, s, K_FINAL_LIST);
-        hipLaunchKernelGGL(k_final_list, grid(K_FINAL_LIST), dim3(kBlock), smem(K_FINAL_LIST), s, G, P, b,
-                           T.tile_cap, uw);
+        hipLaunchKernelGGL(k_final_list, grid(K_FINAL_LIST), dim3(kBlock), smem, s, G, P, b, T.tile_cap, uw);
     }
     if (vort_on) {
         {
@@ -1489,15 +1401,18 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
         hipLaunchKernelGGL(k_xsph_out, dim3(nb), dim3(kBlock), 0, s, G, P, b, x, v);
     } else {
         Mark m(tm, s, K_OUT);
-        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float*)cur, (const float4*)b.R, G.C, x,
-                           v);
+        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float4*)b.Pf, (const float4*)b.R, G.C,
+                           x, v);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, const Tuning& T, uint32_t* out,
                                  uint32_t* counts, uint32_t max_pair, hipStream_t s) {
-    hipLaunchKernelGGL(k_neighbors_view, dim3(T.grid_units), dim3(kBlock), 0, s, G, P, b, out, counts, max_pair);
+    (void)P;
+    (void)T;
+    hipLaunchKernelGGL(k_neighbors_view, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, G, b, out,
+                       counts, max_pair);
     return hipGetLastError();
 }
 
@@ -1508,7 +1423,7 @@ hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uin
 
 hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_lambda, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b.offset, G.C,
-                       b.lam, out);
+                       (const float4*)b.Q, out);
     return hipGetLastError();
 }
 

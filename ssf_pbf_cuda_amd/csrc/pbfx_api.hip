@@ -28,7 +28,7 @@ struct pbfx_ctx {
     float* d_nrm = nullptr;
     uint32_t n_planes = 0;
     float4* d_default_sphere = nullptr;  // pbf_constraint.cu:300: centre (3,3,3), radius 3
-    uint32_t* h_pinned = nullptr;        // CTR_NUM counters + live + in
+    uint32_t* h_pinned = nullptr;        // kCtrWords counters + live + in
     int last_err = 0;
     int report_info = 0;
     uint32_t steps = 0;
@@ -207,19 +207,31 @@ Boundary user_boundary(const pbfx_ctx* c) {
 }
 
 pbfx_status read_counters(pbfx_ctx* c, hipStream_t s) {
-    hipError_t e = hipMemcpyAsync(c->h_pinned, c->b.ctr, CTR_NUM * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpyAsync(c->h_pinned, c->b.ctr, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(c->h_pinned + CTR_NUM, c->b.offset + c->G.C, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           s);
+        e = hipMemcpyAsync(c->h_pinned + kCtrWords, c->b.offset + c->G.C, 2 * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e);
+    // fold the statistics copies into h[0..CTR_NUM)
+    uint32_t* h = c->h_pinned;
+    for (int k = 0; k < CTR_NUM; ++k) {
+        if (k == CTR_UNITS || k == CTR_CUNITS) continue;
+        const bool is_max = (k == CTR_MAXTILE || k == CTR_MAXSWEEP);
+        uint32_t v = 0;
+        for (uint32_t r = 1; r <= kStatRep; ++r) {
+            const uint32_t x = h[CTR_NUM * r + k];
+            v = is_max ? (x > v ? x : v) : v + x;
+        }
+        h[k] = v;
+    }
     return PBFX_OK;
 }
 
 pbfx_status sync_step_result(pbfx_ctx* c, uint32_t* num_inout, hipStream_t s) {
     pbfx_status st = read_counters(c, s);
     if (st != PBFX_OK) return st;
-    *num_inout = c->h_pinned[CTR_NUM];
+    *num_inout = c->h_pinned[kCtrWords];
     if (c->h_pinned[CTR_OVERFLOW]) return PBFX_TILE_OVERFLOW;
     if (c->report_info) {
         if (c->h_pinned[CTR_TRUNC]) return PBFX_NEIGHBOR_TRUNCATED;
@@ -321,10 +333,10 @@ pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const floa
     PBFX_ALLOC(b.fcnt, cap);
     PBFX_ALLOC(b.tflag, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctflag, (size_t)b.nT + 1);
-    PBFX_ALLOC(b.ctr, CTR_NUM);
+    PBFX_ALLOC(b.ctr, kCtrWords);
     PBFX_ALLOC(c->d_default_sphere, 1);
 #undef PBFX_ALLOC
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_pinned, (CTR_NUM + 2) * sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_pinned, (kCtrWords + 2) * sizeof(uint32_t), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemset(b.count, 0, ((size_t)G.C + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b.offset, 0, ((size_t)G.C + 2) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b.tsum, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
@@ -333,7 +345,7 @@ pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const floa
     if (e == hipSuccess) e = hipMemset(b.ctsum, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b.tflag, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b.ctflag, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.ctr, 0, CTR_NUM * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.ctr, 0, kCtrWords * sizeof(uint32_t));
     if (e == hipSuccess) {
         const float4 sph = make_float4(3.f, 3.f, 3.f, 3.f);
         e = hipMemcpy(c->d_default_sphere, &sph, sizeof(sph), hipMemcpyHostToDevice);
@@ -452,7 +464,7 @@ pbfx_status pbfx_get_num(pbfx_ctx* c, void* stream, uint32_t* num) {
     DeviceGuard g(c->device);
     pbfx_status st = read_counters(c, (hipStream_t)stream);
     if (st != PBFX_OK) return st;
-    *num = c->h_pinned[CTR_NUM];
+    *num = c->h_pinned[kCtrWords];
     return PBFX_OK;
 }
 
@@ -462,8 +474,8 @@ pbfx_status pbfx_get_stats(pbfx_ctx* c, void* stream, pbfx_stats* out) {
     pbfx_status st = read_counters(c, (hipStream_t)stream);
     if (st != PBFX_OK) return st;
     const uint32_t* h = c->h_pinned;
-    out->num_in = h[CTR_NUM + 1];
-    out->num_live = h[CTR_NUM];
+    out->num_in = h[kCtrWords + 1];
+    out->num_live = h[kCtrWords];
     out->truncated = h[CTR_TRUNC];
     out->sweep_cap_hits = h[CTR_SWEEPCAP];
     out->max_sweeps_used = h[CTR_MAXSWEEP];

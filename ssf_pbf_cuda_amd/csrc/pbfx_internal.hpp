@@ -17,7 +17,8 @@
 //                                                 the same sort keyed by each particle's final cell
 //   flist, fcnt                    u32[64*cap], u32[cap]  in-range pairs of the final passes by
 //                                                 final-order position (column-major), counts
-//   ctr                            u32[16]       counters of the last step
+//   ctr                            u32[16*65]    allocation counters, then 64 copies of the
+//                                                 statistics counters of the last step
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -50,6 +51,8 @@ enum Ctr : int {
     CTR_LISTOVF,
     CTR_NUM = 16
 };
+constexpr uint32_t kStatRep = 64;                           // copies of the statistics counters
+constexpr uint32_t kCtrWords = CTR_NUM * (1 + kStatRep);    // ctr[0..CTR_NUM): allocation counters
 
 struct Grid {
     uint32_t gx, gy, gz;

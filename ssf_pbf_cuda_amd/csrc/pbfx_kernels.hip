@@ -223,6 +223,58 @@ __device__ __forceinline__ int collide(const Phys& P, const Boundary& B, F3& dp,
     return sweeps;
 }
 
+// The same for a boundary of exactly NP planes and no spheres (the box): the
+// planes are loaded once, and the per-plane terms that do not change between
+// sweeps -- (old_x - P).n, -(P.n) and n.n -- are computed once; each has the
+// value the general form recomputes.
+template <int NP>
+__device__ __forceinline__ int collide_planes(const Phys& P, const Boundary& B, F3& dp, F3 pk, bool& capped) {
+    F3 Pp[NP], n[NP];
+    float pa_n[NP], dd[NP], nn[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        Pp[i] = mk3(B.pts[3 * i], B.pts[3 * i + 1], B.pts[3 * i + 2]);
+        n[i] = mk3(B.nrm[3 * i], B.nrm[3 * i + 1], B.nrm[3 * i + 2]);
+        pa_n[i] = fdot(pk - Pp[i], n[i]);
+        dd[i] = -fdot(Pp[i], n[i]);
+        nn[i] = fdot(n[i], n[i]);
+    }
+    int sweeps = 0;
+    capped = false;
+    while (true) {
+        if (sweeps >= P.max_sweeps) {
+            capped = true;
+            break;
+        }
+        ++sweeps;
+        bool hit = false;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            // response_plane with its constant terms hoisted
+            const F3 pred = pk + dp;
+            const float pb_n = fdot(pred - Pp[i], n[i]);
+            F3 I;
+            if (pa_n[i] == 0.f && pb_n == 0.f) {
+                continue;
+            } else if (pa_n[i] >= 0.f && pb_n < 0.f) {
+                const float frac = fabsf(pa_n[i]) / (fabsf(pa_n[i]) + fabsf(pb_n));
+                I = pk + mulvs(pred - pk, frac);
+            } else if (pa_n[i] < 0.f && pb_n < 0.f) {
+                const float t = -(fdot(pred, n[i]) + dd[i]) / nn[i];
+                I = pred + mulsv(t, n[i]);
+            } else {
+                continue;
+            }
+            hit = true;
+            const F3 di = pred - I;
+            const float c = fdot(di, n[i]);
+            acc(dp, mulsv(-c, n[i]));
+        }
+        if (!hit) break;
+    }
+    return sweeps;
+}
+
 // ------------------------------------------------------------ block helpers
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
@@ -249,23 +301,27 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
     return off + x - v;
 }
 
-// Wave-aggregated atomicAdd(&ctr[k], 1) returning a unique rank per lane
+// atomicAdd(&ctr[k], 1) per active lane, returning a unique rank per lane
 // (ranks among equal keys are arbitrary; the reorder makes them stable).
+// Runs of equal keys in consecutive lanes share one atomic, issued by the
+// run's first lane; all runs' atomics are in flight together.  The active
+// lanes must be a prefix of the wave.
 __device__ __forceinline__ uint32_t wave_rank(uint32_t* ctr, uint32_t k, bool active) {
-    uint64_t pending = __ballot(active);
-    const uint64_t lt = (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id()));
-    uint32_t mine = 0;
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lk = __shfl(k, leader, 64);
-        const uint64_t same = __ballot(active && k == lk) & pending;
-        uint32_t base = 0;
-        if ((int)lane_id() == leader) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
-        base = __shfl(base, leader, 64);
-        if (active && k == lk) mine = base + (uint32_t)__popcll(same & lt);
-        pending &= ~same;
+    const uint32_t lane = lane_id();
+    const uint32_t nact = (uint32_t)__popcll(__ballot(active));
+    const uint32_t kp = (uint32_t)__shfl_up((int)k, 1, 64);
+    const bool head = active && (lane == 0 || kp != k);
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t above = heads & ~upto;
+        const uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : nact;
+        base = atomicAdd(&ctr[k], end - lane);
     }
-    return mine;
+    const int hl = 63 - __clzll((unsigned long long)(heads & upto));
+    base = (uint32_t)__shfl((int)base, hl < 0 ? 0 : hl, 64);
+    return base + (lane - (uint32_t)(hl < 0 ? 0 : hl));
 }
 
 // Marks a scan tile as occupied: a read first, so that the common case (the
@@ -363,7 +419,8 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict_
             tprefix[b0 + j] = run;
             run += tsum[b0 + j];
         }
-    if (reset && threadIdx.x < CTR_NUM) ctr[threadIdx.x] = 0;
+    if (reset)
+        for (uint32_t t = threadIdx.x; t < kCtrWords; t += blockDim.x) ctr[t] = 0;
 }
 
 // ------------------------------------------------- K3: scan of the cell counts
@@ -515,7 +572,11 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 }
 
 // Called by every thread of the block at kernel end.
-__device__ void flush_tally(uint32_t* __restrict__ ctr, const Tally& t) {
+// Statistics go to one of kStatRep copies of the counters (by workgroup), so
+// that waves of a large grid do not all update the same words; the host
+// reduces the copies.
+__device__ void flush_tally(uint32_t* __restrict__ ctr_base, const Tally& t) {
+    uint32_t* ctr = ctr_base + CTR_NUM * (1u + (blockIdx.x & (kStatRep - 1u)));
     const uint32_t tr = wave_sum(t.trunc), sp = wave_sum(t.slowpart), su = wave_sum(t.slowunits),
                    ov = wave_sum(t.overflow), mt = wave_max(t.maxtile), sc = wave_sum(t.sweepcap),
                    ms = wave_max(t.maxsweep), nn = wave_sum(t.nan), lo = wave_sum(t.listovf);
@@ -524,9 +585,9 @@ __device__ void flush_tally(uint32_t* __restrict__ ctr, const Tally& t) {
         if (sp) atomicAdd(&ctr[CTR_SLOWPART], sp);
         if (su) atomicAdd(&ctr[CTR_SLOWUNITS], su);
         if (ov) atomicAdd(&ctr[CTR_OVERFLOW], ov);
-        if (mt) atomicMax(&ctr[CTR_MAXTILE], mt);
+        if (mt && mt > ctr[CTR_MAXTILE]) atomicMax(&ctr[CTR_MAXTILE], mt);  // a stale read only costs an atomic
         if (sc) atomicAdd(&ctr[CTR_SWEEPCAP], sc);
-        if (ms) atomicMax(&ctr[CTR_MAXSWEEP], ms);
+        if (ms && ms > ctr[CTR_MAXSWEEP]) atomicMax(&ctr[CTR_MAXSWEEP], ms);
         if (nn) atomicAdd(&ctr[CTR_XSPHNAN], nn);
         if (lo) atomicAdd(&ctr[CTR_LISTOVF], lo);
     }
@@ -788,7 +849,9 @@ __global__ __launch_bounds__(256) void k_deltap(Grid G, Phys P, Boundary B, Bufs
         }
         F3 dp = mk3(sum.x * P.m * P.inv_rho0, sum.y * P.m * P.inv_rho0, sum.z * P.m * P.inv_rho0);
         bool capped = false;
-        const int sw = collide(P, B, dp, self, capped);
+        const int sw = (B.n_planes == 6 && B.n_inner == 0 && B.n_outer == 0)
+                           ? collide_planes<6>(P, B, dp, self, capped)
+                           : collide(P, B, dp, self, capped);
         const F3 np = self + dp;
         b.Pf[i] = make_float4(np.x, np.y, np.z, 0.f);
         tl.sweepcap = capped ? 1u : 0u;

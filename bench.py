@@ -64,13 +64,14 @@ def parse():
 
 
 def traffic_from_profiles(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
-    (profiles/traffic_<kernel>_<workload>.txt, line 'bytes_per_launch: X')."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.txt")
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE counters scaled by a plain copy of known size
+    (profiles/hbm_traffic_pbf_step_<workload>.txt, lines '<kernel> bytes_per_launch: X')."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_pbf_step_{workload}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("bytes_per_launch:"):
+        if line.startswith(f"{kernel} bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
 

@@ -170,6 +170,21 @@ pbfx_status pbfx_ns_cell_ranges(pbfx_ctx* ctx, uint32_t* cell_start, uint32_t* c
  * iteration, in sorted order, into `out` (device float [num_live]). */
 pbfx_status pbfx_scaling_factor(pbfx_ctx* ctx, float* out, void* stream);
 
+/* Density of each live particle over the last step's neighbour lists,
+ * rho_i = sum_j m * W(|x_i - x_j|) (calcDensity, pbf/solver/cuda/pbf_density.cu:11-99,
+ * with inv_h in the kernel where the reference passes h, pbf_density.cu:22).
+ * x: device AoS float3 in the last step's sorted order (its output);
+ * rho: device float [num_live]. */
+pbfx_status pbfx_density(pbfx_ctx* ctx, const float* x, float* rho, void* stream);
+
+/* Writes the last step's sorted output back to the order before its sort:
+ * x_out[sorted_index[s]] = x_sorted[s] (same for v) for every live slot s.
+ * Replaces neighbor_search::restoreOrder (pbf/interaction/pbf_neighbor_search.cpp:194-200,
+ * restoreOrderCUDA pbf_neighbor_search_util.cu:156-170).  Device AoS float3;
+ * x_out, v_out hold num_in particles (entries of deleted particles are left as they were). */
+pbfx_status pbfx_restore_order(pbfx_ctx* ctx, const float* x_sorted, const float* v_sorted, float* x_out,
+                               float* v_out, void* stream);
+
 typedef struct pbfx_stats {
     uint32_t num_in;           /* particles entering the last step */
     uint32_t num_live;         /* particles after deletion */

@@ -806,6 +806,28 @@ int orc_step_f64(const orc_params* p, const orc_boundary* b, const float* x, con
     return 0;
 }
 
+// calcDensity (pbf/solver/cuda/pbf_density.cu:11-71) over given neighbour
+// lists: rho_i = sum over the listed j, in list order, of W(|x_i - x_j|) * m,
+// with inv_h as the kernel argument (the reference passes h, pbf_density.cu:22).
+int orc_density(const orc_params* p, const float* x, uint32_t n, const uint32_t* neighbors, float* rho) {
+    const float inv_h = 1.f / p->smoothing_length;
+    const uint32_t mp = (uint32_t)p->max_pair;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const V3<float> self = v3<float>(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        float sum = 0.f;
+        for (uint32_t c = 0; c < mp; ++c) {
+            const uint32_t j = neighbors[(size_t)i * mp + c];
+            if (j == ORC_EMPTY) break;
+            const V3<float> d = self - v3<float>(x[3 * j], x[3 * j + 1], x[3 * j + 2]);
+            sum += wpoly6<float>(glm_length(d), inv_h) * p->particle_mass;
+        }
+        rho[i] = sum;
+    }
+    return 0;
+}
+
 int orc_metrics(const orc_params* p, const float* x, const float* v, uint32_t n, double out[4]) {
     apply_threads();
     const uint32_t G[3] = {p->grid[0], p->grid[1], p->grid[2]};

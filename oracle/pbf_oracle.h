@@ -137,6 +137,11 @@ int orc_step_f64(const orc_params* p, const orc_boundary* b,
    out[0]=mean C, out[1]=max C, out[2]=mean max(C,0), out[3]=kinetic energy. */
 int orc_metrics(const orc_params* p, const float* x, const float* v, uint32_t num, double out[4]);
 
+/* Density over given neighbour lists (calcDensity, pbf/solver/cuda/pbf_density.cu:11-71,
+ * inv_h in the kernel): neighbors is n x max_pair, ORC_EMPTY padded, as the
+ * lists orc_step_f32 returns; x in the same (sorted) order. */
+int orc_density(const orc_params* p, const float* x, uint32_t n, const uint32_t* neighbors, float* rho);
+
 /* Host IEEE reference of pbfx_selftest_arith: out[4i..4i+3] =
    a/b, sqrtf(a), fmaf(a,b,a), floorf(a/b). */
 void orc_arith_f32(const float* a, const float* b, float* out, uint32_t n);

@@ -118,6 +118,8 @@ def lib():
         L.orc_step_f64.restype = ctypes.c_int
         L.orc_metrics.argtypes = [ctypes.POINTER(OrcParams), f32p, f32p, ctypes.c_uint32, f64p]
         L.orc_metrics.restype = ctypes.c_int
+        L.orc_density.argtypes = [ctypes.POINTER(OrcParams), f32p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), f32p]
+        L.orc_density.restype = ctypes.c_int
         L.orc_arith_f32.argtypes = [f32p, f32p, f32p, ctypes.c_uint32]
         L.orc_arith_f32.restype = None
         L.orc_set_threads.argtypes = [ctypes.c_int]
@@ -298,6 +300,18 @@ def step_f64(scene, x=None, v=None, iters=None, params=None, boundary=True, grid
         raise RuntimeError(f"orc_step_f64 failed ({rc})")
     nl = int(num[0])
     return {"x": xo[:nl], "v": vo[:nl], "num": nl}
+
+
+def density(scene, x_sorted, neighbors, params=None, grid=None):
+    """rho over the given lists (n x max_pair, EMPTY padded), x in the lists' order."""
+    p = params if params is not None else scene.params
+    grid = tuple(grid) if grid is not None else tuple(scene.grid)
+    op = make_params(p, grid)
+    x = np.ascontiguousarray(x_sorted, dtype=np.float32).reshape(-1, 3)
+    nb = np.ascontiguousarray(neighbors, dtype=np.uint32).reshape(x.shape[0], int(p.max_pair))
+    out = np.zeros(x.shape[0], np.float32)
+    lib().orc_density(ctypes.byref(op), _fp(x), x.shape[0], _up(nb), _fp(out))
+    return out
 
 
 def metrics(scene, x, v, params=None, grid=None):

@@ -527,6 +527,21 @@ pbfx_status pbfx_scaling_factor(pbfx_ctx* c, float* out, void* stream) {
     return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
 }
 
+pbfx_status pbfx_density(pbfx_ctx* c, const float* x, float* rho, void* stream) {
+    if (!c || !x || !rho) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    hipError_t e = launch_density(c->G, make_phys(c->params), c->b, x, rho, (hipStream_t)stream);
+    return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
+}
+
+pbfx_status pbfx_restore_order(pbfx_ctx* c, const float* x_sorted, const float* v_sorted, float* x_out,
+                               float* v_out, void* stream) {
+    if (!c || !x_sorted || !v_sorted || !x_out || !v_out) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    hipError_t e = launch_restore(c->G, c->b, x_sorted, v_sorted, x_out, v_out, (hipStream_t)stream);
+    return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
+}
+
 pbfx_status pbfx_set_tuning(pbfx_ctx* c, uint32_t tile_cap, uint32_t unit_width) {
     if (!c) return PBFX_INVALID_ARG;
     if (tile_cap > kTileCapMax || !is_pow2(unit_width) || unit_width > kMaxW) return PBFX_INVALID_ARG;

@@ -170,6 +170,9 @@ hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, co
 hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uint32_t* end, hipStream_t s);
 hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s);
 hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t n, hipStream_t s);
+hipError_t launch_density(const Grid& G, const Phys& P, const Bufs& b, const float* x, float* rho, hipStream_t s);
+hipError_t launch_restore(const Grid& G, const Bufs& b, const float* xs, const float* vs, float* xo, float* vo,
+                          hipStream_t s);
 // Fills T.grid[] (and T.grid_units) for num_cu compute units.
 void size_unit_grids(Tuning& T, int num_cu);
 

@@ -1316,6 +1316,38 @@ __global__ void k_copy_lambda(const uint32_t* __restrict__ offset, uint32_t C, c
     out[s] = Q[s].w;
 }
 
+// calcDensityCUDA (pbf/solver/cuda/pbf_density.cu:11-71) over the last
+// step's frozen lists: rho_i = sum_j m * W(|x_i - x_j|), W with inv_h (the
+// reference passes h as the kernel's inv_h argument, pbf_density.cu:22).
+// x: AoS float3 in the step's sorted order.
+__global__ void k_density(Grid G, Phys P, Bufs b, const float* __restrict__ x, float* __restrict__ rho) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.offset[G.C]) return;
+    const F3 self = mk3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    const uint32_t n = b.ncnt[i];
+    float sum = 0.f;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t j = b.nlist[(size_t)k * b.cap + i];
+        const F3 d = self - mk3(x[3 * j], x[3 * j + 1], x[3 * j + 2]);
+        sum += wpoly6(glm_length(d), P.inv_h) * P.m;
+    }
+    rho[i] = sum;
+}
+
+// restoreOrderCUDA (pbf/interaction/cuda/pbf_neighbor_search_util.cu:156-170):
+// out[sorted_index[s]] = sorted[s] for the live slots of the last step.
+__global__ void k_restore(Grid G, Bufs b, const float* __restrict__ xs, const float* __restrict__ vs,
+                          float* __restrict__ xo, float* __restrict__ vo) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.offset[G.C]) return;
+    const uint32_t o = b.sidx[s];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        xo[3 * o + c] = xs[3 * s + c];
+        vo[3 * o + c] = vs[3 * s + c];
+    }
+}
+
 __global__ void k_selftest(const float* __restrict__ a, const float* __restrict__ bb, float* __restrict__ out,
                            uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1487,6 +1519,17 @@ hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uin
 hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_lambda, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b.offset, G.C,
                        (const float4*)b.Q, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const Grid& G, const Phys& P, const Bufs& b, const float* x, float* rho, hipStream_t s) {
+    hipLaunchKernelGGL(k_density, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, G, P, b, x, rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_restore(const Grid& G, const Bufs& b, const float* xs, const float* vs, float* xo, float* vo,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_restore, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, G, b, xs, vs, xo, vo);
     return hipGetLastError();
 }
 

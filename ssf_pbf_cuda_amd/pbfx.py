@@ -113,6 +113,8 @@ def lib():
             "pbfx_ns_neighbors": ([vp, vp, vp, vp], ctypes.c_int),
             "pbfx_ns_cell_ranges": ([vp, vp, vp, vp], ctypes.c_int),
             "pbfx_scaling_factor": ([vp, vp, vp], ctypes.c_int),
+            "pbfx_density": ([vp, vp, vp, vp], ctypes.c_int),
+            "pbfx_restore_order": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "pbfx_set_tuning": ([vp, u32, u32], ctypes.c_int),
             "pbfx_set_report_info": ([vp, ctypes.c_int], None),
             "pbfx_get_grid": ([vp, u32p], ctypes.c_int),
@@ -293,6 +295,15 @@ class Solver:
 
     def scaling_factor(self, out, stream=None) -> None:
         _check(self._lib.pbfx_scaling_factor(self._ctx, _ptr(out), _stream(stream)), "pbfx_scaling_factor")
+
+    def density(self, x_sorted, rho, stream=None) -> None:
+        """rho_i over the last step's lists (device AoS x in its sorted order)."""
+        _check(self._lib.pbfx_density(self._ctx, _ptr(x_sorted), _ptr(rho), _stream(stream)), "pbfx_density")
+
+    def restore_order(self, x_sorted, v_sorted, x_out, v_out, stream=None) -> None:
+        """x_out[sorted_index[s]] = x_sorted[s] (and v) for the last step's live slots."""
+        _check(self._lib.pbfx_restore_order(self._ctx, _ptr(x_sorted), _ptr(v_sorted), _ptr(x_out), _ptr(v_out),
+                                            _stream(stream)), "pbfx_restore_order")
 
 
 def selftest_arith(a, b, out, stream=None) -> None:

@@ -174,3 +174,21 @@ def test_fp64_twin_agrees_with_fp32(iters):
     print(f"fp32 vs fp64, {iters} iteration(s): rel_x={ex:.2e} rel_v={ev:.2e}")
     assert ex < 1e-5
     assert ev < 1e-4
+
+
+def test_density_over_lists_is_rest_density_inside_a_lattice():
+    # rho_i = m * sum_j W over the (uncapped) lists at the positions the lists
+    # were built on: interior particles of the exact lattice the rest density
+    # was defined on (rho0 = m * lattice sum of W) get rho0 back.
+    h = 0.1
+    x = lattice((12, 12, 12), h, origin=(1.0, 1.0, 1.0))
+    sc = custom_scene(x, np.zeros_like(x), h=h, max_pair=64)
+    r = orc.step(sc, want_ns=True)
+    p0 = r["pred0"].reshape(-1, 3).astype(np.float64)
+    rho = orc.density(sc, p0.astype(np.float32), r["neighbors"])
+    s = sc.params.stable_distance
+    lo, hi = p0.min(0) + 2.5 * s, p0.max(0) - 2.5 * s
+    inner = np.all((p0 > lo) & (p0 < hi), axis=1)
+    assert inner.sum() > 100
+    np.testing.assert_allclose(rho[inner], sc.params.stable_density, rtol=1e-5)
+    assert rho[~inner].max() < sc.params.stable_density * (1 + 1e-5)  # surface particles miss neighbours

@@ -860,20 +860,25 @@ __global__ __launch_bounds__(256) void k_deltap(Grid G, Phys P, Boundary B, Bufs
     flush_tally(b.ctr, tl);
 }
 
-// ------------------------------------------------- final-cell sort
+// ------------------------------------------------- final-cell regrouping
 // The vorticity, confinement and XSPH sums visit the 27 cells around each
 // particle's FINAL position, bucketed by the search's cell ranges
 // (pbf/interaction/cuda/detail/pbf_contribution.inl:7-57).  Particles are
-// regrouped by final cell (order inside a cell is irrelevant: every sum is
-// per particle) so that each unit's own particles have their 27 cells in the
-// unit's tile.  The same pass packs each slot's record R = (x*, 0), (v, 0)
-// with v = inv_dt*(x* - x_sorted) (updateVelocityCUDA, pbf_update.cu:25-39),
-// so that a particle visited in final-cell order costs one cache line.
+// regrouped by run of uw final cells along x (the work unit of the final
+// passes; the order inside a run is irrelevant, every sum is per particle):
+// a counting sort over C/uw runs.  Each run has nc counters, one per
+// group of workgroups (b mod nc, nc <= 8, i.e. per XCD), so that a run that
+// collects particles from all over the domain -- a wall or corner pile --
+// is not counted through one contended word; key = run * nc + copy.  The same
+// pass packs each slot's record R = (x*, 0), (v, 0) with
+// v = inv_dt*(x* - x_sorted) (updateVelocityCUDA, pbf_update.cu:25-39), so
+// that a particle visited in run order costs one cache line.
 __global__ __launch_bounds__(256) void k_cur_hash(const float4* __restrict__ Pf, const float* __restrict__ X,
                                                   float inv_dt, uint32_t cap, const uint32_t* __restrict__ offset,
-                                                  Grid G, uint32_t* __restrict__ ckey, uint32_t* __restrict__ crank,
-                                                  uint32_t* __restrict__ ccount, uint32_t* __restrict__ ctflag,
-                                                  uint32_t lgT, float4* __restrict__ R) {
+                                                  Grid G, uint32_t lguw, uint32_t lgnc, uint32_t* __restrict__ ckey,
+                                                  uint32_t* __restrict__ crank, uint32_t* __restrict__ ccount,
+                                                  uint32_t* __restrict__ ctflag, uint32_t lgT,
+                                                  float4* __restrict__ R) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nl = offset[G.C];
     const bool active = s < nl;
@@ -885,7 +890,7 @@ __global__ __launch_bounds__(256) void k_cur_hash(const float4* __restrict__ Pf,
         R[2 * s + 1] = make_float4(vv.x, vv.y, vv.z, 0.f);
         int gp[3];
         grid_pos(p, G.cw, gp);
-        k = grid_hash(gp[0], gp[1], gp[2], G);
+        k = ((grid_hash(gp[0], gp[1], gp[2], G) >> lguw) << lgnc) | (blockIdx.x & ((1u << lgnc) - 1u));
     }
     const uint32_t r = wave_rank(ccount, k, active);
     if (active) {
@@ -895,18 +900,30 @@ __global__ __launch_bounds__(256) void k_cur_hash(const float4* __restrict__ Pf,
     }
 }
 
+// Places each particle at its run's position and emits the work items of
+// the runs holding at least kSparse particles (one per kChunk of them).
 __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict__ offset, Grid G,
                                                      const uint32_t* __restrict__ ckey,
                                                      const uint32_t* __restrict__ crank,
                                                      const uint32_t* __restrict__ coffset,
-                                                     uint32_t* __restrict__ cperm, uint32_t uw, uint32_t budget,
-                                                     uint2* __restrict__ cunits, uint32_t* __restrict__ ctr) {
+                                                     uint32_t* __restrict__ cperm, uint32_t uw, uint32_t lguw,
+                                                     uint32_t lgnc, uint2* __restrict__ cunits,
+                                                     uint32_t* __restrict__ ctr) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= offset[G.C]) return;
     const uint32_t k = ckey[s];
     const uint32_t slot = coffset[k] + crank[s];
     cperm[slot] = s;
-    emit_units(coffset, k, slot, uw, budget, kSparse, cunits, &ctr[CTR_CUNITS]);
+    const uint32_t run = k >> lgnc;
+    const uint32_t r0 = coffset[run << lgnc];
+    if (slot == r0) {
+        const uint32_t own = coffset[(run + 1u) << lgnc] - r0;
+        if (own >= kSparse) {
+            const uint32_t nch = (own + kChunk - 1u) / kChunk;
+            const uint32_t u = atomicAdd(&ctr[CTR_CUNITS], nch);
+            for (uint32_t c = 0; c < nch; ++c) cunits[u + c] = make_uint2(run << lguw, uw | (c << 5));
+        }
+    }
 }
 
 // The 27-cell neighbourhood of a particle's final cell, restated from
@@ -1029,7 +1046,7 @@ __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __rest
 // a contiguous share of that order per XCD.  Sparse runs are contiguous
 // stretches of the order.  body(q, g): position q in that order, slot g.
 template <class F>
-__device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_t uw, F&& body) {
+__device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_t lgnc, F&& body) {
     const uint32_t nl = b.offset[G.C];
     uint32_t c0, c1, cs;
     unit_range((nl + kBlock - 1) / kBlock, c0, c1, cs);
@@ -1037,20 +1054,20 @@ __device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_
         const uint32_t q = c * kBlock + threadIdx.x;
         if (q >= nl) continue;
         const uint32_t g = b.cperm[q];
-        const uint32_t kb = b.ckey[g] & ~(uw - 1u);
-        if (b.coffset[kb + uw] - b.coffset[kb] < kSparse) body(q, g);
+        const uint32_t run = b.ckey[g] >> lgnc;
+        if (b.coffset[(run + 1u) << lgnc] - b.coffset[run << lgnc] < kSparse) body(q, g);
     }
 }
 
-// Own particles of a final-cell work item: positions coffset[kb] + chunk*kChunk .. of the final-cell order.
+// Own particles of a final-run work item: positions coffset[run] + chunk*kChunk .. of the run order.
 struct OwnCur {
     uint32_t c0, n;
 };
-__device__ __forceinline__ OwnCur own_cur(const Bufs& b, uint2 unit) {
+__device__ __forceinline__ OwnCur own_cur(const Bufs& b, uint2 unit, uint32_t lguw, uint32_t lgnc) {
     OwnCur o;
-    const uint32_t W = unit.y & 31u, skip = (unit.y >> 5) * kChunk;
-    o.c0 = b.coffset[unit.x] + skip;
-    o.n = min(kChunk, b.coffset[unit.x + W] - o.c0);
+    const uint32_t run = unit.x >> lguw, skip = (unit.y >> 5) * kChunk;
+    o.c0 = b.coffset[run << lgnc] + skip;
+    o.n = min(kChunk, b.coffset[(run + 1u) << lgnc] - o.c0);
     return o;
 }
 
@@ -1071,7 +1088,8 @@ __device__ __forceinline__ float4 pos_slot(const float4* __restrict__ R, uint32_
 // For each particle (by position q of the final-cell order) the slots of its
 // in-range pairs, column-major: flist[k * cap + q], k < kFinalK; fcnt[q] =
 // the full count (above kFinalK: the passes fall back to the direct sum).
-__global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t uw) {
+__global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t lguw,
+                                                       uint32_t lgnc) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
@@ -1102,19 +1120,19 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, u
         if (I.ok) {
             if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_slot(R, j); });
             __syncthreads();
-            const OwnCur o = own_cur(b, unit);
+            const OwnCur o = own_cur(b, unit, lguw, lgnc);
             for (uint32_t i = threadIdx.x; i < o.n; i += kBlock) {
                 if (!I.staged) tl.slowpart++;
                 build(o.c0 + i, b.cperm[o.c0 + i], I.staged);
             }
         } else {
             // no tile index range: the passes sum these particles directly
-            const OwnCur o = own_cur(b, unit);
+            const OwnCur o = own_cur(b, unit, lguw, lgnc);
             for (uint32_t i = threadIdx.x; i < o.n; i += kBlock) b.fcnt[o.c0 + i] = 0xFFFFFFFFu;
         }
         __syncthreads();
     }
-    for_sparse(G, b, uw, [&](uint32_t q, uint32_t g) {
+    for_sparse(G, b, lgnc, [&](uint32_t q, uint32_t g) {
         tl.slowpart++;
         build(q, g, false);
     });
@@ -1467,21 +1485,30 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
                            x, v);
         return hipGetLastError();
     }
+    // runs of uw final cells, nc = min(8, uw) counters each: CR = C / uw * nc
+    // counters (never more than C), scanned in tiles of up to 4096
+    uint32_t lguw = 0;
+    while ((1u << lguw) < uw) ++lguw;
+    const uint32_t lgnc = lguw < 3u ? lguw : 3u;
+    const uint32_t CR = (G.C >> lguw) << lgnc;
+    uint32_t lgTr = 0;
+    while ((1u << (lgTr + 1)) <= CR && lgTr < 12u) ++lgTr;
+    const uint32_t nTr = CR >> lgTr;
     {
         Mark m(tm, s, K_CUR_SORT);
         hipLaunchKernelGGL(k_cur_hash, dim3(nb), dim3(256), 0, s, (const float4*)b.Pf, (const float*)b.X, P.inv_dt,
-                           b.cap, b.offset, G, b.ckey, b.crank, b.ccount, b.ctflag, b.lgT, b.R);
-        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, b.nT, b.lgT,
-                           G.C);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.ctsum, b.ctprefix, b.nT + 1, b.ctr, 0u);
-        hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(256), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
-                           b.ctprefix, b.nT, b.lgT, G.C);
+                           b.cap, b.offset, G, lguw, lgnc, b.ckey, b.crank, b.ccount, b.ctflag, lgTr, b.R);
+        hipLaunchKernelGGL(k_tile_reduce, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, nTr, lgTr, CR);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.ctsum, b.ctprefix, nTr + 1, b.ctr, 0u);
+        hipLaunchKernelGGL(k_scan_cells, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
+                           b.ctprefix, nTr, lgTr, CR);
         hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, G, b.ckey, b.crank, b.coffset,
-                           b.cperm, uw, budget, b.cunits, b.ctr);
+                           b.cperm, uw, lguw, lgnc, b.cunits, b.ctr);
     }
     {
         Mark m(tm, s, K_FINAL_LIST);
-        hipLaunchKernelGGL(k_final_list, grid(K_FINAL_LIST), dim3(kBlock), smem, s, G, P, b, T.tile_cap, uw);
+        hipLaunchKernelGGL(k_final_list, grid(K_FINAL_LIST), dim3(kBlock), smem, s, G, P, b, T.tile_cap, lguw,
+                           lgnc);
     }
     if (vort_on) {
         {

@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -54,6 +56,7 @@ def parse():
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
     ap.add_argument("--steps-8m", type=int, default=200, help="timed steps of the 8M workload (0: skip)")
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps of the per-kernel event pass")
+    ap.add_argument("--batch-steps", type=int, default=10, help="timed steps of the 256-scene batch (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--tile-cap", type=int, default=2048)
     ap.add_argument("--unit-width", type=int, default=16)
@@ -136,6 +139,80 @@ def run_gpu(scene, steps, warmup, kernel_steps, tile_cap, unit_width):
     del xs, vs
     torch.cuda.empty_cache()
     return {"ms_total": ms, "wall_s": wall, "steps": steps, "kernels": kt, "stats": stats}
+
+
+def run_batch(scs, steps, warmup):
+    """The batched form against a per-scene loop, on the same GPU and input.
+
+    batched: one pbfx_batch_step_async for all scenes, captured in a hipGraph
+    together with the restore of the input (each timed step starts from the
+    workload's input, as in run_gpu).  loop: one plain context per scene and
+    one pbfx_step_async per scene, the same restore first.  Returns ms per
+    step of each and whether one step of each gives identical bytes."""
+    import torch
+
+    from ssf_pbf_cuda_amd import pbfx
+
+    counts = [s.n for s in scs]
+    n = sum(counts)
+    iters = scs[0].iters
+    x0 = torch.from_numpy(np.concatenate([s.x for s in scs])).cuda()
+    v0 = torch.from_numpy(np.concatenate([s.v for s in scs])).cuda()
+    off = torch.tensor(np.concatenate([[0], np.cumsum(counts)]).astype(np.int32), device="cuda")
+    x, v = x0.clone(), v0.clone()
+    stream = torch.cuda.Stream()
+    batch = pbfx.BatchSolver(scs)
+    with torch.cuda.stream(stream):
+        for _ in range(max(warmup, 1)):
+            x.copy_(x0)
+            v.copy_(v0)
+            batch.batch_step_async(x, v, off, n, iters, stream=stream)
+    stream.synchronize()
+    xb, vb = x.clone(), v.clone()  # one batched step from the input
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        x.copy_(x0)
+        v.copy_(v0)
+        batch.batch_step_async(x, v, off, n, iters, stream=stream)
+    torch.cuda.synchronize()
+    log(f"batch of {len(scs)} scenes ({n} particles): graph captured, timing {steps} steps")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_batch = e0.elapsed_time(e1) / steps
+    batch.close()
+    del g
+    log(f"batch: {ms_batch:.3f} ms/step; per-scene loop")
+    solvers = [pbfx.Solver.for_scene(sc) for sc in scs]
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    cur = torch.cuda.current_stream()
+
+    def loop_step():
+        x.copy_(x0)
+        v.copy_(v0)
+        for sv, sc, o, c in zip(solvers, scs, offs[:-1], counts):
+            sv.step_async(x[o:o + c], v[o:o + c], c, sc.area_lo, sc.area_hi, iters, cur)
+
+    for _ in range(max(warmup, 1)):
+        loop_step()
+    torch.cuda.synchronize()
+    same = bool(torch.equal(x.view(torch.int32), xb.view(torch.int32)) and
+                torch.equal(v.view(torch.int32), vb.view(torch.int32)))
+    e0.record()
+    for _ in range(steps):
+        loop_step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_loop = e0.elapsed_time(e1) / steps
+    for sv in solvers:
+        sv.close()
+    log(f"loop: {ms_loop:.3f} ms/step")
+    del x, v, x0, v0, xb, vb
+    torch.cuda.empty_cache()
+    return {"ms_batch": ms_batch, "ms_loop": ms_loop, "bit_exact": same, "particles": n}
 
 
 def kernel_report(kt, n, cells):
@@ -248,6 +325,20 @@ def main():
             "step_roofline_frac": round(sb8 / (ms8 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "kernels": kernel_report(r8["kernels"], s8.n, s8.cells),
             "state": {k: r8["stats"][k] for k in ("units", "slow_units", "max_tile", "truncated")},
+        }
+    if a.batch_steps:
+        scs = scenes.batch_config3()
+        rb = run_batch(scs, a.batch_steps, min(a.warmup, 2))
+        out["batched"] = {
+            "workload": f"{len(scs)} dam-break scenes, {rb['particles']} particles (4,096-200,000 each, "
+                        f"log-uniform), per-scene rest density / XSPH / vorticity, 4 iterations",
+            "value": round(rb["particles"] / (rb["ms_batch"] * 1e-3) / 1e6, 3),
+            "unit": "M particle-steps/s",
+            "ms_per_step": round(rb["ms_batch"], 4),
+            "loop_ms_per_step": round(rb["ms_loop"], 4),
+            "speedup_vs_loop": round(rb["ms_loop"] / rb["ms_batch"], 3),
+            "bit_exact_vs_loop": rb["bit_exact"],
+            "steps": a.batch_steps,
         }
     if a.cpu_seconds > 0:
         log(f"cpu baseline ({a.cpu_seconds:.0f} s budget)")

@@ -140,6 +140,48 @@ pbfx_status pbfx_get_num(pbfx_ctx* ctx, void* stream, uint32_t* num);
 pbfx_status pbfx_step_no_boundary(pbfx_ctx* ctx, float* x, float* v, uint32_t* num_inout,
                                   const float lo[3], const float hi[3], int iters, void* stream);
 
+/* ---------------------------------------------------------------- batched form
+ * Many independent scenes advanced by ONE sequence of launches per step: the
+ * scenes occupy disjoint ranges of one cell-key space, are sorted together,
+ * and every kernel reads its particle's scene parameters, grid and boundary
+ * from a table.  Each scene evolves exactly as it would alone (pbfx_step):
+ * same sort order within the scene, same neighbour lists, same arithmetic.
+ * No counterpart in the reference, which runs one scene per process from
+ * global constant memory (pbf/solver/cuda/pbf_position_update.cu:14-17,224-238);
+ * the per-scene semantics are pbf::one_step's (pbf/solver/pbf_solve.cpp:182-216). */
+typedef struct pbfx_scene_desc {
+    pbfx_params params;          /* max_pair must be equal across the batch */
+    uint32_t capacity;           /* particles this scene may hold */
+    float domain_end[3];         /* grid = calcGridSize(2h, domain_end) */
+    const float* inner_spheres4; /* host, float4 (center, radius) */
+    uint32_t n_inner;
+    const float* outer_spheres4;
+    uint32_t n_outer;
+    const float* plane_points3;  /* host, float3 */
+    const float* plane_normals3;
+    uint32_t n_planes;
+    float area_lo[3], area_hi[3];/* simulation area (deletion bounds) */
+} pbfx_scene_desc;
+
+pbfx_status pbfx_batch_create(const pbfx_scene_desc* scenes, uint32_t n_scenes, pbfx_ctx** out);
+
+/* x, v: device AoS float3, the scenes concatenated in order (scene s starts
+ * at the sum of the counts before it).  counts: host [n_scenes], live
+ * particles per scene in, and after the step (one host synchronisation)
+ * live particles per scene out; the output is again concatenated in order. */
+pbfx_status pbfx_batch_step(pbfx_ctx* ctx, float* x, float* v, uint32_t* counts, int iters, void* stream);
+
+/* The same without host synchronisation, for streams of steps and graph
+ * capture: in_offsets is device [n_scenes+1], the first input index of each
+ * scene and then the total; pbfx_batch_offsets writes the output's offsets
+ * in that form (the next step's in_offsets). */
+pbfx_status pbfx_batch_step_async(pbfx_ctx* ctx, float* x, float* v, const uint32_t* in_offsets, uint32_t total,
+                                  int iters, void* stream);
+pbfx_status pbfx_batch_offsets(pbfx_ctx* ctx, uint32_t* out_offsets, void* stream);
+
+/* Replaces one scene's parameters (its grid must not change). */
+pbfx_status pbfx_batch_set_params(pbfx_ctx* ctx, uint32_t scene, const pbfx_params* params);
+
 /* Views of the last neighbour search, for bit-exact checks.  Replaces the
  * neighbor_search getters (pbf/interaction/pbf_neighbor_search.h:49-56).
  * Device pointers owned by the context, valid until the next step. */

@@ -7,19 +7,16 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <vector>
 
 using namespace pbfx;
 
-struct pbfx_ctx {
-    int device = 0;
+// One scene as the host keeps it: its parameters, grid, boundary arrays and area.
+struct SceneHost {
     pbfx_params params{};
     uint32_t capacity = 0;
     float domain_end[3] = {0, 0, 0};
     Grid G{};
-    Bufs b{};
-    Tuning T{};
-    int num_cu = 0;
-    int blocks_per_cu = 1;
     float4* d_inner = nullptr;
     uint32_t n_inner = 0;
     float4* d_outer = nullptr;
@@ -27,13 +24,35 @@ struct pbfx_ctx {
     float* d_pts = nullptr;
     float* d_nrm = nullptr;
     uint32_t n_planes = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+
+struct pbfx_ctx {
+    int device = 0;
+    std::vector<SceneHost> sc;
+    bool batch = false;                  // created by pbfx_batch_create
+    uint32_t capacity = 0;               // all scenes
+    uint32_t csum = 0;                   // cells of all scenes (kbase[n])
+    uint32_t uw_max = 16;                // narrowest grid row of any scene
+    Bufs b{};
+    Tuning T{};
+    int num_cu = 0;
+    int blocks_per_cu = 1;
+    Scene* d_scenes = nullptr;           // scene table on the device
+    Scene* h_scenes = nullptr;           // pinned staging of the table
+    std::vector<Scene> last_table;       // what the device holds
+    uint32_t* d_kbase = nullptr;
+    uint32_t* d_in_off = nullptr;        // per-step input offsets (pbfx_batch_step)
+    uint32_t* h_in_off = nullptr;        // pinned
+    uint32_t* d_scene_off = nullptr;     // per-scene output offsets (pbfx_batch_step)
+    uint32_t* h_scene_off = nullptr;     // pinned
     float4* d_default_sphere = nullptr;  // pbf_constraint.cu:300: centre (3,3,3), radius 3
     uint32_t* h_pinned = nullptr;        // kCtrWords counters + live + in
     int last_err = 0;
     int report_info = 0;
     uint32_t steps = 0;
     uint32_t list_stride = 0;
-    Timer* timer = nullptr;          // per-launch events when kernel timing is on
+    Timer* timer = nullptr;              // per-launch events when kernel timing is on
     double kt_ms[K_NUM] = {};
     uint32_t kt_n[K_NUM] = {};
 };
@@ -128,6 +147,8 @@ Phys make_phys(const pbfx_params& p) {
     P.outer_as_inner = p.outer_as_inner;
     P.cut2_w = support_bound2(P.inv_h);
     P.cut2_wd = support_bound2(P.warg);
+    P.xsph_on = p.xsph_on;
+    P.vort_on = p.vort_on;
     return P;
 }
 
@@ -143,22 +164,35 @@ void free_timer(pbfx_ctx* c) {
     c->timer = nullptr;
 }
 
+void free_boundary(SceneHost& h) {
+    void* ptrs[] = {h.d_inner, h.d_outer, h.d_pts, h.d_nrm};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    h.d_inner = h.d_outer = nullptr;
+    h.d_pts = h.d_nrm = nullptr;
+    h.n_inner = h.n_outer = h.n_planes = 0;
+}
+
 void free_bufs(pbfx_ctx* c) {
     free_timer(c);
     Bufs& b = c->b;
-    void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.Pf,     b.Q,     b.X,      b.R,
+    void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.Pf,     b.Q,     b.X,       b.R,
                     b.OM4,   b.ncnt,    b.nlist,   b.count, b.offset, b.tsum,  b.tprefix, b.units,
                     b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
-                    b.tflag, b.ctflag, b.flist, b.fcnt,
-                    b.ctr,   c->d_inner, c->d_outer, c->d_pts, c->d_nrm, c->d_default_sphere};
+                    b.tflag, b.ctflag,  b.flist,   b.fcnt,  b.isid,   b.sid,   b.ctr,
+                    c->d_scenes, c->d_kbase, c->d_in_off, c->d_scene_off, c->d_default_sphere};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    void* pinned[] = {c->h_pinned, c->h_scenes, c->h_in_off, c->h_scene_off};
+    for (void* p : pinned)
+        if (p) (void)hipHostFree(p);
+    for (auto& h : c->sc) free_boundary(h);
     std::memset(&c->b, 0, sizeof(c->b));
-    c->d_inner = c->d_outer = nullptr;
-    c->d_pts = c->d_nrm = nullptr;
+    c->d_scenes = nullptr;
+    c->d_kbase = c->d_in_off = c->d_scene_off = nullptr;
     c->d_default_sphere = nullptr;
-    c->h_pinned = nullptr;
+    c->h_pinned = c->h_in_off = c->h_scene_off = nullptr;
+    c->h_scenes = nullptr;
 }
 
 pbfx_status hip_fail(pbfx_ctx* c, hipError_t e) {
@@ -170,7 +204,7 @@ pbfx_status alloc_list(pbfx_ctx* c, uint32_t stride) {
     if (c->b.nlist && c->list_stride >= stride) return PBFX_OK;
     if (c->b.nlist) (void)hipFree(c->b.nlist);
     c->b.nlist = nullptr;
-    hipError_t e = dalloc(&c->b.nlist, (size_t)c->capacity * stride);
+    hipError_t e = dalloc(&c->b.nlist, (size_t)c->b.cap * stride);
     if (e != hipSuccess) return hip_fail(c, e);
     c->list_stride = stride;
     return PBFX_OK;
@@ -181,35 +215,85 @@ void set_grid_units(pbfx_ctx* c) {
     c->blocks_per_cu = (int)(c->T.grid_units / (uint32_t)(c->num_cu > 0 ? c->num_cu : 1));
 }
 
-pbfx_status do_step(pbfx_ctx* c, float* x, float* v, uint32_t n, const float lo[3], const float hi[3], int iters,
-                    const Boundary& B, hipStream_t s) {
+pbfx_status make_grid(const pbfx_params& p, const float end[3], Grid& G);
+
+// The device table as the scenes stand now.
+void build_table(pbfx_ctx* c, std::vector<Scene>& t, bool default_sphere) {
+    t.resize(c->sc.size());
+    for (size_t i = 0; i < c->sc.size(); ++i) {
+        const SceneHost& h = c->sc[i];
+        Scene S;
+        std::memset(&S, 0, sizeof(S));
+        S.G = h.G;
+        S.P = make_phys(h.params);
+        if (default_sphere) {
+            S.B.inner = c->d_default_sphere;
+            S.B.n_inner = 1;
+        } else {
+            S.B.inner = h.d_inner;
+            S.B.n_inner = h.n_inner;
+            S.B.outer = h.d_outer;
+            S.B.n_outer = h.n_outer;
+            S.B.pts = h.d_pts;
+            S.B.nrm = h.d_nrm;
+            S.B.n_planes = h.n_planes;
+        }
+        for (int a = 0; a < 3; ++a) {
+            S.lo[a] = h.lo[a];
+            S.hi[a] = h.hi[a];
+        }
+        t[i] = S;
+    }
+}
+
+// Uploads the scene table when it differs from what the device holds (a
+// parameter, boundary or area change); the copy completes before returning.
+pbfx_status sync_table(pbfx_ctx* c, bool default_sphere, hipStream_t s) {
+    std::vector<Scene> t;
+    build_table(c, t, default_sphere);
+    if (t.size() == c->last_table.size() &&
+        std::memcmp(t.data(), c->last_table.data(), t.size() * sizeof(Scene)) == 0)
+        return PBFX_OK;
+    std::memcpy(c->h_scenes, t.data(), t.size() * sizeof(Scene));
+    hipError_t e = hipMemcpyAsync(c->d_scenes, c->h_scenes, t.size() * sizeof(Scene), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e);
+    c->last_table = t;
+    return PBFX_OK;
+}
+
+bool any_flag(const pbfx_ctx* c, int which) {
+    for (const auto& h : c->sc)
+        if (which ? h.params.vort_on : h.params.xsph_on) return true;
+    return false;
+}
+
+uint32_t unit_width(const pbfx_ctx* c) {
+    const uint32_t uw = c->T.unit_width;
+    return uw < c->uw_max ? uw : c->uw_max;
+}
+
+pbfx_status do_step(pbfx_ctx* c, float* x, float* v, uint32_t n, const uint32_t* in_off, int iters,
+                    bool default_sphere, hipStream_t s) {
     if (n > c->capacity) return PBFX_CAPACITY;
-    if (iters < 1 || (n > 0 && (!x || !v)) || !lo || !hi) return PBFX_INVALID_ARG;
-    Phys P = make_phys(c->params);
-    if ((uint32_t)P.stride > c->list_stride) return PBFX_INVALID_ARG;
-    hipError_t e = launch_step(c->G, P, B, c->b, c->T, x, v, n, lo, hi, iters, c->params.xsph_on,
-                               c->params.vort_on, s, c->timer);
+    if (iters < 1 || (n > 0 && (!x || !v))) return PBFX_INVALID_ARG;
+    for (const auto& h : c->sc)
+        if ((uint32_t)make_phys(h.params).stride > c->list_stride) return PBFX_INVALID_ARG;
+    pbfx_status st = sync_table(c, default_sphere, s);
+    if (st != PBFX_OK) return st;
+    Bufs b = c->b;
+    b.in_off = in_off;
+    hipError_t e = launch_step(b, c->last_table[0], c->T, x, v, n, iters, any_flag(c, 0), any_flag(c, 1), unit_width(c),
+                               s, c->timer);
     if (e != hipSuccess) return hip_fail(c, e);
     c->steps++;
     return PBFX_OK;
 }
 
-Boundary user_boundary(const pbfx_ctx* c) {
-    Boundary B{};
-    B.inner = c->d_inner;
-    B.n_inner = c->n_inner;
-    B.outer = c->d_outer;
-    B.n_outer = c->n_outer;
-    B.pts = c->d_pts;
-    B.nrm = c->d_nrm;
-    B.n_planes = c->n_planes;
-    return B;
-}
-
 pbfx_status read_counters(pbfx_ctx* c, hipStream_t s) {
     hipError_t e = hipMemcpyAsync(c->h_pinned, c->b.ctr, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(c->h_pinned + kCtrWords, c->b.offset + c->G.C, 2 * sizeof(uint32_t),
+        e = hipMemcpyAsync(c->h_pinned + kCtrWords, c->b.offset + c->b.CT, 2 * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e);
@@ -228,15 +312,195 @@ pbfx_status read_counters(pbfx_ctx* c, hipStream_t s) {
     return PBFX_OK;
 }
 
-pbfx_status sync_step_result(pbfx_ctx* c, uint32_t* num_inout, hipStream_t s) {
-    pbfx_status st = read_counters(c, s);
-    if (st != PBFX_OK) return st;
-    *num_inout = c->h_pinned[kCtrWords];
+pbfx_status step_status(pbfx_ctx* c) {
     if (c->h_pinned[CTR_OVERFLOW]) return PBFX_TILE_OVERFLOW;
     if (c->report_info) {
         if (c->h_pinned[CTR_TRUNC]) return PBFX_NEIGHBOR_TRUNCATED;
         if (c->h_pinned[CTR_SWEEPCAP]) return PBFX_NOT_CONVERGED;
     }
+    return PBFX_OK;
+}
+
+pbfx_status sync_step_result(pbfx_ctx* c, uint32_t* num_inout, hipStream_t s) {
+    pbfx_status st = read_counters(c, s);
+    if (st != PBFX_OK) return st;
+    *num_inout = c->h_pinned[kCtrWords];
+    return step_status(c);
+}
+
+pbfx_status set_boundary(pbfx_ctx* c, SceneHost& h, const float* inner4, uint32_t n_inner, const float* outer4,
+                         uint32_t n_outer, const float* pts3, const float* nrm3, uint32_t n_planes) {
+    if ((n_inner && !inner4) || (n_outer && !outer4) || (n_planes && (!pts3 || !nrm3))) return PBFX_INVALID_ARG;
+    (void)hipDeviceSynchronize();
+    free_boundary(h);
+    hipError_t e = hipSuccess;
+    if (n_inner) {
+        e = dalloc(&h.d_inner, n_inner);
+        if (e == hipSuccess) e = hipMemcpy(h.d_inner, inner4, n_inner * sizeof(float4), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && n_outer) {
+        e = dalloc(&h.d_outer, n_outer);
+        if (e == hipSuccess) e = hipMemcpy(h.d_outer, outer4, n_outer * sizeof(float4), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && n_planes) {
+        e = dalloc(&h.d_pts, 3 * (size_t)n_planes);
+        if (e == hipSuccess) e = dalloc(&h.d_nrm, 3 * (size_t)n_planes);
+        if (e == hipSuccess) e = hipMemcpy(h.d_pts, pts3, 3 * n_planes * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h.d_nrm, nrm3, 3 * n_planes * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) return hip_fail(c, e);
+    h.n_inner = n_inner;
+    h.n_outer = n_outer;
+    h.n_planes = n_planes;
+    return PBFX_OK;
+}
+
+// Allocates every buffer for the scenes in c->sc (grids set) and the scene table.
+pbfx_status create_common(pbfx_ctx* c) {
+    (void)hipGetDevice(&c->device);
+    const uint32_t n = (uint32_t)c->sc.size();
+    uint32_t cap = 0, csum = 0, uwm = kMaxW;
+    std::vector<uint32_t> kbase(n + 1);
+    for (uint32_t i = 0; i < n; ++i) {
+        SceneHost& h = c->sc[i];
+        h.G.base = csum;
+        kbase[i] = csum;
+        if ((uint64_t)csum + h.G.C > 0x7FFFFFFFull) return PBFX_GRID_TOO_LARGE;
+        csum += h.G.C;
+        if ((uint64_t)cap + h.capacity > 0x7FFFFFFFull) return PBFX_CAPACITY;
+        cap += h.capacity;
+        if (h.G.gx < uwm) uwm = h.G.gx;
+    }
+    kbase[n] = csum;
+    c->capacity = cap;
+    c->csum = csum;
+    c->uw_max = uwm;
+    // key space padded so that scan tiles of T cells cover it exactly
+    uint32_t CT = csum;
+    if (CT <= kScanTile) {
+        uint32_t p2 = 1;
+        while (p2 < CT) p2 <<= 1;
+        CT = p2;
+    } else {
+        CT = (CT + kScanTile - 1) / kScanTile * kScanTile;
+    }
+    Bufs& b = c->b;
+    b.cap = cap > 0 ? cap : 1;
+    b.CT = CT;
+    b.nscenes = n;
+    const uint32_t T = CT < kScanTile ? CT : kScanTile;
+    b.lgT = ilog2(T);
+    b.nT = CT >> b.lgT;
+    const size_t bc = b.cap;
+    const size_t C = CT;
+    hipError_t e = hipSuccess;
+#define PBFX_ALLOC(ptr, cnt)                           \
+    if (e == hipSuccess) e = dalloc(&(ptr), (cnt));
+    PBFX_ALLOC(b.key, bc);
+    PBFX_ALLOC(b.rank, bc);
+    PBFX_ALLOC(b.tmp, bc);
+    PBFX_ALLOC(b.sidx, bc);
+    PBFX_ALLOC(b.isid, bc);
+    PBFX_ALLOC(b.sid, bc);
+    PBFX_ALLOC(b.Pf, bc);
+    PBFX_ALLOC(b.Q, bc);
+    PBFX_ALLOC(b.X, 3 * bc);
+    PBFX_ALLOC(b.R, 2 * bc);
+    PBFX_ALLOC(b.OM4, bc);
+    PBFX_ALLOC(b.ncnt, bc);
+    PBFX_ALLOC(b.count, C + 1);
+    PBFX_ALLOC(b.offset, C + 2);
+    PBFX_ALLOC(b.tsum, (size_t)b.nT + 1);
+    PBFX_ALLOC(b.tprefix, (size_t)b.nT + 1);
+    const size_t items = bc + bc / kChunk + 64 * (size_t)n;  // a unit per own chunk of kChunk particles
+    PBFX_ALLOC(b.units, items);
+    PBFX_ALLOC(b.ckey, bc);
+    PBFX_ALLOC(b.crank, bc);
+    PBFX_ALLOC(b.cperm, bc);
+    PBFX_ALLOC(b.ccount, C + 1);
+    PBFX_ALLOC(b.coffset, C + 2);
+    PBFX_ALLOC(b.ctsum, (size_t)b.nT + 1);
+    PBFX_ALLOC(b.ctprefix, (size_t)b.nT + 1);
+    PBFX_ALLOC(b.cunits, items);
+    PBFX_ALLOC(b.flist, (size_t)kFinalK * bc);
+    PBFX_ALLOC(b.fcnt, bc);
+    PBFX_ALLOC(b.tflag, (size_t)b.nT + 1);
+    PBFX_ALLOC(b.ctflag, (size_t)b.nT + 1);
+    PBFX_ALLOC(b.ctr, kCtrWords);
+    PBFX_ALLOC(c->d_scenes, n);
+    PBFX_ALLOC(c->d_kbase, n + 1);
+    PBFX_ALLOC(c->d_in_off, n + 1);
+    PBFX_ALLOC(c->d_scene_off, n + 1);
+    PBFX_ALLOC(c->d_default_sphere, 1);
+#undef PBFX_ALLOC
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_pinned, (kCtrWords + 2) * sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_scenes, n * sizeof(Scene), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_in_off, (n + 1) * sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_scene_off, (n + 1) * sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemset(b.count, 0, (C + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.offset, 0, (C + 2) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.tsum, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.ccount, 0, (C + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.coffset, 0, (C + 2) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.ctsum, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.tflag, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.ctflag, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.ctr, 0, kCtrWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.sid, 0, bc * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(c->d_kbase, kbase.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const float4 sph = make_float4(3.f, 3.f, 3.f, 3.f);
+        e = hipMemcpy(c->d_default_sphere, &sph, sizeof(sph), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (e != hipSuccess) return hip_fail(c, e);
+    b.scenes = c->d_scenes;
+    b.kbase = c->d_kbase;
+    b.in_off = c->d_in_off;
+    uint32_t stride = 0;
+    for (const auto& h : c->sc) {
+        const uint32_t st = (uint32_t)make_phys(h.params).stride;
+        stride = st > stride ? st : stride;
+    }
+    pbfx_status st = alloc_list(c, stride);
+    if (st != PBFX_OK) return st;
+    c->T.tile_cap = 2048;
+    c->T.unit_width = 16;
+    set_grid_units(c);
+    (void)hipDeviceSynchronize();
+    return PBFX_OK;
+}
+
+pbfx_status make_grid(const pbfx_params& p, const float end[3], Grid& G) {
+    uint32_t grid[3];
+    // pbf_type.cpp:172-174: cell width h, grid from calcGridSize(2h, domain_end)
+    pbfx_status st = pbfx_calc_grid_size(p.smoothing_length * 2.0, end, grid);
+    if (st != PBFX_OK) return st;
+    G.gx = grid[0];
+    G.gy = grid[1];
+    G.gz = grid[2];
+    G.lgx = ilog2(G.gx);
+    G.lgy = ilog2(G.gy);
+    G.C = G.gx * G.gy * G.gz;
+    G.cw = p.smoothing_length;
+    G.base = 0;
+    return PBFX_OK;
+}
+
+pbfx_status set_scene_params(pbfx_ctx* c, SceneHost& h, const pbfx_params* p) {
+    pbfx_status st = check_params(p);
+    if (st != PBFX_OK) return st;
+    if (p->smoothing_length != h.params.smoothing_length) {
+        Grid G{};
+        st = make_grid(*p, h.domain_end, G);
+        if (st != PBFX_OK) return st;
+        if (G.gx != h.G.gx || G.gy != h.G.gy || G.gz != h.G.gz) return PBFX_INVALID_ARG;
+        h.G.cw = p->smoothing_length;
+    }
+    st = alloc_list(c, (uint32_t)make_phys(*p).stride);
+    if (st != PBFX_OK) return st;
+    h.params = *p;
+    set_grid_units(c);
     return PBFX_OK;
 }
 
@@ -278,93 +542,56 @@ pbfx_status pbfx_create(const pbfx_params* params, uint32_t capacity, const floa
     *out = nullptr;
     pbfx_status st = check_params(params);
     if (st != PBFX_OK) return st;
-    uint32_t grid[3];
-    // pbf_type.cpp:172-174: cell width h, grid from calcGridSize(2h, domain_end)
-    st = pbfx_calc_grid_size(params->smoothing_length * 2.0, domain_end, grid);
+    SceneHost h;
+    h.params = *params;
+    h.capacity = capacity;
+    std::memcpy(h.domain_end, domain_end, sizeof(h.domain_end));
+    st = make_grid(*params, domain_end, h.G);
     if (st != PBFX_OK) return st;
     pbfx_ctx* c = new (std::nothrow) pbfx_ctx();
     if (!c) return PBFX_HIP_ERROR;
-    (void)hipGetDevice(&c->device);
-    c->params = *params;
-    c->capacity = capacity;
-    std::memcpy(c->domain_end, domain_end, sizeof(c->domain_end));
-    Grid& G = c->G;
-    G.gx = grid[0];
-    G.gy = grid[1];
-    G.gz = grid[2];
-    G.lgx = ilog2(G.gx);
-    G.lgy = ilog2(G.gy);
-    G.C = G.gx * G.gy * G.gz;
-    G.cw = params->smoothing_length;
-    Bufs& b = c->b;
-    b.cap = capacity > 0 ? capacity : 1;
-    const uint32_t T = G.C < kScanTile ? G.C : kScanTile;
-    b.lgT = ilog2(T);
-    b.nT = G.C >> b.lgT;
-    const size_t cap = b.cap;
-    hipError_t e = hipSuccess;
-#define PBFX_ALLOC(ptr, n)                           \
-    if (e == hipSuccess) e = dalloc(&(ptr), (n));
-    PBFX_ALLOC(b.key, cap);
-    PBFX_ALLOC(b.rank, cap);
-    PBFX_ALLOC(b.tmp, cap);
-    PBFX_ALLOC(b.sidx, cap);
-    PBFX_ALLOC(b.Pf, cap);
-    PBFX_ALLOC(b.Q, cap);
-    PBFX_ALLOC(b.X, 3 * cap);
-    PBFX_ALLOC(b.R, 2 * cap);
-    PBFX_ALLOC(b.OM4, cap);
-    PBFX_ALLOC(b.ncnt, cap);
-    PBFX_ALLOC(b.count, (size_t)G.C + 1);
-    PBFX_ALLOC(b.offset, (size_t)G.C + 2);
-    PBFX_ALLOC(b.tsum, (size_t)b.nT + 1);
-    PBFX_ALLOC(b.tprefix, (size_t)b.nT + 1);
-    const size_t items = cap + cap / kChunk + 64;  // a unit per own chunk of kChunk particles
-    PBFX_ALLOC(b.units, items);
-    PBFX_ALLOC(b.ckey, cap);
-    PBFX_ALLOC(b.crank, cap);
-    PBFX_ALLOC(b.cperm, cap);
-    PBFX_ALLOC(b.ccount, (size_t)G.C + 1);
-    PBFX_ALLOC(b.coffset, (size_t)G.C + 2);
-    PBFX_ALLOC(b.ctsum, (size_t)b.nT + 1);
-    PBFX_ALLOC(b.ctprefix, (size_t)b.nT + 1);
-    PBFX_ALLOC(b.cunits, items);
-    PBFX_ALLOC(b.flist, (size_t)kFinalK * cap);
-    PBFX_ALLOC(b.fcnt, cap);
-    PBFX_ALLOC(b.tflag, (size_t)b.nT + 1);
-    PBFX_ALLOC(b.ctflag, (size_t)b.nT + 1);
-    PBFX_ALLOC(b.ctr, kCtrWords);
-    PBFX_ALLOC(c->d_default_sphere, 1);
-#undef PBFX_ALLOC
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_pinned, (kCtrWords + 2) * sizeof(uint32_t), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMemset(b.count, 0, ((size_t)G.C + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.offset, 0, ((size_t)G.C + 2) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.tsum, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.ccount, 0, ((size_t)G.C + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.coffset, 0, ((size_t)G.C + 2) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.ctsum, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.tflag, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.ctflag, 0, ((size_t)b.nT + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b.ctr, 0, kCtrWords * sizeof(uint32_t));
-    if (e == hipSuccess) {
-        const float4 sph = make_float4(3.f, 3.f, 3.f, 3.f);
-        e = hipMemcpy(c->d_default_sphere, &sph, sizeof(sph), hipMemcpyHostToDevice);
-    }
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, c->device);
-    if (e != hipSuccess) {
+    c->sc.push_back(h);
+    st = create_common(c);
+    if (st != PBFX_OK) {
         free_bufs(c);
         delete c;
-        return PBFX_HIP_ERROR;
+        return st;
     }
-    if (alloc_list(c, make_phys(*params).stride) != PBFX_OK) {
+    *out = c;
+    return PBFX_OK;
+}
+
+pbfx_status pbfx_batch_create(const pbfx_scene_desc* scenes, uint32_t n_scenes, pbfx_ctx** out) {
+    if (!out || !scenes || n_scenes == 0) return PBFX_INVALID_ARG;
+    *out = nullptr;
+    for (uint32_t i = 0; i < n_scenes; ++i) {
+        pbfx_status st = check_params(&scenes[i].params);
+        if (st != PBFX_OK) return st;
+        if (scenes[i].params.max_pair != scenes[0].params.max_pair) return PBFX_INVALID_ARG;
+    }
+    pbfx_ctx* c = new (std::nothrow) pbfx_ctx();
+    if (!c) return PBFX_HIP_ERROR;
+    c->batch = true;
+    pbfx_status st = PBFX_OK;
+    for (uint32_t i = 0; i < n_scenes && st == PBFX_OK; ++i) {
+        SceneHost h;
+        h.params = scenes[i].params;
+        h.capacity = scenes[i].capacity;
+        std::memcpy(h.domain_end, scenes[i].domain_end, sizeof(h.domain_end));
+        std::memcpy(h.lo, scenes[i].area_lo, sizeof(h.lo));
+        std::memcpy(h.hi, scenes[i].area_hi, sizeof(h.hi));
+        st = make_grid(h.params, h.domain_end, h.G);
+        c->sc.push_back(h);
+    }
+    if (st == PBFX_OK) st = create_common(c);
+    for (uint32_t i = 0; i < n_scenes && st == PBFX_OK; ++i)
+        st = set_boundary(c, c->sc[i], scenes[i].inner_spheres4, scenes[i].n_inner, scenes[i].outer_spheres4,
+                          scenes[i].n_outer, scenes[i].plane_points3, scenes[i].plane_normals3, scenes[i].n_planes);
+    if (st != PBFX_OK) {
         free_bufs(c);
         delete c;
-        return PBFX_HIP_ERROR;
+        return st;
     }
-    c->T.tile_cap = 2048;
-    c->T.unit_width = 16;
-    set_grid_units(c);
-    (void)hipDeviceSynchronize();
     *out = c;
     return PBFX_OK;
 }
@@ -378,71 +605,48 @@ void pbfx_destroy(pbfx_ctx* c) {
 }
 
 pbfx_status pbfx_set_params(pbfx_ctx* c, const pbfx_params* p) {
-    if (!c) return PBFX_INVALID_ARG;
-    pbfx_status st = check_params(p);
-    if (st != PBFX_OK) return st;
-    if (p->smoothing_length != c->params.smoothing_length) {
-        uint32_t grid[3];
-        st = pbfx_calc_grid_size(p->smoothing_length * 2.0, c->domain_end, grid);
-        if (st != PBFX_OK) return st;
-        if (grid[0] != c->G.gx || grid[1] != c->G.gy || grid[2] != c->G.gz) return PBFX_INVALID_ARG;
-        c->G.cw = p->smoothing_length;
-    }
+    if (!c || c->batch) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    st = alloc_list(c, make_phys(*p).stride);
-    if (st != PBFX_OK) return st;
-    c->params = *p;
-    set_grid_units(c);
-    return PBFX_OK;
+    return set_scene_params(c, c->sc[0], p);
+}
+
+pbfx_status pbfx_batch_set_params(pbfx_ctx* c, uint32_t scene, const pbfx_params* p) {
+    if (!c || scene >= c->sc.size() || !p) return PBFX_INVALID_ARG;
+    if (p->max_pair != c->sc[0].params.max_pair) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    return set_scene_params(c, c->sc[scene], p);
 }
 
 pbfx_status pbfx_set_boundary(pbfx_ctx* c, const float* inner4, uint32_t n_inner, const float* outer4,
                               uint32_t n_outer, const float* pts3, const float* nrm3, uint32_t n_planes) {
-    if (!c) return PBFX_INVALID_ARG;
-    if ((n_inner && !inner4) || (n_outer && !outer4) || (n_planes && (!pts3 || !nrm3))) return PBFX_INVALID_ARG;
+    if (!c || c->batch) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    (void)hipDeviceSynchronize();
-    if (c->d_inner) (void)hipFree(c->d_inner);
-    if (c->d_outer) (void)hipFree(c->d_outer);
-    if (c->d_pts) (void)hipFree(c->d_pts);
-    if (c->d_nrm) (void)hipFree(c->d_nrm);
-    c->d_inner = c->d_outer = nullptr;
-    c->d_pts = c->d_nrm = nullptr;
-    c->n_inner = c->n_outer = c->n_planes = 0;
-    hipError_t e = hipSuccess;
-    if (n_inner) {
-        e = dalloc(&c->d_inner, n_inner);
-        if (e == hipSuccess) e = hipMemcpy(c->d_inner, inner4, n_inner * sizeof(float4), hipMemcpyHostToDevice);
+    return set_boundary(c, c->sc[0], inner4, n_inner, outer4, n_outer, pts3, nrm3, n_planes);
+}
+
+// The plain step on a single-scene context; lo/hi become the scene's area.
+static pbfx_status plain_step(pbfx_ctx* c, float* x, float* v, uint32_t num, const float lo[3], const float hi[3],
+                              int iters, bool default_sphere, hipStream_t s) {
+    if (c->batch || !lo || !hi) return PBFX_INVALID_ARG;
+    for (int a = 0; a < 3; ++a) {
+        c->sc[0].lo[a] = lo[a];
+        c->sc[0].hi[a] = hi[a];
     }
-    if (e == hipSuccess && n_outer) {
-        e = dalloc(&c->d_outer, n_outer);
-        if (e == hipSuccess) e = hipMemcpy(c->d_outer, outer4, n_outer * sizeof(float4), hipMemcpyHostToDevice);
-    }
-    if (e == hipSuccess && n_planes) {
-        e = dalloc(&c->d_pts, 3 * (size_t)n_planes);
-        if (e == hipSuccess) e = dalloc(&c->d_nrm, 3 * (size_t)n_planes);
-        if (e == hipSuccess) e = hipMemcpy(c->d_pts, pts3, 3 * n_planes * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(c->d_nrm, nrm3, 3 * n_planes * sizeof(float), hipMemcpyHostToDevice);
-    }
-    if (e != hipSuccess) return hip_fail(c, e);
-    c->n_inner = n_inner;
-    c->n_outer = n_outer;
-    c->n_planes = n_planes;
-    return PBFX_OK;
+    return do_step(c, x, v, num, c->d_in_off, iters, default_sphere, s);
 }
 
 pbfx_status pbfx_step_async(pbfx_ctx* c, float* x, float* v, uint32_t num, const float lo[3], const float hi[3],
                             int iters, void* stream) {
     if (!c) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    return do_step(c, x, v, num, lo, hi, iters, user_boundary(c), (hipStream_t)stream);
+    return plain_step(c, x, v, num, lo, hi, iters, false, (hipStream_t)stream);
 }
 
 pbfx_status pbfx_step(pbfx_ctx* c, float* x, float* v, uint32_t* num_inout, const float lo[3], const float hi[3],
                       int iters, void* stream) {
     if (!c || !num_inout) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    pbfx_status st = do_step(c, x, v, *num_inout, lo, hi, iters, user_boundary(c), (hipStream_t)stream);
+    pbfx_status st = plain_step(c, x, v, *num_inout, lo, hi, iters, false, (hipStream_t)stream);
     if (st != PBFX_OK) return st;
     return sync_step_result(c, num_inout, (hipStream_t)stream);
 }
@@ -451,12 +655,53 @@ pbfx_status pbfx_step_no_boundary(pbfx_ctx* c, float* x, float* v, uint32_t* num
                                   const float hi[3], int iters, void* stream) {
     if (!c || !num_inout) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    Boundary B{};
-    B.inner = c->d_default_sphere;
-    B.n_inner = 1;
-    pbfx_status st = do_step(c, x, v, *num_inout, lo, hi, iters, B, (hipStream_t)stream);
+    pbfx_status st = plain_step(c, x, v, *num_inout, lo, hi, iters, true, (hipStream_t)stream);
     if (st != PBFX_OK) return st;
     return sync_step_result(c, num_inout, (hipStream_t)stream);
+}
+
+pbfx_status pbfx_batch_step_async(pbfx_ctx* c, float* x, float* v, const uint32_t* in_offsets, uint32_t total,
+                                  int iters, void* stream) {
+    if (!c || !c->batch || !in_offsets) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    return do_step(c, x, v, total, in_offsets, iters, false, (hipStream_t)stream);
+}
+
+pbfx_status pbfx_batch_offsets(pbfx_ctx* c, uint32_t* out, void* stream) {
+    if (!c || !out) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    hipError_t e = launch_scene_offsets(c->b, out, (hipStream_t)stream);
+    return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
+}
+
+pbfx_status pbfx_batch_step(pbfx_ctx* c, float* x, float* v, uint32_t* counts, int iters, void* stream) {
+    if (!c || !c->batch || !counts) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    const hipStream_t s = (hipStream_t)stream;
+    const uint32_t n = (uint32_t)c->sc.size();
+    uint64_t tot = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (counts[i] > c->sc[i].capacity) return PBFX_CAPACITY;
+        c->h_in_off[i] = (uint32_t)tot;
+        tot += counts[i];
+    }
+    c->h_in_off[n] = (uint32_t)tot;
+    // the copy must complete before h_in_off is written again
+    hipError_t e = hipMemcpyAsync(c->d_in_off, c->h_in_off, (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(c, e);
+    pbfx_status st = do_step(c, x, v, (uint32_t)tot, c->d_in_off, iters, false, s);
+    if (st != PBFX_OK) {
+        (void)hipStreamSynchronize(s);
+        return st;
+    }
+    e = launch_scene_offsets(c->b, c->d_scene_off, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->h_scene_off, c->d_scene_off, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(c, e);
+    st = read_counters(c, s);  // synchronises the stream
+    if (st != PBFX_OK) return st;
+    for (uint32_t i = 0; i < n; ++i) counts[i] = c->h_scene_off[i + 1] - c->h_scene_off[i];
+    return step_status(c);
 }
 
 pbfx_status pbfx_get_num(pbfx_ctx* c, void* stream, uint32_t* num) {
@@ -493,12 +738,13 @@ pbfx_status pbfx_get_stats(pbfx_ctx* c, void* stream, pbfx_stats* out) {
 
 pbfx_status pbfx_ns_view_get(pbfx_ctx* c, pbfx_ns_view* out) {
     if (!c || !out) return PBFX_INVALID_ARG;
-    out->grid[0] = c->G.gx;
-    out->grid[1] = c->G.gy;
-    out->grid[2] = c->G.gz;
-    out->num_cells = c->G.C;
-    out->cell_width = c->G.cw;
-    out->max_pair = (uint32_t)c->params.max_pair;
+    const Grid& G = c->sc[0].G;  // a batch: the first scene; offsets cover every scene's keys
+    out->grid[0] = G.gx;
+    out->grid[1] = G.gy;
+    out->grid[2] = G.gz;
+    out->num_cells = c->batch ? c->b.CT : G.C;
+    out->cell_width = G.cw;
+    out->max_pair = (uint32_t)c->sc[0].params.max_pair;
     out->sorted_index = c->b.sidx;
     out->cell_offset = c->b.offset;
     out->hash = c->b.key;
@@ -508,29 +754,30 @@ pbfx_status pbfx_ns_view_get(pbfx_ctx* c, pbfx_ns_view* out) {
 pbfx_status pbfx_ns_neighbors(pbfx_ctx* c, uint32_t* out, uint32_t* counts, void* stream) {
     if (!c || !out || !counts) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    hipError_t e = launch_neighbors_view(c->G, make_phys(c->params), c->b, c->T, out, counts,
-                                         (uint32_t)c->params.max_pair, (hipStream_t)stream);
+    hipError_t e = launch_neighbors_view(c->b, out, counts, (uint32_t)c->sc[0].params.max_pair,
+                                         (hipStream_t)stream);
     return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
 }
 
 pbfx_status pbfx_ns_cell_ranges(pbfx_ctx* c, uint32_t* start, uint32_t* end, void* stream) {
     if (!c || !start || !end) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    hipError_t e = launch_cell_ranges(c->G, c->b, start, end, (hipStream_t)stream);
+    hipError_t e = launch_cell_ranges(c->b, c->batch ? c->b.CT : c->sc[0].G.C, start, end, (hipStream_t)stream);
     return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
 }
 
 pbfx_status pbfx_scaling_factor(pbfx_ctx* c, float* out, void* stream) {
     if (!c || !out) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    hipError_t e = launch_copy_lambda(c->G, c->b, out, (hipStream_t)stream);
+    hipError_t e = launch_copy_lambda(c->b, out, (hipStream_t)stream);
     return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
 }
 
 pbfx_status pbfx_density(pbfx_ctx* c, const float* x, float* rho, void* stream) {
     if (!c || !x || !rho) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    hipError_t e = launch_density(c->G, make_phys(c->params), c->b, x, rho, (hipStream_t)stream);
+    if (c->last_table.empty()) return PBFX_INVALID_ARG;  // no step yet
+    hipError_t e = launch_density(c->b, c->last_table[0], x, rho, (hipStream_t)stream);
     return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
 }
 
@@ -538,7 +785,7 @@ pbfx_status pbfx_restore_order(pbfx_ctx* c, const float* x_sorted, const float* 
                                float* v_out, void* stream) {
     if (!c || !x_sorted || !v_sorted || !x_out || !v_out) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
-    hipError_t e = launch_restore(c->G, c->b, x_sorted, v_sorted, x_out, v_out, (hipStream_t)stream);
+    hipError_t e = launch_restore(c->b, x_sorted, v_sorted, x_out, v_out, (hipStream_t)stream);
     return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
 }
 
@@ -558,9 +805,9 @@ void pbfx_set_report_info(pbfx_ctx* c, int enable) {
 
 pbfx_status pbfx_get_grid(pbfx_ctx* c, uint32_t grid[3]) {
     if (!c || !grid) return PBFX_INVALID_ARG;
-    grid[0] = c->G.gx;
-    grid[1] = c->G.gy;
-    grid[2] = c->G.gz;
+    grid[0] = c->sc[0].G.gx;
+    grid[1] = c->sc[0].G.gy;
+    grid[2] = c->sc[0].G.gz;
     return PBFX_OK;
 }
 

@@ -54,11 +54,14 @@ enum Ctr : int {
 constexpr uint32_t kStatRep = 64;                           // copies of the statistics counters
 constexpr uint32_t kCtrWords = CTR_NUM * (1 + kStatRep);    // ctr[0..CTR_NUM): allocation counters
 
+// One scene's hash grid.  Scenes occupy consecutive ranges of one cell-key
+// space: cell (x, y, z) of a scene has key base + (z*gy + y)*gx + x.
 struct Grid {
     uint32_t gx, gy, gz;
     uint32_t lgx, lgy;
     uint32_t C;
     float cw;
+    uint32_t base;
 };
 
 // Constants of one step, all computed on the host in IEEE float exactly as
@@ -78,6 +81,7 @@ struct Phys {
     int32_t max_sweeps;
     int32_t outer_as_inner;
     float cut2_w, cut2_wd;  // squared support bounds of W (arg inv_h) and of W' (arg warg): pairs beyond add 0
+    int32_t xsph_on, vort_on;
 };
 
 struct Boundary {
@@ -90,8 +94,24 @@ struct Boundary {
     uint32_t n_planes;
 };
 
+// Everything a kernel needs to know about one scene of the batch.
+struct Scene {
+    Grid G;
+    Phys P;
+    Boundary B;
+    float lo[3], hi[3];   // simulation area: predicted positions outside are deleted
+};
+
 struct Bufs {
     uint32_t cap;
+    // scenes: table, count, total cell keys (padded; key CT = deleted)
+    const Scene* scenes;
+    uint32_t nscenes;
+    uint32_t CT;
+    const uint32_t* in_off;   // [nscenes+1] first input index of each scene in this step
+    const uint32_t* kbase;    // [nscenes+1] first cell key of each scene, then CT
+    uint32_t* isid;           // scene of each input index
+    uint32_t* sid;            // scene of each sorted slot
     uint32_t* key;
     uint32_t* rank;
     uint32_t* tmp;
@@ -161,18 +181,21 @@ struct Tuning {
 };
 
 // Launchers (pbfx_kernels.hip).  All enqueue on `s` and never synchronise.
-hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bufs& b, const Tuning& T,
-                       float* x, float* v, uint32_t n_in, const float lo[3], const float hi[3], int iters,
-                       int xsph_on, int vort_on, hipStream_t s, Timer* timer);
+// n_in: particles entering (all scenes); uw: cells per work unit along x,
+// a power of two no wider than any scene's grid; any_xsph / any_vort: some
+// scene runs XSPH / vorticity.
+// s0: the scene (a single-scene context passes it by value to the kernels).
+hipError_t launch_step(const Bufs& b, const Scene& s0, const Tuning& T, float* x, float* v, uint32_t n_in, int iters,
+                       int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* timer);
+// out[s] = first output slot of scene s, out[nscenes] = live total (device).
+hipError_t launch_scene_offsets(const Bufs& b, uint32_t* out, hipStream_t s);
 const char* kernel_name(int id);
-hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, const Tuning& T, uint32_t* out,
-                                 uint32_t* counts, uint32_t max_pair, hipStream_t s);
-hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uint32_t* end, hipStream_t s);
-hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s);
+hipError_t launch_neighbors_view(const Bufs& b, uint32_t* out, uint32_t* counts, uint32_t max_pair, hipStream_t s);
+hipError_t launch_cell_ranges(const Bufs& b, uint32_t C, uint32_t* start, uint32_t* end, hipStream_t s);
+hipError_t launch_copy_lambda(const Bufs& b, float* out, hipStream_t s);
 hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t n, hipStream_t s);
-hipError_t launch_density(const Grid& G, const Phys& P, const Bufs& b, const float* x, float* rho, hipStream_t s);
-hipError_t launch_restore(const Grid& G, const Bufs& b, const float* xs, const float* vs, float* xo, float* vo,
-                          hipStream_t s);
+hipError_t launch_density(const Bufs& b, const Scene& s0, const float* x, float* rho, hipStream_t s);
+hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float* xo, float* vo, hipStream_t s);
 // Fills T.grid[] (and T.grid_units) for num_cu compute units.
 void size_unit_grids(Tuning& T, int num_cu);
 

@@ -109,6 +109,57 @@ __device__ __forceinline__ uint32_t grid_hash(int gx, int gy, int gz, const Grid
     return z * G.gx * G.gy + y * G.gx + x;
 }
 
+// cell key of a scene's grid position (the scene's range of the key space)
+__device__ __forceinline__ uint32_t scene_key(int gx, int gy, int gz, const Grid& G) {
+    return G.base + grid_hash(gx, gy, gz, G);
+}
+
+// Largest s < n with off[s] <= i (the scene owning index or key i).
+__device__ __forceinline__ uint32_t find_scene(const uint32_t* __restrict__ off, uint32_t n, uint32_t i) {
+    uint32_t lo = 0, hi = n - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// The scenes of a launch: a single scene travels as a kernel argument;
+// several are read from the device table.
+template <bool kB>
+struct Scenes {
+    const Scene* table;
+    Scene s0;
+    __device__ __forceinline__ const Scene& operator[](uint32_t s) const { return kB ? table[s] : s0; }
+};
+
+template <bool kB>
+__device__ __forceinline__ uint32_t scene_index(const uint32_t* __restrict__ off, uint32_t n, uint32_t i) {
+    return kB ? find_scene(off, n, i) : 0u;
+}
+
+// Runs f(s) for the active lanes, one scene at a time: s is uniform over the
+// lanes that call f (read from the first pending lane), so the scene's
+// constants come through scalar loads.  f must not use wave-wide operations.
+template <bool kB, class F>
+__device__ __forceinline__ void by_scene(uint32_t sid, bool active, F&& f) {
+    if (!kB) {
+        if (active) f(0u);
+        return;
+    }
+    bool todo = active;
+    while (__any(todo)) {
+        if (todo) {
+            const uint32_t s = __builtin_amdgcn_readfirstlane(sid);
+            if (sid == s) {
+                f(s);
+                todo = false;
+            }
+        }
+    }
+}
+
 // t^n by binary exponentiation; n = 4 evaluates (t*t)*(t*t)
 __device__ __forceinline__ float powi(float t, int n) {
     float result = 1.f, base = t;
@@ -333,36 +384,42 @@ __device__ __forceinline__ void mark_tile(uint32_t* __restrict__ tflag, uint32_t
 // ------------------------------------------------- K1: predict + hash + histogram
 // pbf_predict.cu:15-45 (v* = fmaf(g,dt,v), x* = fmaf(v*,dt,x)), calcHashCUDA
 // (pbf_neighbor_search_util.cu:20-37), checkDeleteParticleCUDA
-// (pbf_delete.cu:9-28, sentinel key C) and the histogram of insertAndCountCUDA
+// (pbf_delete.cu:9-28, sentinel key CT) and the histogram of insertAndCountCUDA
 // (pbf_counting_sort.cu:21-32) with wave-aggregated atomics; each particle
 // marks its 4096-cell scan tile as occupied (a plain store, no contention).
+// Input i belongs to the scene whose input range holds it.
+template <bool kB>
 __global__ __launch_bounds__(256) void k_predict_hash(const float* __restrict__ x, const float* __restrict__ v,
-                                                      uint32_t n, Grid G, float dt, float g0, float g1, float g2,
-                                                      float lo0, float lo1, float lo2, float hi0, float hi1,
-                                                      float hi2, uint32_t* __restrict__ key,
+                                                      uint32_t n, Bufs b, Scenes<kB> SC, uint32_t* __restrict__ key,
                                                       uint32_t* __restrict__ rank, uint32_t* __restrict__ count,
                                                       uint32_t* __restrict__ tflag, uint32_t lgT) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = i < n;
     uint32_t k = 0;
-    if (active) {
+    const uint32_t si = active ? scene_index<kB>(b.in_off, b.nscenes, i) : 0u;
+    by_scene<kB>(si, active, [&](uint32_t sc) {
+        const Scene& S = SC[sc];
+        const float dt = S.P.dt;
         float q[3];
-        const float gg[3] = {g0, g1, g2};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            float vs = __builtin_fmaf(gg[c], dt, v[3 * i + c]);
+            float vs = __builtin_fmaf(S.P.g[c], dt, v[3 * i + c]);
             q[c] = __builtin_fmaf(vs, dt, x[3 * i + c]);
         }
+        const Grid G = S.G;
         int gp[3];
         grid_pos(mk3(q[0], q[1], q[2]), G.cw, gp);
-        k = grid_hash(gp[0], gp[1], gp[2], G);
-        if (q[0] < lo0 || hi0 < q[0] || q[1] < lo1 || hi1 < q[1] || q[2] < lo2 || hi2 < q[2]) k = G.C;
-    }
+        k = scene_key(gp[0], gp[1], gp[2], G);
+        if (q[0] < S.lo[0] || S.hi[0] < q[0] || q[1] < S.lo[1] || S.hi[1] < q[1] || q[2] < S.lo[2] ||
+            S.hi[2] < q[2])
+            k = b.CT;
+    });
     const uint32_t r = wave_rank(count, k, active);
     if (active) {
         mark_tile(tflag, k >> lgT);
         key[i] = k;
         rank[i] = r;
+        if (kB) b.isid[i] = si;
     }
 }
 
@@ -510,33 +567,35 @@ __device__ __forceinline__ void emit_units(const uint32_t* __restrict__ offset, 
 // and emits the work units: each run of up to `uw` cells along x is split into
 // narrower units where it is dense, so that a unit's 27-cell neighbourhood
 // fits the on-chip tile.
-__global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, const float* __restrict__ v, Grid G,
-                                                 float dt, float g0, float g1, float g2,
+template <bool kB>
+__global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, const float* __restrict__ v, Bufs b, Scenes<kB> SC,
                                                  const uint32_t* __restrict__ key, const uint32_t* __restrict__ tmp,
-                                                 const uint32_t* __restrict__ offset, uint32_t* __restrict__ sidx,
-                                                 float4* __restrict__ Pf, float* __restrict__ X, uint32_t cap,
-                                                 uint32_t n, uint32_t uw, uint32_t budget, uint2* __restrict__ units,
-                                                 uint32_t* __restrict__ ctr) {
+                                                 uint32_t n, uint32_t uw, uint32_t budget) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t i = tmp[s];
-    const uint32_t k = key[i];
-    const uint32_t o0 = offset[k], o1 = offset[k + 1];
-    uint32_t r = 0;
-    for (uint32_t t = o0; t < o1; ++t) r += (tmp[t] < i) ? 1u : 0u;
-    const uint32_t f = o0 + r;
-    sidx[f] = i;
-    const float gg[3] = {g0, g1, g2};
-    float q[3], xo[3];
+    const bool active = s < n;
+    const uint32_t i = active ? tmp[s] : 0u;
+    const uint32_t si = (kB && active) ? b.isid[i] : 0u;
+    by_scene<kB>(si, active, [&](uint32_t sc) {
+        const uint32_t k = key[i];
+        const uint32_t o0 = b.offset[k], o1 = b.offset[k + 1];
+        uint32_t r = 0;
+        for (uint32_t t = o0; t < o1; ++t) r += (tmp[t] < i) ? 1u : 0u;
+        const uint32_t f = o0 + r;
+        b.sidx[f] = i;
+        if (kB) b.sid[f] = sc;
+        const Phys& P = SC[sc].P;
+        const float dt = P.dt;
+        float q[3], xo[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        xo[c] = x[3 * i + c];
-        float vs = __builtin_fmaf(gg[c], dt, v[3 * i + c]);
-        q[c] = __builtin_fmaf(vs, dt, xo[c]);
-    }
-    Pf[f] = make_float4(q[0], q[1], q[2], 0.f);
-    st3(X, cap, f, mk3(xo[0], xo[1], xo[2]));
-    if (k < G.C) emit_units(offset, k, f, uw, budget, 0u, units, &ctr[CTR_UNITS]);
+        for (int c = 0; c < 3; ++c) {
+            xo[c] = x[3 * i + c];
+            float vs = __builtin_fmaf(P.g[c], dt, v[3 * i + c]);
+            q[c] = __builtin_fmaf(vs, dt, xo[c]);
+        }
+        b.Pf[f] = make_float4(q[0], q[1], q[2], 0.f);
+        st3(b.X, b.cap, f, mk3(xo[0], xo[1], xo[2]));
+        if (k < b.CT) emit_units(b.offset, k, f, uw, budget, 0u, b.units, &b.ctr[CTR_UNITS]);
+    });
 }
 
 
@@ -622,7 +681,7 @@ __device__ UnitInfo unit_setup(const Grid& G, const uint32_t* __restrict__ offse
     I.W = unit.y & 31u;
     const uint32_t chunk = unit.y >> 5;
     I.ncell = 9u * (I.W + 2u);
-    const uint32_t kb = unit.x;
+    const uint32_t kb = unit.x - G.base;  // key within the scene's grid
     I.xb = kb & (G.gx - 1u);
     I.y = (kb >> G.lgx) & (G.gy - 1u);
     I.z = kb >> (G.lgx + G.lgy);
@@ -633,7 +692,7 @@ __device__ UnitInfo unit_setup(const Grid& G, const uint32_t* __restrict__ offse
         const uint32_t cx = (I.xb + kk - 1u) & (G.gx - 1u);
         const uint32_t cy = (I.y + (r % 3u) - 1u) & (G.gy - 1u);
         const uint32_t cz = (I.z + (r / 3u) - 1u) & (G.gz - 1u);
-        const uint32_t key = (cz * G.gy + cy) * G.gx + cx;
+        const uint32_t key = G.base + (cz * G.gy + cy) * G.gx + cx;
         const uint32_t gs = offset[key];
         cnt = offset[key + 1] - gs;
         S.gstart[e] = gs;
@@ -710,20 +769,25 @@ __device__ __forceinline__ uint32_t xcd_block() {
 // sorted slots, column-major: nlist[k * cap_particles + slot], with the
 // count in ncnt[slot].  Candidates come from the unit's staged tile, four
 // at a time.
-__global__ __launch_bounds__(kBlock) void k_ns(Grid G, Phys P, Bufs b, uint32_t tile_cap) {
+template <bool kB>
+__global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t tile_cap) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tile = smem;
     const float4* __restrict__ Pf = b.Pf;
     uint32_t* __restrict__ nl = b.nlist;
     const uint32_t cap = b.cap;
-    const uint32_t ncap = (uint32_t)P.cap;
     const uint32_t nU = b.ctr[CTR_UNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
     Tally tl;
     for (uint32_t u = u0; u < u1; u += us) {
-        const UnitInfo I = unit_setup(G, b.offset, b.units[u], tile_cap, S, tl);
+        const uint2 unit = b.units[u];
+        const uint32_t sc = scene_index<kB>(b.kbase, b.nscenes, unit.x);
+        const Grid G = SC[sc].G;
+        const float h2 = SC[sc].P.h2;
+        const uint32_t ncap = (uint32_t)SC[sc].P.cap;
+        const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
             if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tile[t] = Pf[g]; });
             __syncthreads();
@@ -748,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_ns(Grid G, Phys P, Bufs b, uint32_t 
                         for (uint32_t i = 0; i < 4u; ++i) {
                             const F3 d = self - pj[i];
                             const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
-                            if (!trunc && t + i < hi && r2 < P.h2) {
+                            if (!trunc && t + i < hi && r2 < h2) {
                                 if (cnt < ncap) nl[(size_t)cnt++ * cap + g] = gb + t + i;
                                 else trunc = true;
                             }
@@ -772,35 +836,40 @@ __global__ __launch_bounds__(kBlock) void k_ns(Grid G, Phys P, Bufs b, uint32_t 
 // calcScalingFactorCUDA, memo form (pbf_constraint.cu:168-209): over the
 // frozen list, s = C / (m^2 inv_rho0^2 G + eps) with the reference's fma
 // order; writes (x*, s).  One thread per slot.
-__global__ __launch_bounds__(256) void k_lambda(Grid G, Phys P, Bufs b) {
+template <bool kB>
+__global__ __launch_bounds__(256) void k_lambda(Bufs b, Scenes<kB> SC) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= b.offset[G.C]) return;
-    const float4* __restrict__ Pf = b.Pf;
-    const uint32_t* __restrict__ nl = b.nlist;
-    const uint32_t cap = b.cap;
-    const F3 self = f3of(Pf[i]);
-    const uint32_t n = b.ncnt[i];
-    float ks = 0.f, kg2s = 0.f;
-    F3 kgs = mk3(0.f, 0.f, 0.f);
-    for (uint32_t k = 0; k < n; k += 4u) {
-        F3 pj[4];
+    const bool active = i < b.offset[b.CT];
+    by_scene<kB>((kB && active) ? b.sid[i] : 0u, active, [&](uint32_t sc) {
+        const Phys P = SC[sc].P;
+        const float4* __restrict__ Pf = b.Pf;
+        const uint32_t* __restrict__ nl = b.nlist;
+        const uint32_t cap = b.cap;
+        const F3 self = f3of(Pf[i]);
+        const uint32_t n = b.ncnt[i];
+        float ks = 0.f, kg2s = 0.f;
+        F3 kgs = mk3(0.f, 0.f, 0.f);
+        for (uint32_t k = 0; k < n; k += 4u) {
+            F3 pj[4];
 #pragma unroll
-        for (uint32_t m = 0; m < 4u; ++m) pj[m] = f3of(Pf[nl[(size_t)min(k + m, n - 1u) * cap + i]]);
+            for (uint32_t m = 0; m < 4u; ++m) pj[m] = f3of(Pf[nl[(size_t)min(k + m, n - 1u) * cap + i]]);
 #pragma unroll
-        for (uint32_t m = 0; m < 4u; ++m)
-            if (k + m < n) {
-                float W;
-                F3 kg;
-                pair_terms(self, pj[m], P, W, kg);
-                const float kg2 = fdot(kg, kg);
-                ks += W;
-                acc(kgs, kg);
-                kg2s += kg2;
-            }
-    }
-    const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
-    const float Gs = __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
-    b.Q[i] = make_float4(self.x, self.y, self.z, Cc / __builtin_fmaf(P.m2r2, Gs, P.relax));
+            for (uint32_t m = 0; m < 4u; ++m)
+                if (k + m < n) {
+                    float W;
+                    F3 kg;
+                    pair_terms(self, pj[m], P, W, kg);
+                    const float kg2 = fdot(kg, kg);
+                    ks += W;
+                    acc(kgs, kg);
+                    kg2s += kg2;
+                }
+        }
+        const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
+        const float Gs =
+            __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
+        b.Q[i] = make_float4(self.x, self.y, self.z, Cc / __builtin_fmaf(P.m2r2, Gs, P.relax));
+    });
 }
 
 // ------------------------------------------------- K8: delta-p + collision + apply
@@ -808,10 +877,14 @@ __global__ __launch_bounds__(256) void k_lambda(Grid G, Phys P, Bufs b) {
 // responseCollisionCUDA (pbf_response_collision.cu:12-66) and
 // updateInterimPositionCUDA (pbf_constraint.cu:335-345).  Reads (x*, s),
 // writes the new x*.  One thread per slot.
-__global__ __launch_bounds__(256) void k_deltap(Grid G, Phys P, Boundary B, Bufs b) {
+template <bool kB>
+__global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const bool active = i < b.offset[b.CT];
     Tally tl;
-    if (i < b.offset[G.C]) {
+    by_scene<kB>((kB && active) ? b.sid[i] : 0u, active, [&](uint32_t sc) {
+        const Phys P = SC[sc].P;
+        const Boundary B = SC[sc].B;
         const float4* __restrict__ Q = b.Q;
         const uint32_t* __restrict__ nl = b.nlist;
         const uint32_t cap = b.cap;
@@ -856,7 +929,7 @@ __global__ __launch_bounds__(256) void k_deltap(Grid G, Phys P, Boundary B, Bufs
         b.Pf[i] = make_float4(np.x, np.y, np.z, 0.f);
         tl.sweepcap = capped ? 1u : 0u;
         tl.maxsweep = (uint32_t)sw;
-    }
+    });
     flush_tally(b.ctr, tl);
 }
 
@@ -873,36 +946,33 @@ __global__ __launch_bounds__(256) void k_deltap(Grid G, Phys P, Boundary B, Bufs
 // pass packs each slot's record R = (x*, 0), (v, 0) with
 // v = inv_dt*(x* - x_sorted) (updateVelocityCUDA, pbf_update.cu:25-39), so
 // that a particle visited in run order costs one cache line.
-__global__ __launch_bounds__(256) void k_cur_hash(const float4* __restrict__ Pf, const float* __restrict__ X,
-                                                  float inv_dt, uint32_t cap, const uint32_t* __restrict__ offset,
-                                                  Grid G, uint32_t lguw, uint32_t lgnc, uint32_t* __restrict__ ckey,
-                                                  uint32_t* __restrict__ crank, uint32_t* __restrict__ ccount,
-                                                  uint32_t* __restrict__ ctflag, uint32_t lgT,
-                                                  float4* __restrict__ R) {
+template <bool kB>
+__global__ __launch_bounds__(256) void k_cur_hash(Bufs b, Scenes<kB> SC, uint32_t lguw, uint32_t lgnc, uint32_t lgT) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nl = offset[G.C];
-    const bool active = s < nl;
+    const bool active = s < b.offset[b.CT];
     uint32_t k = 0;
-    if (active) {
-        const F3 p = f3of(Pf[s]);
-        const F3 vv = mulsv(inv_dt, p - ld3(X, cap, s));
-        R[2 * s] = make_float4(p.x, p.y, p.z, 0.f);
-        R[2 * s + 1] = make_float4(vv.x, vv.y, vv.z, 0.f);
+    by_scene<kB>((kB && active) ? b.sid[s] : 0u, active, [&](uint32_t sc) {
+        const Scene& S = SC[sc];
+        const F3 p = f3of(b.Pf[s]);
+        const F3 vv = mulsv(S.P.inv_dt, p - ld3(b.X, b.cap, s));
+        b.R[2 * s] = make_float4(p.x, p.y, p.z, 0.f);
+        b.R[2 * s + 1] = make_float4(vv.x, vv.y, vv.z, 0.f);
+        const Grid G = S.G;
         int gp[3];
         grid_pos(p, G.cw, gp);
-        k = ((grid_hash(gp[0], gp[1], gp[2], G) >> lguw) << lgnc) | (blockIdx.x & ((1u << lgnc) - 1u));
-    }
-    const uint32_t r = wave_rank(ccount, k, active);
+        k = ((scene_key(gp[0], gp[1], gp[2], G) >> lguw) << lgnc) | (blockIdx.x & ((1u << lgnc) - 1u));
+    });
+    const uint32_t r = wave_rank(b.ccount, k, active);
     if (active) {
-        mark_tile(ctflag, k >> lgT);
-        ckey[s] = k;
-        crank[s] = r;
+        mark_tile(b.ctflag, k >> lgT);
+        b.ckey[s] = k;
+        b.crank[s] = r;
     }
 }
 
 // Places each particle at its run's position and emits the work items of
 // the runs holding at least kSparse particles (one per kChunk of them).
-__global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict__ offset, Grid G,
+__global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict__ offset, uint32_t CT,
                                                      const uint32_t* __restrict__ ckey,
                                                      const uint32_t* __restrict__ crank,
                                                      const uint32_t* __restrict__ coffset,
@@ -910,7 +980,7 @@ __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict_
                                                      uint32_t lgnc, uint2* __restrict__ cunits,
                                                      uint32_t* __restrict__ ctr) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= offset[G.C]) return;
+    if (s >= offset[CT]) return;
     const uint32_t k = ckey[s];
     const uint32_t slot = coffset[k] + crank[s];
     cperm[slot] = s;
@@ -974,7 +1044,7 @@ __device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __res
     auto row = [&](int r, uint32_t* st, uint32_t* en) {
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
-            const uint32_t c = grid_hash(gp[0] + dx - 1, gp[1] + (r % 3) - 1, gp[2] + (r / 3) - 1, G);
+            const uint32_t c = scene_key(gp[0] + dx - 1, gp[1] + (r % 3) - 1, gp[2] + (r / 3) - 1, G);
             st[dx] = offset[c];
             en[dx] = offset[c + 1];
         }
@@ -1021,7 +1091,7 @@ __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __rest
         uint32_t st[9], en[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-            const uint32_t c = grid_hash(gp[0] + (q % 3) - 1, gp[1] + (q / 3) - 1, gp[2] + dz, G);
+            const uint32_t c = scene_key(gp[0] + (q % 3) - 1, gp[1] + (q / 3) - 1, gp[2] + dz, G);
             st[q] = offset[c];
             en[q] = offset[c + 1];
         }
@@ -1046,8 +1116,8 @@ __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __rest
 // a contiguous share of that order per XCD.  Sparse runs are contiguous
 // stretches of the order.  body(q, g): position q in that order, slot g.
 template <class F>
-__device__ __forceinline__ void for_sparse(const Grid& G, const Bufs& b, uint32_t lgnc, F&& body) {
-    const uint32_t nl = b.offset[G.C];
+__device__ __forceinline__ void for_sparse(const Bufs& b, uint32_t lgnc, F&& body) {
+    const uint32_t nl = b.offset[b.CT];
     uint32_t c0, c1, cs;
     unit_range((nl + kBlock - 1) / kBlock, c0, c1, cs);
     for (uint32_t c = c0; c < c1; c += cs) {
@@ -1088,18 +1158,17 @@ __device__ __forceinline__ float4 pos_slot(const float4* __restrict__ R, uint32_
 // For each particle (by position q of the final-cell order) the slots of its
 // in-range pairs, column-major: flist[k * cap + q], k < kFinalK; fcnt[q] =
 // the full count (above kFinalK: the passes fall back to the direct sum).
-__global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, uint32_t tile_cap, uint32_t lguw,
-                                                       uint32_t lgnc) {
+template <bool kB>
+__global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, uint32_t tile_cap, uint32_t lguw, uint32_t lgnc) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
     const float4* __restrict__ R = b.R;
     const uint32_t cap = b.cap;
-    const float cut2 = fmaxf(P.cut2_w, P.cut2_wd);
     uint32_t* __restrict__ fl = b.flist;
     Tally tl;
     UnitInfo I;
-    auto build = [&](uint32_t q, uint32_t g, bool tiled) {
+    auto build = [&](uint32_t q, uint32_t g, bool tiled, const Grid& G, float cut2) {
         const F3 self = f3of(R[2 * g]);
         uint32_t n = 0;
         auto emit = [&](uint32_t j, bool first) {
@@ -1111,11 +1180,15 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, u
         b.fcnt[q] = n;
         tl.listovf += n > kFinalK ? 1u : 0u;
     };
+    auto cut2_of = [&](uint32_t sc) { return fmaxf(SC[sc].P.cut2_w, SC[sc].P.cut2_wd); };
     const uint32_t nU = b.ctr[CTR_CUNITS];
     uint32_t u0, u1, us;
     unit_range(nU, u0, u1, us);
     for (uint32_t u = u0; u < u1; u += us) {
         const uint2 unit = b.cunits[u];
+        const uint32_t sc = scene_index<kB>(b.kbase, b.nscenes, unit.x);
+        const Grid G = SC[sc].G;
+        const float cut2 = cut2_of(sc);
         I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
             if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_slot(R, j); });
@@ -1123,7 +1196,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, u
             const OwnCur o = own_cur(b, unit, lguw, lgnc);
             for (uint32_t i = threadIdx.x; i < o.n; i += kBlock) {
                 if (!I.staged) tl.slowpart++;
-                build(o.c0 + i, b.cperm[o.c0 + i], I.staged);
+                build(o.c0 + i, b.cperm[o.c0 + i], I.staged, G, cut2);
             }
         } else {
             // no tile index range: the passes sum these particles directly
@@ -1132,9 +1205,9 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Grid G, Phys P, Bufs b, u
         }
         __syncthreads();
     }
-    for_sparse(G, b, lgnc, [&](uint32_t q, uint32_t g) {
+    for_sparse(b, lgnc, [&](uint32_t q, uint32_t g) {
         tl.slowpart++;
-        build(q, g, false);
+        by_scene<kB>(kB ? b.sid[g] : 0u, true, [&](uint32_t sc) { build(q, g, false, SC[sc].G, cut2_of(sc)); });
     });
     flush_tally(b.ctr, tl);
 }
@@ -1166,12 +1239,14 @@ __device__ __forceinline__ F3 sum_list(const uint32_t* __restrict__ fl, uint32_t
     return sum;
 }
 
-// Visits position q of the final-cell order with its slot and stored pair count.
-template <class F>
-__device__ __forceinline__ void for_final(const Grid& G, const Bufs& b, F&& body) {
+// Visits position q of the final-cell order: body(q, slot, stored pair
+// count, scene), grouped by scene (no wave-wide operations inside).
+template <bool kB, class F>
+__device__ __forceinline__ void for_final(const Bufs& b, F&& body) {
     const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
-    if (q >= b.offset[G.C]) return;
-    body(q, b.cperm[q], b.fcnt[q]);
+    const bool active = q < b.offset[b.CT];
+    const uint32_t g = active ? b.cperm[q] : 0u;
+    by_scene<kB>((kB && active) ? b.sid[g] : 0u, active, [&](uint32_t sc) { body(q, g, b.fcnt[q], sc); });
 }
 
 // vorticity pair term (pbf_vorticity_confinement.cu:15-31)
@@ -1205,10 +1280,17 @@ __device__ __forceinline__ F3 xsph_pair(F3 self, F3 selfv, F3 pj, F3 vj, float i
 // calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
 // spiky derivative (or h, when reproducing the reference's argument);
 // writes (omega, |omega|).  Elements: (x*_j, slot j).
-__global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b) {
+template <bool kB>
+__global__ __launch_bounds__(kBlock) void k_vort(Bufs b, Scenes<kB> SC) {
     const float4* __restrict__ R = b.R;
-    const float mr = P.m * P.inv_rho0;
-    for_final(G, b, [&](uint32_t q, uint32_t g, uint32_t n) {
+    for_final<kB>(b, [&](uint32_t q, uint32_t g, uint32_t n, uint32_t sc) {
+        const Phys P = SC[sc].P;
+        if (!P.vort_on) {
+            b.OM4[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+            return;
+        }
+        const Grid G = SC[sc].G;
+        const float mr = P.m * P.inv_rho0;
         const F3 self = f3of(R[2 * g]);
         const F3 selfv = f3of(R[2 * g + 1]);
         auto elem = [&](uint32_t j) { return pos_slot(R, j); };
@@ -1225,10 +1307,14 @@ __global__ __launch_bounds__(kBlock) void k_vort(Grid G, Phys P, Bufs b) {
 // ------------------------------------------------- K11: vorticity confinement
 // addVorticityConfinementCUDA (pbf_vorticity_confinement.cu:192-222), in place
 // on the record's velocity.  Elements: (x*_j, |omega_j|).
-__global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b) {
+template <bool kB>
+__global__ __launch_bounds__(kBlock) void k_confine(Bufs b, Scenes<kB> SC) {
     float4* __restrict__ R = b.R;
     const float4* __restrict__ OM4 = b.OM4;
-    for_final(G, b, [&](uint32_t q, uint32_t g, uint32_t n) {
+    for_final<kB>(b, [&](uint32_t q, uint32_t g, uint32_t n, uint32_t sc) {
+        const Phys P = SC[sc].P;
+        if (!P.vort_on) return;
+        const Grid G = SC[sc].G;
         const F3 self = f3of(R[2 * g]);
         auto elem = [&](uint32_t j) {
             const float4 r = R[2 * j];
@@ -1251,13 +1337,24 @@ __global__ __launch_bounds__(kBlock) void k_confine(Grid G, Phys P, Bufs b) {
 // calcXSPHCUDA (pbf_xsph.cu:84-113) and updatePosition (pbf_update.cu:16-23):
 // x_out = x*, v_out = v + c*m*inv_rho0*sum unless the sum is NaN, with v the
 // record's velocity (after confinement when that ran).
-__global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, float* __restrict__ xo,
-                                                     float* __restrict__ vo) {
+template <bool kB>
+__global__ __launch_bounds__(kBlock) void k_xsph_out(Bufs b, Scenes<kB> SC, float* __restrict__ xo, float* __restrict__ vo) {
     const float4* __restrict__ R = b.R;
     Tally tl;
-    for_final(G, b, [&](uint32_t q, uint32_t g, uint32_t n) {
+    for_final<kB>(b, [&](uint32_t q, uint32_t g, uint32_t n, uint32_t sc) {
+        const Phys P = SC[sc].P;
         const F3 self = f3of(R[2 * g]);
         const F3 selfv = f3of(R[2 * g + 1]);
+        if (!P.xsph_on) {
+            xo[3 * g + 0] = self.x;
+            xo[3 * g + 1] = self.y;
+            xo[3 * g + 2] = self.z;
+            vo[3 * g + 0] = selfv.x;
+            vo[3 * g + 1] = selfv.y;
+            vo[3 * g + 2] = selfv.z;
+            return;
+        }
+        const Grid G = SC[sc].G;
         auto elem = [&](uint32_t j) { return pos_slot(R, j); };
         auto term = [&](float4 el) {
             return xsph_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.inv_h);
@@ -1284,11 +1381,11 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Grid G, Phys P, Bufs b, flo
 // ------------------------------------------------- K13: write-back without XSPH
 // pbf_solve.cpp:125-145 as shipped: v = inv_dt*(x* - x_sorted), x = x*; with
 // R (vorticity ran without XSPH) the record's position and confined velocity.
-__global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float4* __restrict__ Pf,
-                                             const float4* __restrict__ R, uint32_t C, float* __restrict__ xo,
+template <bool kB>
+__global__ __launch_bounds__(256) void k_out(Bufs b, Scenes<kB> SC, const float4* __restrict__ R, float* __restrict__ xo,
                                              float* __restrict__ vo) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nl = b.offset[C];
+    const uint32_t nl = b.offset[b.CT];
     if (s >= nl) return;
     const uint32_t cap = b.cap;
     F3 p, vv;
@@ -1296,8 +1393,8 @@ __global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float4* __res
         p = f3of(R[2 * s]);
         vv = f3of(R[2 * s + 1]);
     } else {
-        p = f3of(Pf[s]);
-        vv = mulsv(P.inv_dt, p - ld3(b.X, cap, s));
+        p = f3of(b.Pf[s]);
+        vv = mulsv(SC[kB ? b.sid[s] : 0u].P.inv_dt, p - ld3(b.X, cap, s));
     }
     xo[3 * s + 0] = p.x;
     xo[3 * s + 1] = p.y;
@@ -1308,10 +1405,10 @@ __global__ __launch_bounds__(256) void k_out(Phys P, Bufs b, const float4* __res
 }
 
 // ------------------------------------------------- views
-__global__ __launch_bounds__(256) void k_neighbors_view(Grid G, Bufs b, uint32_t* __restrict__ out,
+__global__ __launch_bounds__(256) void k_neighbors_view(Bufs b, uint32_t* __restrict__ out,
                                                         uint32_t* __restrict__ counts, uint32_t max_pair) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= b.offset[G.C]) return;
+    if (g >= b.offset[b.CT]) return;
     const uint32_t cnt = b.ncnt[g];
     counts[g] = cnt;
     for (uint32_t c = 0; c < max_pair; ++c)
@@ -1338,9 +1435,11 @@ __global__ void k_copy_lambda(const uint32_t* __restrict__ offset, uint32_t C, c
 // step's frozen lists: rho_i = sum_j m * W(|x_i - x_j|), W with inv_h (the
 // reference passes h as the kernel's inv_h argument, pbf_density.cu:22).
 // x: AoS float3 in the step's sorted order.
-__global__ void k_density(Grid G, Phys P, Bufs b, const float* __restrict__ x, float* __restrict__ rho) {
+template <bool kB>
+__global__ void k_density(Bufs b, Scenes<kB> SC, const float* __restrict__ x, float* __restrict__ rho) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.offset[G.C]) return;
+    if (i >= b.offset[b.CT]) return;
+    const Phys& P = SC[kB ? b.sid[i] : 0u].P;
     const F3 self = mk3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
     const uint32_t n = b.ncnt[i];
     float sum = 0.f;
@@ -1354,16 +1453,23 @@ __global__ void k_density(Grid G, Phys P, Bufs b, const float* __restrict__ x, f
 
 // restoreOrderCUDA (pbf/interaction/cuda/pbf_neighbor_search_util.cu:156-170):
 // out[sorted_index[s]] = sorted[s] for the live slots of the last step.
-__global__ void k_restore(Grid G, Bufs b, const float* __restrict__ xs, const float* __restrict__ vs,
+__global__ void k_restore(Bufs b, const float* __restrict__ xs, const float* __restrict__ vs,
                           float* __restrict__ xo, float* __restrict__ vo) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= b.offset[G.C]) return;
+    if (s >= b.offset[b.CT]) return;
     const uint32_t o = b.sidx[s];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         xo[3 * o + c] = xs[3 * s + c];
         vo[3 * o + c] = vs[3 * s + c];
     }
+}
+
+// First sorted slot of each scene after the step (the particles of scene s
+// are slots [out[s], out[s+1]) of the output), out[nscenes] = live total.
+__global__ void k_scene_offsets(Bufs b, uint32_t* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= b.nscenes) out[s] = b.offset[b.kbase[s]];
 }
 
 __global__ void k_selftest(const float* __restrict__ a, const float* __restrict__ bb, float* __restrict__ out,
@@ -1389,10 +1495,13 @@ void size_unit_grids(Tuning& T, int num_cu) {
     struct K {
         int id;
         const void* f;
-    } ks[] = {{K_NS, (const void*)k_ns}, {K_FINAL_LIST, (const void*)k_final_list}};
+    } ks[] = {{K_NS, (const void*)k_ns<true>}, {K_FINAL_LIST, (const void*)k_final_list<true>}};
     const uint32_t ncu = num_cu > 0 ? (uint32_t)num_cu : 1u;
     uint32_t lo = 0xFFFFFFFFu;
     for (int i = 0; i < K_NUM; ++i) T.grid[i] = 0;
+    const void* single[] = {(const void*)k_ns<false>, (const void*)k_final_list<false>};
+    for (const void* f : single)
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_of(kTileCapMax));
     for (auto& k : ks) {
         // allow the largest tile, then ask how many workgroups of the
         // current one fit a CU
@@ -1433,29 +1542,29 @@ struct Mark {
 };
 }  // namespace
 
-hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bufs& b, const Tuning& T, float* x,
-                       float* v, uint32_t n, const float lo[3], const float hi[3], int iters, int xsph_on,
-                       int vort_on, hipStream_t s, Timer* tm) {
+namespace {
+template <bool kB>
+hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float* x, float* v, uint32_t n, int iters,
+                    int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* tm) {
     const uint32_t nb = ceil_div(n > 0 ? n : 1, 256);
-    const uint32_t uw = T.unit_width < G.gx ? T.unit_width : G.gx;
+    const uint32_t CT = b.CT;
     const uint32_t budget = (T.tile_cap * 3u) / 4u > 0 ? (T.tile_cap * 3u) / 4u : 1u;
     auto grid = [&](int k) { return dim3(T.grid[k] ? T.grid[k] : T.grid_units); };
     const size_t smem = smem_of(T.tile_cap);
     if (n > 0) {
         Mark m(tm, s, K_PREDICT_HASH);
-        hipLaunchKernelGGL(k_predict_hash, dim3(nb), dim3(256), 0, s, x, v, n, G, P.dt, P.g[0], P.g[1], P.g[2], lo[0],
-                           lo[1], lo[2], hi[0], hi[1], hi[2], b.key, b.rank, b.count, b.tflag, b.lgT);
+        hipLaunchKernelGGL(k_predict_hash<kB>, dim3(nb), dim3(256), 0, s, x, v, n, b, SC, b.key, b.rank, b.count, b.tflag,
+                           b.lgT);
     }
     {
         Mark m(tm, s, K_SCAN_TILES);
-        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.tflag, b.tsum, b.nT, b.lgT,
-                           G.C);
+        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.tflag, b.tsum, b.nT, b.lgT, CT);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.tsum, b.tprefix, b.nT + 1, b.ctr, 1u);
     }
     {
         Mark m(tm, s, K_SCAN_CELLS);
         hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.offset, b.tsum, b.tflag,
-                           b.tprefix, b.nT, b.lgT, G.C);
+                           b.tprefix, b.nT, b.lgT, CT);
     }
     if (n == 0) return hipGetLastError();
     {
@@ -1464,99 +1573,114 @@ hipError_t launch_step(const Grid& G, const Phys& P, const Boundary& B, const Bu
     }
     {
         Mark m(tm, s, K_REORDER);
-        hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, x, v, G, P.dt, P.g[0], P.g[1], P.g[2], b.key, b.tmp,
-                           b.offset, b.sidx, b.Pf, b.X, b.cap, n, uw, budget, b.units, b.ctr);
+        hipLaunchKernelGGL(k_reorder<kB>, dim3(nb), dim3(256), 0, s, x, v, b, SC, b.key, b.tmp, n, uw, budget);
     }
     {
         Mark m(tm, s, K_NS);
-        hipLaunchKernelGGL(k_ns, grid(K_NS), dim3(kBlock), smem, s, G, P, b, T.tile_cap);
+        hipLaunchKernelGGL(k_ns<kB>, grid(K_NS), dim3(kBlock), smem, s, b, SC, T.tile_cap);
     }
     for (int it = 0; it < iters; ++it) {
         {
             Mark m(tm, s, K_LAMBDA);
-            hipLaunchKernelGGL(k_lambda, dim3(nb), dim3(256), 0, s, G, P, b);
+            hipLaunchKernelGGL(k_lambda<kB>, dim3(nb), dim3(256), 0, s, b, SC);
         }
         Mark m(tm, s, K_DELTAP);
-        hipLaunchKernelGGL(k_deltap, dim3(nb), dim3(256), 0, s, G, P, B, b);
+        hipLaunchKernelGGL(k_deltap<kB>, dim3(nb), dim3(256), 0, s, b, SC);
     }
-    if (!vort_on && !xsph_on) {
+    if (!any_vort && !any_xsph) {
         Mark m(tm, s, K_OUT);
-        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float4*)b.Pf, (const float4*)nullptr, G.C,
-                           x, v);
+        hipLaunchKernelGGL(k_out<kB>, dim3(nb), dim3(256), 0, s, b, SC, (const float4*)nullptr, x, v);
         return hipGetLastError();
     }
-    // runs of uw final cells, nc = min(8, uw) counters each: CR = C / uw * nc
-    // counters (never more than C), scanned in tiles of up to 4096
+    // runs of uw final cells, nc = min(8, uw) counters each: CR = CT / uw * nc
+    // counters (never more than CT), scanned in tiles of up to 4096 that divide CR
     uint32_t lguw = 0;
     while ((1u << lguw) < uw) ++lguw;
     const uint32_t lgnc = lguw < 3u ? lguw : 3u;
-    const uint32_t CR = (G.C >> lguw) << lgnc;
+    const uint32_t CR = (CT >> lguw) << lgnc;
     uint32_t lgTr = 0;
-    while ((1u << (lgTr + 1)) <= CR && lgTr < 12u) ++lgTr;
+    while (lgTr < 12u && ((CR >> (lgTr + 1)) << (lgTr + 1)) == CR && (1u << (lgTr + 1)) <= CR) ++lgTr;
     const uint32_t nTr = CR >> lgTr;
     {
         Mark m(tm, s, K_CUR_SORT);
-        hipLaunchKernelGGL(k_cur_hash, dim3(nb), dim3(256), 0, s, (const float4*)b.Pf, (const float*)b.X, P.inv_dt,
-                           b.cap, b.offset, G, lguw, lgnc, b.ckey, b.crank, b.ccount, b.ctflag, lgTr, b.R);
+        hipLaunchKernelGGL(k_cur_hash<kB>, dim3(nb), dim3(256), 0, s, b, SC, lguw, lgnc, lgTr);
         hipLaunchKernelGGL(k_tile_reduce, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, nTr, lgTr, CR);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.ctsum, b.ctprefix, nTr + 1, b.ctr, 0u);
         hipLaunchKernelGGL(k_scan_cells, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
                            b.ctprefix, nTr, lgTr, CR);
-        hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, G, b.ckey, b.crank, b.coffset,
+        hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, CT, b.ckey, b.crank, b.coffset,
                            b.cperm, uw, lguw, lgnc, b.cunits, b.ctr);
     }
     {
         Mark m(tm, s, K_FINAL_LIST);
-        hipLaunchKernelGGL(k_final_list, grid(K_FINAL_LIST), dim3(kBlock), smem, s, G, P, b, T.tile_cap, lguw,
-                           lgnc);
+        hipLaunchKernelGGL(k_final_list<kB>, grid(K_FINAL_LIST), dim3(kBlock), smem, s, b, SC, T.tile_cap, lguw, lgnc);
     }
-    if (vort_on) {
+    if (any_vort) {
         {
             Mark m(tm, s, K_VORT);
-            hipLaunchKernelGGL(k_vort, dim3(nb), dim3(kBlock), 0, s, G, P, b);
+            hipLaunchKernelGGL(k_vort<kB>, dim3(nb), dim3(kBlock), 0, s, b, SC);
         }
         Mark m(tm, s, K_CONFINE);
-        hipLaunchKernelGGL(k_confine, dim3(nb), dim3(kBlock), 0, s, G, P, b);
+        hipLaunchKernelGGL(k_confine<kB>, dim3(nb), dim3(kBlock), 0, s, b, SC);
     }
-    if (xsph_on) {
+    if (any_xsph) {
         Mark m(tm, s, K_XSPH_OUT);
-        hipLaunchKernelGGL(k_xsph_out, dim3(nb), dim3(kBlock), 0, s, G, P, b, x, v);
+        hipLaunchKernelGGL(k_xsph_out<kB>, dim3(nb), dim3(kBlock), 0, s, b, SC, x, v);
     } else {
         Mark m(tm, s, K_OUT);
-        hipLaunchKernelGGL(k_out, dim3(nb), dim3(256), 0, s, P, b, (const float4*)b.Pf, (const float4*)b.R, G.C,
-                           x, v);
+        hipLaunchKernelGGL(k_out<kB>, dim3(nb), dim3(256), 0, s, b, SC, (const float4*)b.R, x, v);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_neighbors_view(const Grid& G, const Phys& P, const Bufs& b, const Tuning& T, uint32_t* out,
-                                 uint32_t* counts, uint32_t max_pair, hipStream_t s) {
-    (void)P;
-    (void)T;
-    hipLaunchKernelGGL(k_neighbors_view, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, G, b, out,
-                       counts, max_pair);
+}  // namespace
+
+hipError_t launch_step(const Bufs& b, const Scene& s0, const Tuning& T, float* x, float* v, uint32_t n, int iters,
+                       int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* tm) {
+    if (b.nscenes == 1) {
+        const Scenes<false> SC{b.scenes, s0};
+        return step_seq<false>(b, SC, T, x, v, n, iters, any_xsph, any_vort, uw, s, tm);
+    }
+    const Scenes<true> SC{b.scenes, s0};
+    return step_seq<true>(b, SC, T, x, v, n, iters, any_xsph, any_vort, uw, s, tm);
+}
+
+hipError_t launch_scene_offsets(const Bufs& b, uint32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_scene_offsets, dim3(ceil_div(b.nscenes + 1u, 256)), dim3(256), 0, s, b, out);
     return hipGetLastError();
 }
 
-hipError_t launch_cell_ranges(const Grid& G, const Bufs& b, uint32_t* start, uint32_t* end, hipStream_t s) {
-    hipLaunchKernelGGL(k_cell_ranges, dim3(ceil_div(G.C, 256)), dim3(256), 0, s, b.offset, G.C, start, end);
+hipError_t launch_neighbors_view(const Bufs& b, uint32_t* out, uint32_t* counts, uint32_t max_pair, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbors_view, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b, out, counts,
+                       max_pair);
     return hipGetLastError();
 }
 
-hipError_t launch_copy_lambda(const Grid& G, const Bufs& b, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy_lambda, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b.offset, G.C,
+hipError_t launch_cell_ranges(const Bufs& b, uint32_t C, uint32_t* start, uint32_t* end, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_ranges, dim3(ceil_div(C, 256)), dim3(256), 0, s, b.offset, C, start, end);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_lambda(const Bufs& b, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_lambda, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b.offset, b.CT,
                        (const float4*)b.Q, out);
     return hipGetLastError();
 }
 
-hipError_t launch_density(const Grid& G, const Phys& P, const Bufs& b, const float* x, float* rho, hipStream_t s) {
-    hipLaunchKernelGGL(k_density, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, G, P, b, x, rho);
+hipError_t launch_density(const Bufs& b, const Scene& s0, const float* x, float* rho, hipStream_t s) {
+    const dim3 g(ceil_div(b.cap > 0 ? b.cap : 1, 256));
+    if (b.nscenes == 1) {
+        const Scenes<false> SC{b.scenes, s0};
+        hipLaunchKernelGGL(k_density<false>, g, dim3(256), 0, s, b, SC, x, rho);
+    } else {
+        const Scenes<true> SC{b.scenes, s0};
+        hipLaunchKernelGGL(k_density<true>, g, dim3(256), 0, s, b, SC, x, rho);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_restore(const Grid& G, const Bufs& b, const float* xs, const float* vs, float* xo, float* vo,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_restore, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, G, b, xs, vs, xo, vo);
+hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float* xo, float* vo, hipStream_t s) {
+    hipLaunchKernelGGL(k_restore, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b, xs, vs, xo, vo);
     return hipGetLastError();
 }
 

@@ -63,6 +63,23 @@ class Params(ctypes.Structure):
     ]
 
 
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("params", Params),
+        ("capacity", ctypes.c_uint32),
+        ("domain_end", ctypes.c_float * 3),
+        ("inner_spheres4", ctypes.c_void_p),
+        ("n_inner", ctypes.c_uint32),
+        ("outer_spheres4", ctypes.c_void_p),
+        ("n_outer", ctypes.c_uint32),
+        ("plane_points3", ctypes.c_void_p),
+        ("plane_normals3", ctypes.c_void_p),
+        ("n_planes", ctypes.c_uint32),
+        ("area_lo", ctypes.c_float * 3),
+        ("area_hi", ctypes.c_float * 3),
+    ]
+
+
 class NsView(ctypes.Structure):
     _fields_ = [
         ("grid", ctypes.c_uint32 * 3),
@@ -108,6 +125,11 @@ def lib():
             "pbfx_step_async": ([vp, vp, vp, u32, f32p, f32p, ctypes.c_int, vp], ctypes.c_int),
             "pbfx_step_no_boundary": ([vp, vp, vp, u32p, f32p, f32p, ctypes.c_int, vp], ctypes.c_int),
             "pbfx_get_num": ([vp, vp, u32p], ctypes.c_int),
+            "pbfx_batch_create": ([ctypes.POINTER(SceneDesc), u32, ctypes.POINTER(vp)], ctypes.c_int),
+            "pbfx_batch_step": ([vp, vp, vp, u32p, ctypes.c_int, vp], ctypes.c_int),
+            "pbfx_batch_step_async": ([vp, vp, vp, vp, u32, ctypes.c_int, vp], ctypes.c_int),
+            "pbfx_batch_offsets": ([vp, vp, vp], ctypes.c_int),
+            "pbfx_batch_set_params": ([vp, u32, ctypes.POINTER(Params)], ctypes.c_int),
             "pbfx_get_stats": ([vp, vp, ctypes.POINTER(Stats)], ctypes.c_int),
             "pbfx_ns_view_get": ([vp, ctypes.POINTER(NsView)], ctypes.c_int),
             "pbfx_ns_neighbors": ([vp, vp, vp, vp], ctypes.c_int),
@@ -338,3 +360,52 @@ def d2h(ptr: int, count: int, dtype=np.uint32) -> np.ndarray:
         if rc != 0:
             raise RuntimeError(f"hipMemcpy failed ({rc})")
     return out
+
+
+class BatchSolver(Solver):
+    """A batched pbfx context: independent scenes advanced by one sequence of
+    launches per step (pbfx_batch_*).  Particle arrays hold the scenes
+    concatenated in order."""
+
+    def __init__(self, scenes: Sequence[Scene], capacities: Optional[Sequence[int]] = None, boundary: bool = True):
+        self._lib = lib()
+        self.scenes = list(scenes)
+        caps = [int(c) for c in (capacities or [sc.n for sc in scenes])]
+        self.capacities = caps
+        self.capacity = sum(caps)
+        self.params = scenes[0].params
+        descs = (SceneDesc * len(scenes))()
+        keep = []
+        for d, sc, cap in zip(descs, scenes, caps):
+            d.params = make_params(sc.params)
+            d.capacity = cap
+            d.domain_end[:] = [float(c) for c in sc.box_hi]
+            d.area_lo[:] = [float(c) for c in sc.area_lo]
+            d.area_hi[:] = [float(c) for c in sc.area_hi]
+            if boundary:
+                arrs = [np.ascontiguousarray(a, np.float32) for a in
+                        (sc.inner_spheres, sc.outer_spheres, sc.plane_points, sc.plane_normals)]
+                keep.extend(arrs)
+                d.inner_spheres4, d.outer_spheres4, d.plane_points3, d.plane_normals3 = [
+                    a.ctypes.data if a.size else None for a in arrs]
+                d.n_inner, d.n_outer, d.n_planes = arrs[0].shape[0], arrs[1].shape[0], arrs[2].shape[0]
+        self._ctx = ctypes.c_void_p()
+        _check(self._lib.pbfx_batch_create(descs, len(scenes), ctypes.byref(self._ctx)), "pbfx_batch_create")
+
+    def batch_step(self, x, v, counts: Sequence[int], iters: int, stream=None):
+        """One step of every scene; returns the live counts per scene."""
+        c = (ctypes.c_uint32 * len(counts))(*[int(n) for n in counts])
+        _check(self._lib.pbfx_batch_step(self._ctx, _ptr(x), _ptr(v), c, int(iters), _stream(stream)),
+               "pbfx_batch_step")
+        return list(c)
+
+    def batch_step_async(self, x, v, in_offsets, total: int, iters: int, stream=None) -> None:
+        _check(self._lib.pbfx_batch_step_async(self._ctx, _ptr(x), _ptr(v), _ptr(in_offsets), int(total), int(iters),
+                                               _stream(stream)), "pbfx_batch_step_async")
+
+    def batch_offsets(self, out, stream=None) -> None:
+        _check(self._lib.pbfx_batch_offsets(self._ctx, _ptr(out), _stream(stream)), "pbfx_batch_offsets")
+
+    def batch_set_params(self, scene: int, params: StepParams) -> None:
+        _check(self._lib.pbfx_batch_set_params(self._ctx, int(scene), ctypes.byref(make_params(params))),
+               "pbfx_batch_set_params")

@@ -296,6 +296,34 @@ def config4(seed: int = 1) -> Scene:
                      v_sigma=0.1)
 
 
+def batch_config3(n_scenes: int = 256, seed: int = 3, n_min: int = 4096, n_max: int = 200000,
+                  iters: int = 4) -> List[Scene]:
+    """The batched workload: n_scenes independent dam breaks with particle
+    counts log-uniform in [n_min, n_max] (seeded), each the 32k dam's
+    geometry scaled to its count (an a x a x c lattice of spacing 0.5h,
+    h = 0.1, in the corner of a box 2a x 2a x c lattice spacings wide), and
+    per-scene rest density scale U[0.9, 1.1], XSPH c U[0.005, 0.05] and
+    vorticity eps U[0, 0.001]."""
+    h = 0.1
+    u = uniform01_f32(seed, np.arange(4 * n_scenes, dtype=np.uint64)).astype(np.float64).reshape(n_scenes, 4)
+    out = []
+    for k in range(n_scenes):
+        n = int(round(math.exp(math.log(n_min) + u[k, 0] * (math.log(n_max) - math.log(n_min)))))
+        a = max(2, int(round(n ** (1.0 / 3.0))))
+        c = max(1, int(round(n / (a * a))))
+        s = 0.5 * h
+        box = (2 * a * s, 2 * a * s, c * s + s)
+        sc = dam_break(f"batch{k}_{a}x{a}x{c}", h, box, [corner_block(h, (a, a, c))], iters=iters,
+                       seed=seed * 1000003 + k)
+        p = sc.params
+        rho_scale = 0.9 + 0.2 * u[k, 1]
+        p = dataclasses.replace(p, stable_density=float(np.float32(p.stable_density * rho_scale)),
+                                xsph_parameter=float(np.float32(0.005 + 0.045 * u[k, 2])),
+                                vc_parameter=float(np.float32(0.001 * u[k, 3])))
+        out.append(dataclasses.replace(sc, params=p))
+    return out
+
+
 def small_dam(nx: int = 12, ny: int = 10, nz: int = 8, h: float = 0.1, box=(1.6, 1.6, 0.8), iters: int = 4,
               seed: int = 7, **overrides) -> Scene:
     """A small dam break for fast parity checks."""

@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps of the per-kernel event pass")
     ap.add_argument("--batch-steps", type=int, default=10, help="timed steps of the 256-scene batch (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
-    ap.add_argument("--tile-cap", type=int, default=2048)
+    ap.add_argument("--tile-cap", type=int, default=None, help="LDS tile (particles); default: autotuned")
     ap.add_argument("--unit-width", type=int, default=16)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="finish with a plain device copy of this many MiB (a known-size kernel for "
@@ -97,7 +97,6 @@ def run_gpu(scene, steps, warmup, kernel_steps, tile_cap, unit_width):
 
     n = scene.n
     solver = pbfx.Solver.for_scene(scene)
-    solver.set_tuning(tile_cap, unit_width)
     stream = torch.cuda.current_stream()
     x0 = torch.from_numpy(scene.x).cuda()
     v0 = torch.from_numpy(scene.v).cuda()
@@ -105,6 +104,12 @@ def run_gpu(scene, steps, warmup, kernel_steps, tile_cap, unit_width):
     nbuf = max(steps, 1)
     xs = [x0.clone() for _ in range(nbuf)]
     vs = [v0.clone() for _ in range(nbuf)]
+    if tile_cap is None:
+        tile_cap, unit_width = solver.autotune(x0, v0, n, lo, hi, iters, xs[0], vs[0], stream=stream)
+        xs[0].copy_(x0)
+        vs[0].copy_(v0)
+        log(f"{scene.name}: autotuned tile_cap={tile_cap} unit_width={unit_width}")
+    solver.set_tuning(tile_cap, unit_width)
     for k in range(warmup):
         solver.step_async(xs[k % nbuf], vs[k % nbuf], n, lo, hi, iters, stream)
     for k in range(min(warmup, nbuf)):
@@ -138,7 +143,8 @@ def run_gpu(scene, steps, warmup, kernel_steps, tile_cap, unit_width):
     solver.close()
     del xs, vs
     torch.cuda.empty_cache()
-    return {"ms_total": ms, "wall_s": wall, "steps": steps, "kernels": kt, "stats": stats}
+    return {"ms_total": ms, "wall_s": wall, "steps": steps, "kernels": kt, "stats": stats,
+            "tuning": {"tile_cap": tile_cap, "unit_width": unit_width}}
 
 
 def run_batch(scs, steps, warmup):
@@ -162,6 +168,8 @@ def run_batch(scs, steps, warmup):
     x, v = x0.clone(), v0.clone()
     stream = torch.cuda.Stream()
     batch = pbfx.BatchSolver(scs)
+    tune = batch.autotune(x0, v0, n, (0, 0, 0), (0, 0, 0), iters, x, v, in_offsets=off, stream=stream)
+    log(f"batch: autotuned tile_cap={tune[0]} unit_width={tune[1]}")
     with torch.cuda.stream(stream):
         for _ in range(max(warmup, 1)):
             x.copy_(x0)
@@ -212,7 +220,8 @@ def run_batch(scs, steps, warmup):
     log(f"loop: {ms_loop:.3f} ms/step")
     del x, v, x0, v0, xb, vb
     torch.cuda.empty_cache()
-    return {"ms_batch": ms_batch, "ms_loop": ms_loop, "bit_exact": same, "particles": n}
+    return {"ms_batch": ms_batch, "ms_loop": ms_loop, "bit_exact": same, "particles": n,
+            "tuning": {"tile_cap": tune[0], "unit_width": tune[1]}}
 
 
 def kernel_report(kt, n, cells):
@@ -290,6 +299,7 @@ def main():
             "particles": n,
             "cells": C,
             "iterations": I,
+            "tuning": r["tuning"],
         },
         "roofline": {
             "bound": "hbm",
@@ -338,6 +348,7 @@ def main():
             "loop_ms_per_step": round(rb["ms_loop"], 4),
             "speedup_vs_loop": round(rb["ms_loop"] / rb["ms_batch"], 3),
             "bit_exact_vs_loop": rb["bit_exact"],
+            "tuning": rb["tuning"],
             "steps": a.batch_steps,
         }
     if a.cpu_seconds > 0:

@@ -252,6 +252,17 @@ pbfx_status pbfx_get_stats(pbfx_ctx* ctx, void* stream, pbfx_stats* out);
  * unit_width: max cells per work unit along x (power of two 1..16, default 16). */
 pbfx_status pbfx_set_tuning(pbfx_ctx* ctx, uint32_t tile_cap, uint32_t unit_width);
 
+/* Autotuner: times one step of the given state for every candidate
+ * (tile_cap in {1024, 2048, 3072, 4096} x unit_width in {4, 8, 16}, within
+ * the grids' limits) on the scratch copies x_scratch / v_scratch (x and v are
+ * only read), applies the fastest and returns it.  A single-scene context
+ * steps n particles with area [lo, hi]; a batch steps the scenes described
+ * by the device offsets in_offsets (lo, hi ignored).  No counterpart in the
+ * reference. */
+pbfx_status pbfx_autotune(pbfx_ctx* ctx, const float* x, const float* v, uint32_t n, const uint32_t* in_offsets,
+                          const float lo[3], const float hi[3], int iters, float* x_scratch, float* v_scratch,
+                          uint32_t* tile_cap_out, uint32_t* unit_width_out, void* stream);
+
 /* When non-zero, pbfx_step returns PBFX_NEIGHBOR_TRUNCATED / PBFX_NOT_CONVERGED
  * after a step in which those events happened (default 0: they are only
  * counted in pbfx_stats, matching the reference's silent behaviour). */

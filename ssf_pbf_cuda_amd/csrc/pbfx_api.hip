@@ -799,6 +799,69 @@ pbfx_status pbfx_set_tuning(pbfx_ctx* c, uint32_t tile_cap, uint32_t unit_width)
     return PBFX_OK;
 }
 
+pbfx_status pbfx_autotune(pbfx_ctx* c, const float* x, const float* v, uint32_t n, const uint32_t* in_offsets,
+                          const float lo[3], const float hi[3], int iters, float* xs, float* vs, uint32_t* tile_out,
+                          uint32_t* uw_out, void* stream) {
+    if (!c || !x || !v || !xs || !vs || (c->batch && !in_offsets) || (!c->batch && (!lo || !hi)))
+        return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    const hipStream_t s = (hipStream_t)stream;
+    if (!c->batch)
+        for (int a = 0; a < 3; ++a) {
+            c->sc[0].lo[a] = lo[a];
+            c->sc[0].hi[a] = hi[a];
+        }
+    Timer* keep = c->timer;
+    c->timer = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const Tuning saved = c->T;
+    uint32_t best_tile = saved.tile_cap, best_uw = saved.unit_width;
+    float best = 3.4e38f;
+    const uint32_t tiles[] = {1024, 2048, 3072, 4096};
+    const uint32_t widths[] = {4, 8, 16};
+    pbfx_status st = PBFX_OK;
+    for (uint32_t tc : tiles)
+        for (uint32_t uw : widths) {
+            if (e != hipSuccess || st != PBFX_OK) break;
+            if (uw > c->uw_max && uw != widths[0]) continue;
+            c->T.tile_cap = tc;
+            c->T.unit_width = uw;
+            set_grid_units(c);
+            float t = 3.4e38f;
+            for (int rep = 0; rep < 2 && e == hipSuccess && st == PBFX_OK; ++rep) {
+                e = hipMemcpyAsync(xs, x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = hipMemcpyAsync(vs, v, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = hipEventRecord(e0, s);
+                if (e == hipSuccess)
+                    st = do_step(c, xs, vs, n, c->batch ? in_offsets : c->d_in_off, iters, false, s);
+                if (e == hipSuccess && st == PBFX_OK) e = hipEventRecord(e1, s);
+                if (e == hipSuccess && st == PBFX_OK) e = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (e == hipSuccess && st == PBFX_OK) e = hipEventElapsedTime(&ms, e0, e1);
+                if (ms < t) t = ms;
+            }
+            if (t < best) {
+                best = t;
+                best_tile = tc;
+                best_uw = uw;
+            }
+        }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->timer = keep;
+    c->T = saved;
+    c->T.tile_cap = best_tile;
+    c->T.unit_width = best_uw;
+    set_grid_units(c);
+    if (e != hipSuccess) return hip_fail(c, e);
+    if (st != PBFX_OK) return st;
+    if (tile_out) *tile_out = best_tile;
+    if (uw_out) *uw_out = best_uw;
+    return PBFX_OK;
+}
+
 void pbfx_set_report_info(pbfx_ctx* c, int enable) {
     if (c) c->report_info = enable;
 }

@@ -138,6 +138,7 @@ def lib():
             "pbfx_density": ([vp, vp, vp, vp], ctypes.c_int),
             "pbfx_restore_order": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "pbfx_set_tuning": ([vp, u32, u32], ctypes.c_int),
+            "pbfx_autotune": ([vp, vp, vp, u32, vp, f32p, f32p, ctypes.c_int, vp, vp, u32p, u32p, vp], ctypes.c_int),
             "pbfx_set_report_info": ([vp, ctypes.c_int], None),
             "pbfx_get_grid": ([vp, u32p], ctypes.c_int),
             "pbfx_calc_grid_size": ([ctypes.c_double, f32p, u32p], ctypes.c_int),
@@ -252,6 +253,14 @@ class Solver:
         self._keep = (ai, ao, pp, pn)
         _check(self._lib.pbfx_set_boundary(self._ctx, fp(ai), ai.shape[0], fp(ao), ao.shape[0], fp(pp), fp(pn),
                                            pp.shape[0]), "pbfx_set_boundary")
+
+    def autotune(self, x, v, num: int, lo, hi, iters: int, x_scratch, v_scratch, in_offsets=None, stream=None):
+        """Times each (tile_cap, unit_width) candidate on scratch copies of x, v and keeps the fastest."""
+        t, w = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self._lib.pbfx_autotune(self._ctx, _ptr(x), _ptr(v), int(num), _ptr(in_offsets), _f3(lo), _f3(hi),
+                                       int(iters), _ptr(x_scratch), _ptr(v_scratch), ctypes.byref(t),
+                                       ctypes.byref(w), _stream(stream)), "pbfx_autotune")
+        return int(t.value), int(w.value)
 
     def set_tuning(self, tile_cap: int = 2048, unit_width: int = 16) -> None:
         _check(self._lib.pbfx_set_tuning(self._ctx, int(tile_cap), int(unit_width)), "pbfx_set_tuning")

@@ -89,3 +89,27 @@ def test_batch_deletion_per_scene():
     res = _run_batch(scs, 1)
     assert res[1][2] < scs[1].n
     _check_against_oracle(scs, res, 1)
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_results():
+    # the autotuner only times candidates on scratch copies; the step after
+    # it is still the oracle's, bit for bit
+    import torch
+
+    from ssf_pbf_cuda_amd import pbfx
+
+    sc = scenes.small_dam(12, 10, 8)
+    print(f"autotune: {sc.name} n={sc.n}", flush=True)
+    solver = pbfx.Solver.for_scene(sc)
+    x = torch.from_numpy(sc.x).cuda()
+    v = torch.from_numpy(sc.v).cuda()
+    xs, vs = torch.empty_like(x), torch.empty_like(v)
+    tile, width = solver.autotune(x, v, sc.n, sc.area_lo, sc.area_hi, sc.iters, xs, vs)
+    assert tile in (1024, 2048, 3072, 4096) and width in (4, 8, 16)
+    assert np.array_equal(x.cpu().numpy(), sc.x)
+    n = solver.step(x, v, sc.n, sc.area_lo, sc.area_hi, sc.iters)
+    ref = oracle_run(sc)
+    assert n == ref["num"]
+    assert np.array_equal(x.cpu().numpy()[:n].view(np.uint32), ref["x"].view(np.uint32))
+    solver.close()

@@ -295,6 +295,42 @@ pbfx_status pbfx_set_kernel_timing(pbfx_ctx* ctx, int enable);
 pbfx_status pbfx_kernel_timing_read(pbfx_ctx* ctx, void* stream, pbfx_kernel_time* out, uint32_t max,
                                     uint32_t* count);
 
+/* ---- Differentiable form.  No counterpart in the reference: its step has no
+ * adjoint.  Gradients of a scalar loss L of the state after `steps` plain
+ * steps of a single-scene context, with respect to the initial positions and
+ * velocities, the XSPH coefficient c (xsph_parameter) and the vorticity
+ * epsilon (vc_parameter).  The step is piecewise smooth; the gradient is that
+ * of the smooth map left by the forward run's discrete decisions (sort and
+ * frozen neighbour lists, boundary responses that fired, final-pass cells,
+ * XSPH NaN guard).
+ *
+ * pbfx_adj_enable reserves, for up to max_steps steps, a checkpoint of the
+ * state every `every` steps and the tape of one window of `every` steps
+ * (about (200 + 16 iters) bytes per particle per step of the window).
+ * Requires every grid axis to have at least 4 cells. */
+pbfx_status pbfx_adj_enable(pbfx_ctx* ctx, uint32_t max_steps, uint32_t every);
+
+/* Runs `steps` steps from (x, v, *num) as pbfx_step does (device AoS float3,
+ * overwritten with the final state, *num updated), keeping the checkpoints
+ * and the identity of every final slot. */
+pbfx_status pbfx_adj_forward(pbfx_ctx* ctx, float* x, float* v, uint32_t* num, const float lo[3],
+                             const float hi[3], int iters, uint32_t steps, void* stream);
+
+/* id[s] = index in the initial state of the particle in final slot s
+ * (device uint32 [num after the forward]). */
+pbfx_status pbfx_adj_identity(pbfx_ctx* ctx, uint32_t* id, void* stream);
+
+/* Backward pass: gx, gv = dL/d(final x, v) (device AoS float3, final order);
+ * gx0, gv0 = dL/d(initial x, v) (device AoS float3, initial order);
+ * gparam[0] = dL/dc, gparam[1] = dL/d(vorticity epsilon) (host).  Every
+ * window of steps is recomputed from its checkpoint with its tape recorded,
+ * then the adjoint kernels run through it in reverse.  The adjoint carries a
+ * power-of-two scale: the true gradients are the returned values times
+ * 2^(*exponent) (0 unless they would leave the FP32 range).  gx and gv are
+ * not modified. */
+pbfx_status pbfx_adj_backward(pbfx_ctx* ctx, const float* gx, const float* gv, float* gx0, float* gv0,
+                              double gparam[2], int32_t* exponent, void* stream);
+
 /* Arithmetic self-test on the device: for n inputs computes
  * out[4*i+0] = a/b, out[4*i+1] = sqrtf(a), out[4*i+2] = fmaf(a,b,a),
  * out[4*i+3] = floorf(a/b), bitwise comparable with IEEE host results.

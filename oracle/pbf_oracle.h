@@ -146,6 +146,41 @@ int orc_density(const orc_params* p, const float* x, uint32_t n, const uint32_t*
    a/b, sqrtf(a), fmaf(a,b,a), floorf(a/b). */
 void orc_arith_f32(const float* a, const float* b, float* out, uint32_t n);
 
+/* ---- The step as a map with frozen discrete decisions, and its FP64 adjoint
+ * (pbf_adjoint.cpp).  A tape holds, per step, the decisions of an FP32 run
+ * (sort, frozen lists, boundary responses fired, final-pass cells, XSPH NaN
+ * guard) and the values computed under them. */
+typedef struct orc_tape orc_tape;
+orc_tape* orc_tape_new(void);
+void orc_tape_free(orc_tape* t);
+uint32_t orc_tape_steps(const orc_tape* t);
+/* which = 0: particles entering step `step`; 1: live after it. */
+uint32_t orc_tape_num(const orc_tape* t, uint32_t step, int which);
+
+/* One FP32 step, bit-identical to orc_step_f32, appended to the tape. */
+int orc_adj_step_f32(const orc_params* p, const orc_boundary* b, float* x, float* v, uint32_t* num,
+                     const float lo[3], const float hi[3], int iters, orc_tape* t);
+
+/* Every recorded step again in FP64 from (x0, v0) (initial order) under the
+ * tape's decisions.  coef: {XSPH coefficient, vorticity epsilon} in double
+ * (NULL: the float ones of p).  record != 0 replaces the tape's values by the
+ * FP64 ones.  x_out, v_out: the final state (live count of the last step,
+ * final order); may be NULL. */
+int orc_adj_replay_f64(const orc_params* p, const orc_boundary* b, orc_tape* t, const double* x0, const double* v0,
+                       const double* coef, double* x_out, double* v_out, int record);
+
+/* Adjoint through every recorded step, at the tape's values.  gx, gv:
+ * dL/d(final x, v) in the final order; gx0, gv0: dL/d(x0, v0) in the initial
+ * order; gparam[0] = dL/d xsph_parameter, gparam[1] = dL/d vc_parameter;
+ * coef as for the replay.  The adjoint is kept near 1 by a power-of-two
+ * scale: the true gradients are the outputs times 2^(*exponent) (may be NULL). */
+int orc_adj_backward_f64(const orc_params* p, const orc_boundary* b, const orc_tape* t, const double* coef,
+                         const double* gx, const double* gv, double* gx0, double* gv0, double gparam[2],
+                         int32_t* exponent);
+
+/* id[s] = initial index of the particle in final slot s. */
+int orc_adj_identity(const orc_tape* t, uint32_t* id);
+
 /* Number of OpenMP threads the oracle uses (0 = runtime default). */
 void orc_set_threads(int n);
 int orc_get_threads(void);

@@ -122,6 +122,26 @@ def lib():
         L.orc_density.restype = ctypes.c_int
         L.orc_arith_f32.argtypes = [f32p, f32p, f32p, ctypes.c_uint32]
         L.orc_arith_f32.restype = None
+        vp = ctypes.c_void_p
+        L.orc_tape_new.argtypes = []
+        L.orc_tape_new.restype = vp
+        L.orc_tape_free.argtypes = [vp]
+        L.orc_tape_free.restype = None
+        L.orc_tape_steps.argtypes = [vp]
+        L.orc_tape_steps.restype = ctypes.c_uint32
+        L.orc_tape_num.argtypes = [vp, ctypes.c_uint32, ctypes.c_int]
+        L.orc_tape_num.restype = ctypes.c_uint32
+        L.orc_adj_step_f32.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcBoundary), f32p, f32p, u32p,
+                                       f32p, f32p, ctypes.c_int, vp]
+        L.orc_adj_step_f32.restype = ctypes.c_int
+        L.orc_adj_replay_f64.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcBoundary), vp, f64p, f64p,
+                                         f64p, f64p, f64p, ctypes.c_int]
+        L.orc_adj_replay_f64.restype = ctypes.c_int
+        L.orc_adj_backward_f64.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcBoundary), vp, f64p, f64p,
+                                           f64p, f64p, f64p, f64p, ctypes.POINTER(ctypes.c_int32)]
+        L.orc_adj_backward_f64.restype = ctypes.c_int
+        L.orc_adj_identity.argtypes = [vp, u32p]
+        L.orc_adj_identity.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_threads.restype = None
         L.orc_get_threads.argtypes = []
@@ -324,3 +344,113 @@ def metrics(scene, x, v, params=None, grid=None):
     out = np.zeros(4, np.float64)
     lib().orc_metrics(ctypes.byref(op), _fp(x), _fp(v), x.shape[0], _dp(out))
     return {"mean_C": out[0], "max_C": out[1], "mean_pos_C": out[2], "kinetic_energy": out[3]}
+
+
+class Tape:
+    """The step as a map with frozen discrete decisions (pbf_adjoint.cpp):
+    ``run`` records FP32 steps (bit-identical to :func:`step`), ``replay``
+    re-evaluates them in FP64 under the recorded decisions (optionally with
+    other XSPH / vorticity coefficients), ``backward`` is the FP64 adjoint at
+    the tape's values."""
+
+    def __init__(self, scene, params=None, boundary=True, grid=None):
+        self.scene = scene
+        self.params = params if params is not None else scene.params
+        self.grid = tuple(grid) if grid is not None else tuple(scene.grid)
+        self._bnd = _Boundary(scene) if boundary else None
+        self._h = lib().orc_tape_new()
+
+    def close(self):
+        if self._h:
+            lib().orc_tape_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _bp(self):
+        return ctypes.byref(self._bnd.struct) if self._bnd else None
+
+    def _op(self, params=None):
+        return make_params(params if params is not None else self.params, self.grid)
+
+    @property
+    def steps(self) -> int:
+        return int(lib().orc_tape_steps(self._h))
+
+    def run(self, steps, x=None, v=None, iters=None):
+        """Appends `steps` FP32 steps from (x, v); returns the final (x, v)."""
+        sc = self.scene
+        x = np.array(sc.x if x is None else x, dtype=np.float32, copy=True).reshape(-1, 3)
+        v = np.array(sc.v if v is None else v, dtype=np.float32, copy=True).reshape(-1, 3)
+        iters = sc.iters if iters is None else iters
+        op = self._op()
+        lo = np.ascontiguousarray(sc.area_lo, dtype=np.float32)
+        hi = np.ascontiguousarray(sc.area_hi, dtype=np.float32)
+        num = np.array([x.shape[0]], np.uint32)
+        for _ in range(steps):
+            rc = lib().orc_adj_step_f32(ctypes.byref(op), self._bp(), _fp(x), _fp(v), _up(num), _fp(lo), _fp(hi),
+                                        int(iters), self._h)
+            if rc:
+                raise RuntimeError(f"orc_adj_step_f32 failed ({rc})")
+            x, v = x[: int(num[0])].copy(), v[: int(num[0])].copy()
+        return x, v
+
+    def num(self, step, live=True) -> int:
+        return int(lib().orc_tape_num(self._h, int(step), 1 if live else 0))
+
+    def _coef(self, coef):
+        if coef is None:
+            return None, None
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(2)
+        return c, _dp(c)
+
+    def replay(self, x0, v0, coef=None, record=False):
+        """FP64 map under the recorded decisions with coef = (XSPH c,
+        vorticity eps) in double (None: the scene's); returns the final (x, v)."""
+        x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1, 3)
+        v0 = np.ascontiguousarray(v0, dtype=np.float64).reshape(-1, 3)
+        nl = self.num(self.steps - 1)
+        xo = np.zeros((nl, 3), np.float64)
+        vo = np.zeros((nl, 3), np.float64)
+        op = self._op()
+        keep, cp = self._coef(coef)
+        rc = lib().orc_adj_replay_f64(ctypes.byref(op), self._bp(), self._h, _dp(x0), _dp(v0), cp, _dp(xo), _dp(vo),
+                                      1 if record else 0)
+        if rc:
+            raise RuntimeError(f"orc_adj_replay_f64 failed ({rc})")
+        return xo, vo
+
+    def backward(self, gx, gv, coef=None, with_exponent=False):
+        """FP64 adjoint: returns (dL/dx0, dL/dv0, dL/dc, dL/d vorticity eps),
+        each to be multiplied by 2**exponent; with_exponent appends the
+        exponent, otherwise the values are returned scaled (inf if beyond
+        double range)."""
+        gx = np.ascontiguousarray(gx, dtype=np.float64).reshape(-1, 3)
+        gv = np.ascontiguousarray(gv, dtype=np.float64).reshape(-1, 3)
+        n0 = self.num(0, live=False)
+        gx0 = np.zeros((n0, 3), np.float64)
+        gv0 = np.zeros((n0, 3), np.float64)
+        gp = np.zeros(2, np.float64)
+        op = self._op()
+        keep, cp = self._coef(coef)
+        ex = ctypes.c_int32(0)
+        rc = lib().orc_adj_backward_f64(ctypes.byref(op), self._bp(), self._h, cp, _dp(gx), _dp(gv), _dp(gx0),
+                                        _dp(gv0), _dp(gp), ctypes.byref(ex))
+        if rc:
+            raise RuntimeError(f"orc_adj_backward_f64 failed ({rc})")
+        if with_exponent:
+            return gx0, gv0, float(gp[0]), float(gp[1]), int(ex.value)
+        e = int(ex.value)
+        with np.errstate(over="ignore"):
+            return np.ldexp(gx0, e), np.ldexp(gv0, e), float(np.ldexp(gp[0], e)), float(np.ldexp(gp[1], e))
+
+    def identity(self):
+        nl = self.num(self.steps - 1)
+        out = np.zeros(nl, np.uint32)
+        if lib().orc_adj_identity(self._h, _up(out)):
+            raise RuntimeError("orc_adj_identity failed")
+        return out

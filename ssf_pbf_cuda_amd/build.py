@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libpbfx.so")
-SOURCES = ["pbfx_kernels.hip", "pbfx_api.hip"]
+SOURCES = ["pbfx_kernels.hip", "pbfx_adjoint.hip", "pbfx_api.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -27,6 +27,25 @@ struct SceneHost {
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 };
 
+// Differentiable form: checkpoints, the tape of one window of steps and the
+// adjoint work arrays.
+struct AdjState {
+    uint32_t max_steps = 0, every = 0, iters = 0, steps = 0;
+    uint32_t n0 = 0, n_final = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    std::vector<float*> ck_x, ck_v;   // state at steps 0, every, 2*every, ... (device AoS)
+    std::vector<uint32_t> ck_n;
+    uint32_t* id[2] = {nullptr, nullptr};
+    int id_cur = 0;
+    std::vector<StepTape> tape;       // one window
+    uint32_t tape_iters = 0;          // iterations the tapes' Q arrays hold
+    float* xw = nullptr;              // recompute state
+    float* vw = nullptr;
+    AdjBufs a{};
+    uint32_t* h_amax = nullptr;       // pinned
+    double* h_psum = nullptr;         // pinned [2]
+};
+
 struct pbfx_ctx {
     int device = 0;
     std::vector<SceneHost> sc;
@@ -55,6 +74,7 @@ struct pbfx_ctx {
     Timer* timer = nullptr;              // per-launch events when kernel timing is on
     double kt_ms[K_NUM] = {};
     uint32_t kt_n[K_NUM] = {};
+    AdjState* adj = nullptr;             // pbfx_adj_enable
 };
 
 namespace {
@@ -173,7 +193,42 @@ void free_boundary(SceneHost& h) {
     h.n_inner = h.n_outer = h.n_planes = 0;
 }
 
+void free_tape_q(AdjState* A) {
+    for (auto& t : A->tape)
+        if (t.Q) {
+            (void)hipFree(t.Q);
+            t.Q = nullptr;
+        }
+    A->tape_iters = 0;
+}
+
+void free_adj(pbfx_ctx* c) {
+    AdjState* A = c->adj;
+    if (!A) return;
+    free_tape_q(A);
+    std::vector<void*> ptrs;
+    for (float* p : A->ck_x) ptrs.push_back(p);
+    for (float* p : A->ck_v) ptrs.push_back(p);
+    for (auto& t : A->tape) {
+        void* tp[] = {t.sidx, t.X, t.nlist, t.ncnt, t.offset, t.PI, t.R, t.OM4};
+        for (void* p : tp) ptrs.push_back(p);
+    }
+    AdjBufs& a = A->a;
+    void* ap[] = {A->id[0], A->id[1], A->xw, A->vw, a.ax, a.av, a.axin, a.avin, a.aP, a.aQ, a.aV, a.aVc, a.aOm,
+                  a.etab, a.abar, a.coef, a.g2, a.aX, a.xnan, a.ckey, a.crank, a.ctmp, a.cperm, a.ccount,
+                  a.coffset, a.ctsum, a.ctflag, a.ctprefix, a.last, a.rcount, a.roff, a.rtsum, a.rtflag,
+                  a.rtprefix, a.rlist, a.part, a.psum, a.amax};
+    for (void* p : ap) ptrs.push_back(p);
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (A->h_amax) (void)hipHostFree(A->h_amax);
+    if (A->h_psum) (void)hipHostFree(A->h_psum);
+    delete A;
+    c->adj = nullptr;
+}
+
 void free_bufs(pbfx_ctx* c) {
+    free_adj(c);
     free_timer(c);
     Bufs& b = c->b;
     void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.Pf,     b.Q,     b.X,       b.R,
@@ -274,7 +329,7 @@ uint32_t unit_width(const pbfx_ctx* c) {
 }
 
 pbfx_status do_step(pbfx_ctx* c, float* x, float* v, uint32_t n, const uint32_t* in_off, int iters,
-                    bool default_sphere, hipStream_t s) {
+                    bool default_sphere, hipStream_t s, float4* qtape = nullptr) {
     if (n > c->capacity) return PBFX_CAPACITY;
     if (iters < 1 || (n > 0 && (!x || !v))) return PBFX_INVALID_ARG;
     for (const auto& h : c->sc)
@@ -284,7 +339,7 @@ pbfx_status do_step(pbfx_ctx* c, float* x, float* v, uint32_t n, const uint32_t*
     Bufs b = c->b;
     b.in_off = in_off;
     hipError_t e = launch_step(b, c->last_table[0], c->T, x, v, n, iters, any_flag(c, 0), any_flag(c, 1), unit_width(c),
-                               s, c->timer);
+                               s, c->timer, qtape);
     if (e != hipSuccess) return hip_fail(c, e);
     c->steps++;
     return PBFX_OK;
@@ -626,13 +681,13 @@ pbfx_status pbfx_set_boundary(pbfx_ctx* c, const float* inner4, uint32_t n_inner
 
 // The plain step on a single-scene context; lo/hi become the scene's area.
 static pbfx_status plain_step(pbfx_ctx* c, float* x, float* v, uint32_t num, const float lo[3], const float hi[3],
-                              int iters, bool default_sphere, hipStream_t s) {
+                              int iters, bool default_sphere, hipStream_t s, float4* qtape = nullptr) {
     if (c->batch || !lo || !hi) return PBFX_INVALID_ARG;
     for (int a = 0; a < 3; ++a) {
         c->sc[0].lo[a] = lo[a];
         c->sc[0].hi[a] = hi[a];
     }
-    return do_step(c, x, v, num, c->d_in_off, iters, default_sphere, s);
+    return do_step(c, x, v, num, c->d_in_off, iters, default_sphere, s, qtape);
 }
 
 pbfx_status pbfx_step_async(pbfx_ctx* c, float* x, float* v, uint32_t num, const float lo[3], const float hi[3],
@@ -942,6 +997,271 @@ pbfx_status pbfx_kernel_timing_read(pbfx_ctx* c, void* stream, pbfx_kernel_time*
         ++k;
     }
     *count = k;
+    return PBFX_OK;
+}
+
+// ---------------------------------------------------------- differentiable form
+
+static uint32_t pad_tiles(uint32_t n, uint32_t& lgT, uint32_t& nT) {
+    uint32_t N = n > 0 ? n : 1;
+    if (N <= kScanTile) {
+        uint32_t p2 = 1;
+        while (p2 < N) p2 <<= 1;
+        N = p2;
+    } else {
+        N = (N + kScanTile - 1) / kScanTile * kScanTile;
+    }
+    const uint32_t T = N < kScanTile ? N : kScanTile;
+    lgT = ilog2(T);
+    nT = N >> lgT;
+    return N;
+}
+
+pbfx_status pbfx_adj_enable(pbfx_ctx* c, uint32_t max_steps, uint32_t every) {
+    if (!c || c->batch || max_steps == 0 || every == 0) return PBFX_INVALID_ARG;
+    const Grid& G = c->sc[0].G;
+    if (G.gx < 4 || G.gy < 4 || G.gz < 4) return PBFX_NOT_SUPPORTED;
+    if (c->b.cap > kAdjMaxBlocks * 256u) return PBFX_CAPACITY;
+    DeviceGuard g(c->device);
+    (void)hipDeviceSynchronize();
+    free_adj(c);
+    AdjState* A = new (std::nothrow) AdjState();
+    if (!A) return PBFX_HIP_ERROR;
+    c->adj = A;
+    A->max_steps = max_steps;
+    A->every = every < max_steps ? every : max_steps;
+    const size_t cap = c->b.cap, CT = c->b.CT, nT = c->b.nT;
+    const uint32_t nck = (max_steps + A->every - 1) / A->every;
+    hipError_t e = hipSuccess;
+#define PBFX_ALLOC(ptr, cnt)                           \
+    if (e == hipSuccess) e = dalloc(&(ptr), (cnt));
+    A->ck_x.assign(nck, nullptr);
+    A->ck_v.assign(nck, nullptr);
+    A->ck_n.assign(nck, 0u);
+    for (uint32_t k = 0; k < nck; ++k) {
+        PBFX_ALLOC(A->ck_x[k], 3 * cap);
+        PBFX_ALLOC(A->ck_v[k], 3 * cap);
+    }
+    PBFX_ALLOC(A->id[0], cap);
+    PBFX_ALLOC(A->id[1], cap);
+    PBFX_ALLOC(A->xw, 3 * cap);
+    PBFX_ALLOC(A->vw, 3 * cap);
+    A->tape.assign(A->every, StepTape{});
+    for (auto& t : A->tape) {
+        PBFX_ALLOC(t.sidx, cap);
+        PBFX_ALLOC(t.X, 3 * cap);
+        PBFX_ALLOC(t.nlist, cap * c->list_stride);
+        PBFX_ALLOC(t.ncnt, cap);
+        PBFX_ALLOC(t.offset, CT + 2);
+        PBFX_ALLOC(t.PI, cap);
+        PBFX_ALLOC(t.R, 2 * cap);
+        PBFX_ALLOC(t.OM4, cap);
+    }
+    AdjBufs& a = A->a;
+    a.cap = (uint32_t)cap;
+    PBFX_ALLOC(a.ax, 3 * cap);
+    PBFX_ALLOC(a.av, 3 * cap);
+    PBFX_ALLOC(a.axin, 3 * cap);
+    PBFX_ALLOC(a.avin, 3 * cap);
+    PBFX_ALLOC(a.aP, cap);
+    PBFX_ALLOC(a.aQ, cap);
+    PBFX_ALLOC(a.aV, cap);
+    PBFX_ALLOC(a.aVc, cap);
+    PBFX_ALLOC(a.aOm, cap);
+    PBFX_ALLOC(a.etab, cap);
+    PBFX_ALLOC(a.abar, cap);
+    PBFX_ALLOC(a.coef, cap);
+    PBFX_ALLOC(a.g2, cap);
+    PBFX_ALLOC(a.aX, cap);
+    PBFX_ALLOC(a.xnan, cap);
+    PBFX_ALLOC(a.ckey, cap);
+    PBFX_ALLOC(a.crank, cap);
+    PBFX_ALLOC(a.ctmp, cap);
+    PBFX_ALLOC(a.cperm, cap);
+    PBFX_ALLOC(a.ccount, CT + 1);
+    PBFX_ALLOC(a.coffset, CT + 2);
+    PBFX_ALLOC(a.ctsum, nT + 1);
+    PBFX_ALLOC(a.ctflag, nT + 1);
+    PBFX_ALLOC(a.ctprefix, nT + 1);
+    PBFX_ALLOC(a.last, cap);
+    a.RT = pad_tiles((uint32_t)cap, a.lgRT, a.nRT);
+    PBFX_ALLOC(a.rcount, (size_t)a.RT + 1);
+    PBFX_ALLOC(a.roff, (size_t)a.RT + 2);
+    PBFX_ALLOC(a.rtsum, (size_t)a.nRT + 1);
+    PBFX_ALLOC(a.rtflag, (size_t)a.nRT + 1);
+    PBFX_ALLOC(a.rtprefix, (size_t)a.nRT + 1);
+    PBFX_ALLOC(a.rlist, cap * c->list_stride);
+    PBFX_ALLOC(a.part, 2 * (size_t)kAdjMaxBlocks);
+    PBFX_ALLOC(a.psum, 2);
+    PBFX_ALLOC(a.amax, 1);
+#undef PBFX_ALLOC
+    if (e == hipSuccess) e = hipMemset(a.ccount, 0, (CT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(a.ctflag, 0, (nT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(a.rcount, 0, ((size_t)a.RT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(a.rtflag, 0, ((size_t)a.nRT + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&A->h_amax, sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&A->h_psum, 2 * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        free_adj(c);
+        return hip_fail(c, e);
+    }
+    return PBFX_OK;
+}
+
+// Tapes hold (x*, s) of `iters` Jacobi iterations.
+static pbfx_status adj_tape_iters(pbfx_ctx* c, uint32_t iters) {
+    AdjState* A = c->adj;
+    if (A->tape_iters >= iters) return PBFX_OK;
+    free_tape_q(A);
+    for (auto& t : A->tape) {
+        hipError_t e = dalloc(&t.Q, (size_t)iters * c->b.cap);
+        if (e != hipSuccess) {
+            free_tape_q(A);
+            return hip_fail(c, e);
+        }
+    }
+    A->tape_iters = iters;
+    return PBFX_OK;
+}
+
+pbfx_status pbfx_adj_forward(pbfx_ctx* c, float* x, float* v, uint32_t* num, const float lo[3], const float hi[3],
+                             int iters, uint32_t steps, void* stream) {
+    if (!c || !c->adj || c->batch || !x || !v || !num || !lo || !hi || iters < 1) return PBFX_INVALID_ARG;
+    AdjState* A = c->adj;
+    if (steps == 0 || steps > A->max_steps || *num > c->capacity) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    const hipStream_t s = (hipStream_t)stream;
+    pbfx_status st = adj_tape_iters(c, (uint32_t)iters);
+    if (st != PBFX_OK) return st;
+    A->iters = (uint32_t)iters;
+    A->steps = 0;
+    for (int k = 0; k < 3; ++k) {
+        A->lo[k] = lo[k];
+        A->hi[k] = hi[k];
+    }
+    uint32_t n = *num;
+    A->n0 = n;
+    A->id_cur = 0;
+    hipError_t e = launch_adj_ident(nullptr, nullptr, A->id[0], n, s);
+    if (e != hipSuccess) return hip_fail(c, e);
+    for (uint32_t t = 0; t < steps; ++t) {
+        if (t % A->every == 0) {
+            const uint32_t k = t / A->every;
+            e = hipMemcpyAsync(A->ck_x[k], x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(A->ck_v[k], v, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) return hip_fail(c, e);
+            A->ck_n[k] = n;
+        }
+        st = plain_step(c, x, v, n, lo, hi, iters, false, s);
+        if (st == PBFX_OK) st = sync_step_result(c, &n, s);
+        if (st != PBFX_OK) return st;
+        e = launch_adj_ident(A->id[A->id_cur], c->b.sidx, A->id[1 - A->id_cur], n, s);
+        if (e != hipSuccess) return hip_fail(c, e);
+        A->id_cur ^= 1;
+        A->steps = t + 1;
+    }
+    A->n_final = n;
+    *num = n;
+    return PBFX_OK;
+}
+
+pbfx_status pbfx_adj_identity(pbfx_ctx* c, uint32_t* id, void* stream) {
+    if (!c || !c->adj || !id || c->adj->steps == 0) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    AdjState* A = c->adj;
+    hipError_t e = hipMemcpyAsync(id, A->id[A->id_cur], (size_t)A->n_final * sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    return e == hipSuccess ? PBFX_OK : hip_fail(c, e);
+}
+
+// Runs one step from (xw, vw, *n) recording its tape.
+static pbfx_status adj_record_step(pbfx_ctx* c, StepTape& t, uint32_t* n, hipStream_t s) {
+    AdjState* A = c->adj;
+    const Bufs& b = c->b;
+    t.n_in = *n;
+    pbfx_status st = plain_step(c, A->xw, A->vw, *n, A->lo, A->hi, (int)A->iters, false, s, t.Q);
+    if (st == PBFX_OK) st = sync_step_result(c, n, s);
+    if (st != PBFX_OK) return st;
+    t.n_live = *n;
+    const size_t cap = b.cap;
+    struct Cp {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    } cps[] = {{t.sidx, b.sidx, cap * sizeof(uint32_t)},
+               {t.X, b.X, 3 * cap * sizeof(float)},
+               {t.nlist, b.nlist, cap * c->list_stride * sizeof(uint32_t)},
+               {t.ncnt, b.ncnt, cap * sizeof(uint32_t)},
+               {t.offset, b.offset, ((size_t)b.CT + 2) * sizeof(uint32_t)},
+               {t.PI, b.Pf, cap * sizeof(float4)},
+               {t.R, b.R, 2 * cap * sizeof(float4)},
+               {t.OM4, b.OM4, cap * sizeof(float4)}};
+    for (const Cp& q : cps) {
+        hipError_t e = hipMemcpyAsync(q.dst, q.src, q.bytes, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(c, e);
+    }
+    return PBFX_OK;
+}
+
+pbfx_status pbfx_adj_backward(pbfx_ctx* c, const float* gx, const float* gv, float* gx0, float* gv0,
+                              double gparam[2], int32_t* exponent, void* stream) {
+    if (!c || !c->adj || !gx || !gv || !gx0 || !gv0 || !gparam || !exponent) return PBFX_INVALID_ARG;
+    AdjState* A = c->adj;
+    if (A->steps == 0) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    const hipStream_t s = (hipStream_t)stream;
+    AdjBufs a = A->a;
+    hipError_t e = hipMemcpyAsync(a.ax, gx, (size_t)A->n_final * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(a.av, gv, (size_t)A->n_final * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return hip_fail(c, e);
+    double gp[2] = {0.0, 0.0};
+    int32_t E = 0;
+    const uint32_t nwin = (A->steps + A->every - 1) / A->every;
+    for (uint32_t w = nwin; w-- > 0;) {
+        const uint32_t t0 = w * A->every;
+        const uint32_t t1 = (t0 + A->every < A->steps) ? t0 + A->every : A->steps;
+        uint32_t n = A->ck_n[w];
+        e = hipMemcpyAsync(A->xw, A->ck_x[w], (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(A->vw, A->ck_v[w], (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(c, e);
+        for (uint32_t t = t0; t < t1; ++t) {
+            pbfx_status st = adj_record_step(c, A->tape[t - t0], &n, s);
+            if (st != PBFX_OK) return st;
+        }
+        for (uint32_t t = t1; t-- > t0;) {
+            const StepTape& T = A->tape[t - t0];
+            e = launch_adj_step(c->b, c->last_table[0], T, a, (int)A->iters, s);
+            if (e == hipSuccess) e = launch_adj_absmax(a, T.n_in, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(A->h_psum, a.psum, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(A->h_amax, a.amax, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return hip_fail(c, e);
+            gp[0] += A->h_psum[0];
+            gp[1] += A->h_psum[1];
+            float m;
+            std::memcpy(&m, A->h_amax, sizeof(m));
+            // keep the adjoint near 1 once it grows past 2^20: scale by 2^-k
+            if (std::isfinite(m) && m > 1048576.f) {
+                const int k = std::ilogb(m);
+                const float f = std::ldexp(1.f, -k);
+                e = launch_adj_scale(a.axin, a.avin, 3 * T.n_in, f, s);
+                if (e != hipSuccess) return hip_fail(c, e);
+                gp[0] = std::ldexp(gp[0], -k);
+                gp[1] = std::ldexp(gp[1], -k);
+                E += k;
+            }
+            std::swap(a.ax, a.axin);
+            std::swap(a.av, a.avin);
+        }
+    }
+    e = hipMemcpyAsync(gx0, a.ax, (size_t)A->n0 * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(gv0, a.av, (size_t)A->n0 * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e);
+    gparam[0] = gp[0];
+    gparam[1] = gp[1];
+    *exponent = E;
     return PBFX_OK;
 }
 

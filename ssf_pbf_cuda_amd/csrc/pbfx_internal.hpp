@@ -180,13 +180,67 @@ struct Tuning {
     uint32_t grid[K_NUM];   // persistent workgroups per unit kernel, from its own occupancy
 };
 
+// ---------------------------------------------------------- differentiable form
+// What the backward pass needs of one recorded step (device arrays of a
+// single-scene context, capacity cap).
+struct StepTape {
+    uint32_t n_in, n_live;
+    uint32_t* sidx;     // [cap]   slot -> input index
+    float* X;           // [3*cap] sorted input positions, SoA with stride cap
+    uint32_t* nlist;    // [stride*cap] frozen lists, column-major
+    uint32_t* ncnt;     // [cap]
+    uint32_t* offset;   // [CT+2]  cell ranges of the search
+    float4* Q;          // [iters*cap] (x*, s) entering each Jacobi iteration
+    float4* PI;         // [cap]   final positions
+    float4* R;          // [2*cap] (x*, 0), (velocity after confinement, 0) when the final passes ran
+    float4* OM4;        // [cap]   (omega, |omega|) when vorticity ran
+};
+
+// Work arrays of the adjoint of one step.  Adjoints are FP32 and carry a
+// common power-of-two scale kept by the host.
+struct AdjBufs {
+    uint32_t cap;
+    float* ax;          // [3*cap] dL/d(step output x), output order
+    float* av;          // [3*cap] dL/d(step output v)
+    float* axin;        // [3*cap] dL/d(step input x), input order
+    float* avin;        // [3*cap] dL/d(step input v)
+    float4* aP;         // dL/d positions of the current stage
+    float4* aQ;         // dL/d positions, partial (pre-collision path + first pass)
+    float4* aV;         // dL/d velocity before confinement
+    float4* aVc;        // dL/d velocity after confinement
+    float4* aOm;        // dL/d vorticity
+    float4* etab;       // dL/d confinement location vector
+    float4* abar;       // dL/d pre-collision positions
+    float4* coef;       // (dL/d sum W, 2 dL/dG * sum kg) per particle
+    float* g2;          // 2 dL/dG per particle
+    float4* aX;         // dL/d sorted input positions
+    uint32_t* xnan;     // XSPH NaN guard per slot
+    // particles grouped by final cell: the transposed 27-cell sets
+    uint32_t *ckey, *crank, *ctmp, *cperm, *ccount, *coffset, *ctsum, *ctflag, *ctprefix;
+    // reverse lists: the particles that list each slot
+    uint2* last;        // (27-cell index, slot) of a full list's last entry; (255, 0) if not full
+    uint32_t *rcount, *roff, *rtsum, *rtflag, *rtprefix, *rlist;
+    uint32_t RT, lgRT, nRT;   // padded slot space of the reverse-count scan
+    double* part;       // [2 * kAdjMaxBlocks] per-workgroup partial sums of dL/dc, dL/d eps
+    double* psum;       // [2]
+    uint32_t* amax;     // [1] largest |adjoint| of the step input (float bits)
+};
+constexpr uint32_t kAdjMaxBlocks = 4096;
+
 // Launchers (pbfx_kernels.hip).  All enqueue on `s` and never synchronise.
 // n_in: particles entering (all scenes); uw: cells per work unit along x,
 // a power of two no wider than any scene's grid; any_xsph / any_vort: some
 // scene runs XSPH / vorticity.
 // s0: the scene (a single-scene context passes it by value to the kernels).
+// qtape (may be NULL, single scene only): (x*, s) of Jacobi iteration k is
+// written to qtape + k*cap instead of b.Q.
 hipError_t launch_step(const Bufs& b, const Scene& s0, const Tuning& T, float* x, float* v, uint32_t n_in, int iters,
-                       int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* timer);
+                       int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* timer,
+                       float4* qtape = nullptr);
+// Exclusive scan of count[0..C] (C a multiple of 2^lgT, nT = C >> lgT tiles,
+// tflag marking the occupied tiles) into offset[0..C+1]; clears count and tflag.
+hipError_t launch_count_scan(uint32_t* count, uint32_t* offset, uint32_t* tsum, uint32_t* tflag, uint32_t* tprefix,
+                             uint32_t nT, uint32_t lgT, uint32_t C, hipStream_t s);
 // out[s] = first output slot of scene s, out[nscenes] = live total (device).
 hipError_t launch_scene_offsets(const Bufs& b, uint32_t* out, hipStream_t s);
 const char* kernel_name(int id);
@@ -198,5 +252,17 @@ hipError_t launch_density(const Bufs& b, const Scene& s0, const float* x, float*
 hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float* xo, float* vo, hipStream_t s);
 // Fills T.grid[] (and T.grid_units) for num_cu compute units.
 void size_unit_grids(Tuning& T, int num_cu);
+
+// Adjoint launchers (pbfx_adjoint.hip).  launch_adj_step: dL/d(step input)
+// into a.axin / a.avin from a.ax / a.av, and the step's dL/dc, dL/d eps into
+// a.psum, for the recorded step t of the single scene s0.
+hipError_t launch_adj_step(const Bufs& b, const Scene& s0, const StepTape& t, const AdjBufs& a, int iters,
+                           hipStream_t s);
+// a.amax = max |a.axin|, |a.avin| over n rows (as float bits).
+hipError_t launch_adj_absmax(const AdjBufs& a, uint32_t n, hipStream_t s);
+// x[0..n) *= f for each of the arrays (may be NULL).
+hipError_t launch_adj_scale(float* x, float* y, uint32_t n, float f, hipStream_t s);
+// id_out[q] = id_in[sidx[q]] for q < n; id_in NULL: the identity.
+hipError_t launch_adj_ident(const uint32_t* id_in, const uint32_t* sidx, uint32_t* id_out, uint32_t n, hipStream_t s);
 
 }  // namespace pbfx

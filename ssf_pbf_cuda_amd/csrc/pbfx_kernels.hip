@@ -17,369 +17,12 @@
 // operation order with IEEE division/sqrt and explicit fmaf (the library is
 // compiled with -ffp-contract=off), so results are reproducible bit-for-bit
 // on the host.
-#include "pbfx_internal.hpp"
+#include "pbfx_device.hpp"
 
 #include <climits>
 
 namespace pbfx {
 namespace {
-
-// ---------------------------------------------------------------- vector math
-struct F3 {
-    float x, y, z;
-};
-__device__ __forceinline__ F3 mk3(float x, float y, float z) {
-    F3 r;
-    r.x = x;
-    r.y = y;
-    r.z = z;
-    return r;
-}
-__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ void acc(F3& a, F3 b) {
-    a.x += b.x;
-    a.y += b.y;
-    a.z += b.z;
-}
-__device__ __forceinline__ F3 mulvs(F3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ F3 mulsv(float s, F3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ F3 divvs(F3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
-// GLM 0.9.7.1 dot/length/cross (external/include/glm/detail/func_geometric.inl:65-72,95-100,134-142)
-__device__ __forceinline__ float glm_dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ float glm_length(F3 a) { return __builtin_sqrtf(glm_dot(a, a)); }
-__device__ __forceinline__ F3 glm_cross(F3 x, F3 y) {
-    return mk3(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y);
-}
-// the hand-fused dot of the solver kernels: fmaf(a.z,b.z, fmaf(a.y,b.y, a.x*b.x))
-__device__ __forceinline__ float fdot(F3 a, F3 b) { return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x)); }
-
-__device__ __forceinline__ F3 ld3(const float* __restrict__ a, uint32_t cap, uint32_t i) {
-    return mk3(a[i], a[cap + i], a[2 * cap + i]);
-}
-__device__ __forceinline__ void st3(float* __restrict__ a, uint32_t cap, uint32_t i, F3 v) {
-    a[i] = v.x;
-    a[cap + i] = v.y;
-    a[2 * cap + i] = v.z;
-}
-
-// pbf/kernel/cuda/detail/pbf_kernel.inl:34-48
-__device__ __forceinline__ float wpoly6(float r, float inv_h) {
-    float q = r * inv_h;
-    float alpha = 1.56668147065f * inv_h * inv_h * inv_h;
-    if (q <= 1.f) {
-        float q2 = q * q;
-        float t = 1.f - q2;
-        return alpha * t * t * t;
-    }
-    return 0.f;
-}
-
-// pbf/kernel/cuda/detail/pbf_kernel.inl:50-63
-__device__ __forceinline__ float wspiky_deriv(float r, float inv_h) {
-    float q = r * inv_h;
-    float inv_h2 = inv_h * inv_h;
-    if (q <= 1.f) {
-        float t = 1.f - q;
-        float inv_h4 = inv_h2 * inv_h2;
-        return -14.3239448745f * inv_h4 * t * t;
-    }
-    return 0.f;
-}
-
-// float -> int with a defined result everywhere (NaN -> 0, saturation), the
-// behaviour of the device conversion the reference relies on.
-__device__ __forceinline__ int f2i(float f) {
-    if (!(f == f)) return 0;
-    if (f >= 2147483648.0f) return INT_MAX;
-    if (f <= -2147483648.0f) return INT_MIN;
-    return (int)f;
-}
-
-// pbf/interaction/cuda/pbf_grid.h:20-28 (IEEE division) and :54-61 (wrap)
-__device__ __forceinline__ void grid_pos(F3 p, float cw, int g[3]) {
-    g[0] = f2i(floorf(p.x / cw));
-    g[1] = f2i(floorf(p.y / cw));
-    g[2] = f2i(floorf(p.z / cw));
-}
-__device__ __forceinline__ uint32_t grid_hash(int gx, int gy, int gz, const Grid& G) {
-    uint32_t x = (uint32_t)gx & (G.gx - 1u);
-    uint32_t y = (uint32_t)gy & (G.gy - 1u);
-    uint32_t z = (uint32_t)gz & (G.gz - 1u);
-    return z * G.gx * G.gy + y * G.gx + x;
-}
-
-// cell key of a scene's grid position (the scene's range of the key space)
-__device__ __forceinline__ uint32_t scene_key(int gx, int gy, int gz, const Grid& G) {
-    return G.base + grid_hash(gx, gy, gz, G);
-}
-
-// Largest s < n with off[s] <= i (the scene owning index or key i).
-__device__ __forceinline__ uint32_t find_scene(const uint32_t* __restrict__ off, uint32_t n, uint32_t i) {
-    uint32_t lo = 0, hi = n - 1u;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1u) >> 1;
-        if (off[mid] <= i) lo = mid;
-        else hi = mid - 1u;
-    }
-    return lo;
-}
-
-// The scenes of a launch: a single scene travels as a kernel argument;
-// several are read from the device table.
-template <bool kB>
-struct Scenes {
-    const Scene* table;
-    Scene s0;
-    __device__ __forceinline__ const Scene& operator[](uint32_t s) const { return kB ? table[s] : s0; }
-};
-
-template <bool kB>
-__device__ __forceinline__ uint32_t scene_index(const uint32_t* __restrict__ off, uint32_t n, uint32_t i) {
-    return kB ? find_scene(off, n, i) : 0u;
-}
-
-// Runs f(s) for the active lanes, one scene at a time: s is uniform over the
-// lanes that call f (read from the first pending lane), so the scene's
-// constants come through scalar loads.  f must not use wave-wide operations.
-template <bool kB, class F>
-__device__ __forceinline__ void by_scene(uint32_t sid, bool active, F&& f) {
-    if (!kB) {
-        if (active) f(0u);
-        return;
-    }
-    bool todo = active;
-    while (__any(todo)) {
-        if (todo) {
-            const uint32_t s = __builtin_amdgcn_readfirstlane(sid);
-            if (sid == s) {
-                f(s);
-                todo = false;
-            }
-        }
-    }
-}
-
-// t^n by binary exponentiation; n = 4 evaluates (t*t)*(t*t)
-__device__ __forceinline__ float powi(float t, int n) {
-    float result = 1.f, base = t;
-    bool first = true;
-    while (n > 0) {
-        if (n & 1) {
-            result = first ? base : result * base;
-            first = false;
-        }
-        n >>= 1;
-        if (n) base = base * base;
-    }
-    return result;
-}
-
-// pbf/solver/cuda/pbf_kernel_memorization.cu:15-56 (memorizeKernelPair):
-// W and grad W of one listed pair (rsqrtf restated as 1/sqrtf).
-__device__ __forceinline__ void pair_terms(F3 self, F3 pj, const Phys& P, float& W, F3& kg) {
-    F3 d = self - pj;
-    float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
-    float inv_r = 1.f / __builtin_sqrtf(r2);
-    float q2 = r2 * P.inv_h * P.inv_h;
-    W = 0.f;
-    if (q2 <= 1.f) {
-        float t = 1.f - q2;
-        W = P.a * t * t * t;
-    }
-    float r = r2 * inv_r;
-    float u = __builtin_fmaf(-r, P.inv_h, 1.f);
-    float kd = 0.f;
-    if (P.inv_h <= inv_r) kd = P.ad * u * u;
-    kg = mk3(0.f, 0.f, 0.f);
-    if (r2 > 0.f) kg = mulsv(kd, mulvs(d, inv_r));
-}
-
-// pbf/solver/cuda/boundary/pbf_plane_boundary.cu:6-45
-__device__ __forceinline__ void response_plane(bool& hit, F3& dp, F3 old_p, F3 pred, F3 Pp, F3 n) {
-    F3 pa = old_p - Pp;
-    F3 pb = pred - Pp;
-    float pa_n = fdot(pa, n);
-    float pb_n = fdot(pb, n);
-    F3 I;
-    if (pa_n == 0.f && pb_n == 0.f) {
-        return;
-    } else if (pa_n >= 0.f && pb_n < 0.f) {
-        hit = true;
-        float frac = fabsf(pa_n) / (fabsf(pa_n) + fabsf(pb_n));
-        I = old_p + mulvs(pred - old_p, frac);
-    } else if (pa_n < 0.f && pb_n < 0.f) {
-        hit = true;
-        float d = -fdot(Pp, n);
-        float t = -(fdot(pred, n) + d) / fdot(n, n);
-        I = pred + mulsv(t, n);
-    } else {
-        return;
-    }
-    F3 di = pred - I;
-    float c = fdot(di, n);
-    acc(dp, mulsv(-c, n));
-}
-
-// pbf/solver/cuda/boundary/pbf_sphere_boundary.cu:8-34 (inner), :36-64 (outer)
-__device__ __forceinline__ void response_sphere(bool& hit, F3& dp, F3 pred, F3 center, float radius, bool inner) {
-    F3 cd = center - pred;
-    float d2 = fdot(cd, cd);
-    float rinv = 1.f / __builtin_sqrtf(d2);
-    bool coll = inner ? (d2 > radius * radius) : (d2 < radius * radius);
-    if (!coll) return;
-    hit = true;
-    float nudge = inner ? 1.0001f : 0.9999f;
-    dp.x = __builtin_fmaf(cd.x, nudge, dp.x);
-    dp.y = __builtin_fmaf(cd.y, nudge, dp.y);
-    dp.z = __builtin_fmaf(cd.z, nudge, dp.z);
-    F3 tc = mulvs(cd, rinv);
-    dp.x = __builtin_fmaf(-radius, tc.x, dp.x);
-    dp.y = __builtin_fmaf(-radius, tc.y, dp.y);
-    dp.z = __builtin_fmaf(-radius, tc.z, dp.z);
-}
-
-// pbf/solver/cuda/pbf_response_collision.cu:12-66: sweeps until none collides
-// (capped).  Returns the sweeps used; `capped` when the cap stopped it.
-__device__ __forceinline__ int collide(const Phys& P, const Boundary& B, F3& dp, F3 pk, bool& capped) {
-    const F3 pred_x = pk, old_x = pk;
-    int sweeps = 0;
-    capped = false;
-    while (true) {
-        if (sweeps >= P.max_sweeps) {
-            capped = true;
-            break;
-        }
-        ++sweeps;
-        F3 pp = pred_x + dp;
-        bool hit = false;
-        for (uint32_t i = 0; i < B.n_inner; ++i) {
-            float4 s = B.inner[i];
-            response_sphere(hit, dp, pp, mk3(s.x, s.y, s.z), s.w, true);
-            pp = pred_x + dp;
-        }
-        for (uint32_t i = 0; i < B.n_outer; ++i) {
-            float4 s = B.outer[i];
-            response_sphere(hit, dp, pp, mk3(s.x, s.y, s.z), s.w, P.outer_as_inner != 0);
-            pp = pred_x + dp;
-        }
-        for (uint32_t i = 0; i < B.n_planes; ++i) {
-            F3 Pp = mk3(B.pts[3 * i], B.pts[3 * i + 1], B.pts[3 * i + 2]);
-            F3 n = mk3(B.nrm[3 * i], B.nrm[3 * i + 1], B.nrm[3 * i + 2]);
-            response_plane(hit, dp, old_x, pp, Pp, n);
-            pp = pred_x + dp;
-        }
-        if (!hit) break;
-    }
-    return sweeps;
-}
-
-// The same for a boundary of exactly NP planes and no spheres (the box): the
-// planes are loaded once, and the per-plane terms that do not change between
-// sweeps -- (old_x - P).n, -(P.n) and n.n -- are computed once; each has the
-// value the general form recomputes.
-template <int NP>
-__device__ __forceinline__ int collide_planes(const Phys& P, const Boundary& B, F3& dp, F3 pk, bool& capped) {
-    F3 Pp[NP], n[NP];
-    float pa_n[NP], dd[NP], nn[NP];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-        Pp[i] = mk3(B.pts[3 * i], B.pts[3 * i + 1], B.pts[3 * i + 2]);
-        n[i] = mk3(B.nrm[3 * i], B.nrm[3 * i + 1], B.nrm[3 * i + 2]);
-        pa_n[i] = fdot(pk - Pp[i], n[i]);
-        dd[i] = -fdot(Pp[i], n[i]);
-        nn[i] = fdot(n[i], n[i]);
-    }
-    int sweeps = 0;
-    capped = false;
-    while (true) {
-        if (sweeps >= P.max_sweeps) {
-            capped = true;
-            break;
-        }
-        ++sweeps;
-        bool hit = false;
-#pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            // response_plane with its constant terms hoisted
-            const F3 pred = pk + dp;
-            const float pb_n = fdot(pred - Pp[i], n[i]);
-            F3 I;
-            if (pa_n[i] == 0.f && pb_n == 0.f) {
-                continue;
-            } else if (pa_n[i] >= 0.f && pb_n < 0.f) {
-                const float frac = fabsf(pa_n[i]) / (fabsf(pa_n[i]) + fabsf(pb_n));
-                I = pk + mulvs(pred - pk, frac);
-            } else if (pa_n[i] < 0.f && pb_n < 0.f) {
-                const float t = -(fdot(pred, n[i]) + dd[i]) / nn[i];
-                I = pred + mulsv(t, n[i]);
-            } else {
-                continue;
-            }
-            hit = true;
-            const F3 di = pred - I;
-            const float c = fdot(di, n[i]);
-            acc(dp, mulsv(-c, n[i]));
-        }
-        if (!hit) break;
-    }
-    return sweeps;
-}
-
-// ------------------------------------------------------------ block helpers
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-
-// Exclusive scan over the block (kBlock threads); `total` = block sum.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
-    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    uint32_t off = 0, tot = 0;
-    const uint32_t nw = blockDim.x >> 6;
-    for (uint32_t w = 0; w < nw; ++w) {
-        uint32_t s = wsum[w];
-        if (w < wid) off += s;
-        tot += s;
-    }
-    __syncthreads();
-    total = tot;
-    return off + x - v;
-}
-
-// atomicAdd(&ctr[k], 1) per active lane, returning a unique rank per lane
-// (ranks among equal keys are arbitrary; the reorder makes them stable).
-// Runs of equal keys in consecutive lanes share one atomic, issued by the
-// run's first lane; all runs' atomics are in flight together.  The active
-// lanes must be a prefix of the wave.
-__device__ __forceinline__ uint32_t wave_rank(uint32_t* ctr, uint32_t k, bool active) {
-    const uint32_t lane = lane_id();
-    const uint32_t nact = (uint32_t)__popcll(__ballot(active));
-    const uint32_t kp = (uint32_t)__shfl_up((int)k, 1, 64);
-    const bool head = active && (lane == 0 || kp != k);
-    const uint64_t heads = __ballot(head);
-    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    uint32_t base = 0;
-    if (head) {
-        const uint64_t above = heads & ~upto;
-        const uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : nact;
-        base = atomicAdd(&ctr[k], end - lane);
-    }
-    const int hl = 63 - __clzll((unsigned long long)(heads & upto));
-    base = (uint32_t)__shfl((int)base, hl < 0 ? 0 : hl, 64);
-    return base + (lane - (uint32_t)(hl < 0 ? 0 : hl));
-}
-
-// Marks a scan tile as occupied: a read first, so that the common case (the
-// tile is already marked) issues no store.
-__device__ __forceinline__ void mark_tile(uint32_t* __restrict__ tflag, uint32_t t) {
-    if (tflag[t] == 0u) tflag[t] = 1u;
-}
 
 // ------------------------------------------------- K1: predict + hash + histogram
 // pbf_predict.cu:15-45 (v* = fmaf(g,dt,v), x* = fmaf(v*,dt,x)), calcHashCUDA
@@ -749,17 +392,6 @@ __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, 
         else hi = mid - 1u;
     }
     return lo - base;
-}
-
-__device__ __forceinline__ F3 f3of(float4 f) { return mk3(f.x, f.y, f.z); }
-
-// Block -> index range, contiguous per XCD (workgroup b runs on XCD b % 8),
-// so that neighbouring slots share an L2.
-__device__ __forceinline__ uint32_t xcd_block() {
-    const uint32_t nb = gridDim.x, b = blockIdx.x, nx = 8;
-    if (nb < nx) return b;
-    const uint32_t per = nb / nx, rem = nb % nx, x = b % nx;
-    return x * per + min(x, rem) + b / nx;
 }
 
 // ------------------------------------------------- K6: neighbour lists
@@ -1545,7 +1177,7 @@ struct Mark {
 namespace {
 template <bool kB>
 hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float* x, float* v, uint32_t n, int iters,
-                    int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* tm) {
+                    int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* tm, float4* qtape) {
     const uint32_t nb = ceil_div(n > 0 ? n : 1, 256);
     const uint32_t CT = b.CT;
     const uint32_t budget = (T.tile_cap * 3u) / 4u > 0 ? (T.tile_cap * 3u) / 4u : 1u;
@@ -1580,12 +1212,14 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
         hipLaunchKernelGGL(k_ns<kB>, grid(K_NS), dim3(kBlock), smem, s, b, SC, T.tile_cap);
     }
     for (int it = 0; it < iters; ++it) {
+        Bufs bi = b;
+        if (qtape) bi.Q = qtape + (size_t)it * b.cap;
         {
             Mark m(tm, s, K_LAMBDA);
-            hipLaunchKernelGGL(k_lambda<kB>, dim3(nb), dim3(256), 0, s, b, SC);
+            hipLaunchKernelGGL(k_lambda<kB>, dim3(nb), dim3(256), 0, s, bi, SC);
         }
         Mark m(tm, s, K_DELTAP);
-        hipLaunchKernelGGL(k_deltap<kB>, dim3(nb), dim3(256), 0, s, b, SC);
+        hipLaunchKernelGGL(k_deltap<kB>, dim3(nb), dim3(256), 0, s, bi, SC);
     }
     if (!any_vort && !any_xsph) {
         Mark m(tm, s, K_OUT);
@@ -1636,13 +1270,22 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
 }  // namespace
 
 hipError_t launch_step(const Bufs& b, const Scene& s0, const Tuning& T, float* x, float* v, uint32_t n, int iters,
-                       int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* tm) {
+                       int any_xsph, int any_vort, uint32_t uw, hipStream_t s, Timer* tm, float4* qtape) {
     if (b.nscenes == 1) {
         const Scenes<false> SC{b.scenes, s0};
-        return step_seq<false>(b, SC, T, x, v, n, iters, any_xsph, any_vort, uw, s, tm);
+        return step_seq<false>(b, SC, T, x, v, n, iters, any_xsph, any_vort, uw, s, tm, qtape);
     }
+    if (qtape) return hipErrorInvalidValue;
     const Scenes<true> SC{b.scenes, s0};
-    return step_seq<true>(b, SC, T, x, v, n, iters, any_xsph, any_vort, uw, s, tm);
+    return step_seq<true>(b, SC, T, x, v, n, iters, any_xsph, any_vort, uw, s, tm, nullptr);
+}
+
+hipError_t launch_count_scan(uint32_t* count, uint32_t* offset, uint32_t* tsum, uint32_t* tflag, uint32_t* tprefix,
+                             uint32_t nT, uint32_t lgT, uint32_t C, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_reduce, dim3(nT + 1), dim3(256), 0, s, count, tflag, tsum, nT, lgT, C);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tsum, tprefix, nT + 1, (uint32_t*)nullptr, 0u);
+    hipLaunchKernelGGL(k_scan_cells, dim3(nT + 1), dim3(256), 0, s, count, offset, tsum, tflag, tprefix, nT, lgT, C);
+    return hipGetLastError();
 }
 
 hipError_t launch_scene_offsets(const Bufs& b, uint32_t* out, hipStream_t s) {

@@ -147,6 +147,11 @@ def lib():
             "pbfx_selftest_arith": ([vp, vp, vp, u32, vp], ctypes.c_int),
             "pbfx_set_kernel_timing": ([vp, ctypes.c_int], ctypes.c_int),
             "pbfx_kernel_timing_read": ([vp, vp, ctypes.POINTER(KernelTime), u32, u32p], ctypes.c_int),
+            "pbfx_adj_enable": ([vp, u32, u32], ctypes.c_int),
+            "pbfx_adj_forward": ([vp, vp, vp, u32p, f32p, f32p, ctypes.c_int, u32, vp], ctypes.c_int),
+            "pbfx_adj_identity": ([vp, vp, vp], ctypes.c_int),
+            "pbfx_adj_backward": ([vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_int32), vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -335,6 +340,32 @@ class Solver:
         """x_out[sorted_index[s]] = x_sorted[s] (and v) for the last step's live slots."""
         _check(self._lib.pbfx_restore_order(self._ctx, _ptr(x_sorted), _ptr(v_sorted), _ptr(x_out), _ptr(v_out),
                                             _stream(stream)), "pbfx_restore_order")
+
+    # ---- differentiable form
+    def adj_enable(self, max_steps: int, every: int) -> None:
+        """Checkpoints every `every` steps, up to max_steps steps."""
+        _check(self._lib.pbfx_adj_enable(self._ctx, int(max_steps), int(every)), "pbfx_adj_enable")
+
+    def adj_forward(self, x, v, num: int, lo, hi, iters: int, steps: int, stream=None) -> int:
+        """`steps` steps from (x, v) in place, kept for the backward pass; returns the live count."""
+        n = ctypes.c_uint32(num)
+        _check(self._lib.pbfx_adj_forward(self._ctx, _ptr(x), _ptr(v), ctypes.byref(n), _f3(lo), _f3(hi), int(iters),
+                                          int(steps), _stream(stream)), "pbfx_adj_forward")
+        return int(n.value)
+
+    def adj_identity(self, out, stream=None) -> None:
+        """out[s] = initial index of the particle in final slot s (device uint32)."""
+        _check(self._lib.pbfx_adj_identity(self._ctx, _ptr(out), _stream(stream)), "pbfx_adj_identity")
+
+    def adj_backward(self, gx, gv, gx0, gv0, stream=None):
+        """dL/d(final x, v) -> dL/d(initial x, v) into gx0, gv0; returns
+        (dL/dc, dL/d vorticity eps, exponent): every gradient is the value
+        returned times 2**exponent."""
+        gp = (ctypes.c_double * 2)()
+        ex = ctypes.c_int32(0)
+        _check(self._lib.pbfx_adj_backward(self._ctx, _ptr(gx), _ptr(gv), _ptr(gx0), _ptr(gv0), gp, ctypes.byref(ex),
+                                           _stream(stream)), "pbfx_adj_backward")
+        return float(gp[0]), float(gp[1]), int(ex.value)
 
 
 def selftest_arith(a, b, out, stream=None) -> None:

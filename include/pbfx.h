@@ -320,15 +320,15 @@ pbfx_status pbfx_adj_forward(pbfx_ctx* ctx, float* x, float* v, uint32_t* num, c
  * (device uint32 [num after the forward]). */
 pbfx_status pbfx_adj_identity(pbfx_ctx* ctx, uint32_t* id, void* stream);
 
-/* Backward pass: gx, gv = dL/d(final x, v) (device AoS float3, final order);
- * gx0, gv0 = dL/d(initial x, v) (device AoS float3, initial order);
+/* Backward pass: gx, gv = dL/d(final x, v) (device AoS double3, final
+ * order); gx0, gv0 = dL/d(initial x, v) (device AoS double3, initial order);
  * gparam[0] = dL/dc, gparam[1] = dL/d(vorticity epsilon) (host).  Every
  * window of steps is recomputed from its checkpoint with its tape recorded,
- * then the adjoint kernels run through it in reverse.  The adjoint carries a
- * power-of-two scale: the true gradients are the returned values times
- * 2^(*exponent) (0 unless they would leave the FP32 range).  gx and gv are
- * not modified. */
-pbfx_status pbfx_adj_backward(pbfx_ctx* ctx, const float* gx, const float* gv, float* gx0, float* gv0,
+ * then the adjoint kernels run through it in reverse, in FP64 at the FP32
+ * forward's values.  The adjoint carries a power-of-two scale: the true
+ * gradients are the returned values times 2^(*exponent) (0 while the
+ * adjoint stays below 2^20).  gx and gv are not modified. */
+pbfx_status pbfx_adj_backward(pbfx_ctx* ctx, const double* gx, const double* gv, double* gx0, double* gv0,
                               double gparam[2], int32_t* exponent, void* stream);
 
 /* Arithmetic self-test on the device: for n inputs computes

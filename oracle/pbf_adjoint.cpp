@@ -70,6 +70,7 @@ struct StepRec {
     std::vector<double> P;           // [(iters+1)*3*nl] positions entering each iteration, then the final ones
     std::vector<double> S;           // [iters*nl] scaling factor of each iteration
     std::vector<double> V, Om, Vc;   // [3*nl] velocity, vorticity, velocity after confinement
+    bool f32 = false;                // the values are the FP32 run's
 };
 
 template <class R>
@@ -116,16 +117,72 @@ void apply_events(const orc_params* p, const orc_boundary* b, V3<R>& dp, V3<R> p
     }
 }
 
-// The step's constants; coef (may be NULL): the XSPH coefficient and the
-// vorticity epsilon in double, in place of the float ones of p.
+// Constants of the map: the FP32 step's own constants (make_consts<float>,
+// and the float factors wpoly6 / wspiky_deriv form), widened when R is
+// double, so that the FP64 replay and the adjoint differentiate the FP32
+// map.  coef (may be NULL): the XSPH coefficient and the vorticity epsilon
+// in double, in place of the float ones of p.
 template <class R>
-Consts<R> consts_with(const orc_params* p, const double* coef) {
-    Consts<R> k = make_consts<R>(p);
+struct MapK {
+    Consts<R> k;
+    R m2r2, alpha_x, K0, warg;
+};
+template <class R>
+MapK<R> map_consts(const orc_params* p, const double* coef) {
+    const Consts<float> f = make_consts<float>(p);
+    MapK<R> m;
+    Consts<R>& k = m.k;
+    k.h = R(f.h);
+    k.inv_h = R(f.inv_h);
+    k.m = R(f.m);
+    k.inv_rho0 = R(f.inv_rho0);
+    k.relax = R(f.relax);
+    k.a = R(f.a);
+    k.ad = R(f.ad);
+    k.inv_k = R(f.inv_k);
+    k.inv_wdq = R(f.inv_wdq);
+    k.sk = R(f.sk);
+    k.inv_dt = R(f.inv_dt);
+    k.dtc = R(f.dtc);
+    k.xsph = R(f.xsph);
+    k.vc = R(f.vc);
+    k.mode = f.mode;
+    k.n = f.n;
     if (coef) {
         k.xsph = R(coef[0]);
         k.vc = R(coef[1]);
     }
-    return k;
+    m.m2r2 = R(f.m * f.m * f.inv_rho0 * f.inv_rho0);
+    m.alpha_x = R(KC<float>::poly6 * f.inv_h * f.inv_h * f.inv_h);
+    const float wf = p->vort_h_as_inv_h ? f.h : f.inv_h;
+    const float w2 = wf * wf;
+    m.K0 = R(KC<float>::spiky * (w2 * w2));
+    m.warg = R(wf);
+    return m;
+}
+
+// wpoly6(r, inv_h) (pbf_kernel.inl:34-48) with the map's alpha: for R =
+// float the same operations as the FP32 step.
+template <class R>
+inline R xsph_w(R r, const MapK<R>& m) {
+    const R q = r * m.k.inv_h;
+    if (q <= R(1)) {
+        const R q2 = q * q;
+        const R t = R(1) - q2;
+        return m.alpha_x * t * t * t;
+    }
+    return R(0);
+}
+
+// wspiky_deriv(r, warg) (pbf_kernel.inl:50-63) with the map's factor.
+template <class R>
+inline R spiky_k(R r, const MapK<R>& m) {
+    const R q = r * m.warg;
+    if (q <= R(1)) {
+        const R t = R(1) - q;
+        return m.K0 * t * t;
+    }
+    return R(0);
 }
 
 // One step of the map.  live: the decisions are made here (R = float: the
@@ -135,7 +192,8 @@ template <class R>
 void step_map(const orc_params* p, const orc_boundary* b, int iters, StepRec& st, bool live, bool rec, const R* x,
               const R* v, uint32_t n_in, const float lo[3], const float hi[3], std::vector<R>& xo,
               std::vector<R>& vo, const double* coef = nullptr) {
-    const Consts<R> k = consts_with<R>(p, coef);
+    const MapK<R> mk = map_consts<R>(p, coef);
+    const Consts<R>& k = mk.k;
     NSData& ns = st.ns;
     if constexpr (std::is_same<R, float>::value) {
         if (live) {
@@ -164,6 +222,7 @@ void step_map(const orc_params* p, const orc_boundary* b, int iters, StepRec& st
         }
     }
     if (rec) {
+        st.f32 = std::is_same<R, float>::value;
         st.xin.assign(x, x + 3 * (size_t)n);
         st.P.assign((size_t)(iters + 1) * 3 * nl, 0.0);
         st.S.assign((size_t)iters * nl, 0.0);
@@ -196,8 +255,7 @@ void step_map(const orc_params* p, const orc_boundary* b, int iters, StepRec& st
             }
             R C = std::fma(ks * k.m, k.inv_rho0, R(-1));
             R G = std::fma(kgs.x, kgs.x, std::fma(kgs.z, kgs.z, std::fma(kgs.y, kgs.y, kg2s)));
-            R m2r2 = k.m * k.m * k.inv_rho0 * k.inv_rho0;
-            s[i] = C / std::fma(m2r2, G, k.relax);
+            s[i] = C / std::fma(mk.m2r2, G, k.relax);
         }
         if (rec)
             for (uint32_t i = 0; i < nl; ++i) st.S[(size_t)it * nl + i] = double(s[i]);
@@ -266,7 +324,6 @@ void step_map(const orc_params* p, const orc_boundary* b, int iters, StepRec& st
         }
     }
     const int32_t* gc = st.gcur.data();
-    const R warg = p->vort_h_as_inv_h ? k.h : k.inv_h;
     const R cut_w = R(support_bound2((float)k.inv_h));
     const R cut_wd = R(support_bound2(p->vort_h_as_inv_h ? p->smoothing_length : 1.f / p->smoothing_length));
     std::vector<R> Wv(3 * (size_t)nl, R(0));
@@ -280,7 +337,7 @@ void step_map(const orc_params* p, const orc_boundary* b, int iters, StepRec& st
                 R r = std::sqrt(r2);
                 V3<R> vd = load3(V, j) - selfv;
                 V3<R> vort = v3<R>(R(0), R(0), R(0));
-                if (r > R(0)) vort = mulvs(glm_cross(vd, divvs(pd, r)), wspiky_deriv<R>(r, warg));
+                if (r > R(0)) vort = mulvs(glm_cross(vd, divvs(pd, r)), spiky_k<R>(r, mk));
                 return vort;
             });
             store3(Wv, i, mulsv(k.m * k.inv_rho0, w));
@@ -294,7 +351,7 @@ void step_map(const orc_params* p, const orc_boundary* b, int iters, StepRec& st
                 R r = std::sqrt(r2);
                 R w = glm_length(load3(Wv, j));
                 V3<R> loc = v3<R>(R(0), R(0), R(0));
-                if (r > R(0)) loc = mulvs(mulsv(w, divvs(pd, r)), wspiky_deriv<R>(r, warg));
+                if (r > R(0)) loc = mulvs(mulsv(w, divvs(pd, r)), spiky_k<R>(r, mk));
                 return loc;
             });
             if (glm_length(N) > R(0)) {
@@ -320,7 +377,7 @@ void step_map(const orc_params* p, const orc_boundary* b, int iters, StepRec& st
             V3<R> acc = sum27_at<R>(ns, P, self, gc + 3 * (size_t)i, cut_w, [&](uint32_t j, V3<R>, R r2) {
                 R r = std::sqrt(r2);
                 V3<R> vd = load3(V, j) - selfv;
-                return mulvs(vd, wpoly6<R>(r, k.inv_h));
+                return mulvs(vd, xsph_w<R>(r, mk));
             });
             bool nan = std::isnan(acc.x) || std::isnan(acc.y) || std::isnan(acc.z);
             if (live)
@@ -411,17 +468,64 @@ struct SpikyD {
         return scale(e, Kp * ey) + scale(y - scale(e, ey), K / r);
     }
 };
-inline bool spiky_d(V3d pd, double warg, SpikyD& o) {
+inline bool spiky_d(V3d pd, const MapK<double>& m, SpikyD& o) {
     const double r2 = dot(pd, pd);
     if (r2 <= 0.0) return false;
     o.r = std::sqrt(r2);
-    const double q = o.r * warg;
+    const double q = o.r * m.warg;
     if (q > 1.0) return false;
     o.e = scale(pd, 1.0 / o.r);
-    const double w2 = warg * warg, w4 = w2 * w2;
-    o.K = KC<double>::spiky * w4 * (1.0 - q) * (1.0 - q);
-    o.Kp = -2.0 * KC<double>::spiky * w4 * warg * (1.0 - q);
+    o.K = m.K0 * (1.0 - q) * (1.0 - q);
+    o.Kp = -2.0 * m.K0 * m.warg * (1.0 - q);
     return true;
+}
+
+// Jacobian factors of the boundary responses, in firing order: kind 0 a
+// sphere ((1 - nudge) I + (R/|cd|)(I - u u^T), u = cd/|cd|), kind 1 a plane
+// (I - n n^T).
+struct JacF {
+    int kind;
+    V3d n;
+    double c0, c1;
+};
+
+inline JacF sphere_factor(V3d cd, double radius, bool inner) {
+    const double nudge = inner ? (double)KC<float>::nudge_in : (double)KC<float>::nudge_out;
+    const double L = std::sqrt(dot(cd, cd));
+    return JacF{0, scale(cd, 1.0 / L), 1.0 - nudge, radius / L};
+}
+
+// FP32 tapes: collide() exactly as the FP32 step runs it (same decisions,
+// same points), logging the factors; cd is formed in double from the float
+// point.
+void collide_factors_f32(const orc_params* p, const orc_boundary* b, V3<float>& dp, V3<float> pk,
+                         std::vector<JacF>& fs) {
+    const V3<float> pred_x = pk, old_x = pk;
+    int sweeps = 0;
+    while (sweeps < p->max_sweeps) {
+        ++sweeps;
+        V3<float> pp = pred_x + dp;
+        bool hit = false;
+        if (b) {
+            for (uint32_t q = 0; q < b->n_inner + b->n_outer; ++q) {
+                const bool is_inner = q < b->n_inner;
+                const float* s = is_inner ? b->inner + 4 * q : b->outer + 4 * (q - b->n_inner);
+                const bool inner = is_inner || p->outer_as_inner != 0;
+                const V3d cd = v3<double>((double)s[0] - pp.x, (double)s[1] - pp.y, (double)s[2] - pp.z);
+                if (response_sphere<float>(hit, dp, pp, v3<float>(s[0], s[1], s[2]), s[3], inner))
+                    fs.push_back(sphere_factor(cd, s[3], inner));
+                pp = pred_x + dp;
+            }
+            for (uint32_t q = 0; q < b->n_planes; ++q) {
+                const float* P = b->plane_points + 3 * q;
+                const float* n = b->plane_normals + 3 * q;
+                if (response_plane<float>(hit, dp, old_x, pp, v3<float>(P[0], P[1], P[2]), v3<float>(n[0], n[1], n[2])))
+                    fs.push_back(JacF{1, v3<double>(n[0], n[1], n[2]), 0.0, 0.0});
+                pp = pred_x + dp;
+            }
+        }
+        if (!hit) break;
+    }
 }
 
 // Adjoint of one step.  gx, gv: dL/d(x_out, v_out) [3*nl]; out: dL/d(x_in,
@@ -430,7 +534,8 @@ inline bool spiky_d(V3d pd, double warg, SpikyD& o) {
 void step_adjoint(const orc_params* p, const orc_boundary* b, int iters, const StepRec& st, const double* gx,
                   const double* gv, std::vector<double>& gxin, std::vector<double>& gvin, double& gc, double& gvc,
                   const double* coef) {
-    const Consts<double> k = consts_with<double>(p, coef);
+    const MapK<double> mk = map_consts<double>(p, coef);
+    const Consts<double>& k = mk.k;
     const NSData& ns = st.ns;
     const uint32_t nl = st.n_live, n = st.n_in;
     const int M = ns.max_pair;
@@ -438,9 +543,8 @@ void step_adjoint(const orc_params* p, const orc_boundary* b, int iters, const S
     const double beta = k.m * k.inv_rho0;
     const double gamma = k.xsph * beta;
     const double kappa = k.vc * k.dtc / k.inv_rho0;
-    const double warg = p->vort_h_as_inv_h ? k.h : k.inv_h;
     const double cut_w = (double)support_bound2((float)k.inv_h);
-    const double alpha = KC<double>::poly6 * k.inv_h * k.inv_h * k.inv_h;
+    const double alpha = mk.alpha_x;
     const size_t PI = (size_t)iters * n3;  // base of the final positions in st.P
     const std::vector<double>& Pv = st.P;
     const int32_t* g = st.gcur.data();
@@ -505,7 +609,7 @@ void step_adjoint(const orc_params* p, const orc_boundary* b, int iters, const S
             V3d eta = v3<double>(0, 0, 0);
             for27(ns, g + 3 * (size_t)i, [&](uint32_t j) {
                 SpikyD sd;
-                if (!spiky_d(at(Pv, PI, j) - self, warg, sd)) return;
+                if (!spiky_d(at(Pv, PI, j) - self, mk, sd)) return;
                 eta += scale(sd.f(), std::sqrt(dot(at(st.Om, 0, j), at(st.Om, 0, j))));
             });
             const double nr = std::sqrt(dot(eta, eta));
@@ -534,7 +638,7 @@ void step_adjoint(const orc_params* p, const orc_boundary* b, int iters, const S
                 V3d apos = v3<double>(0, 0, 0);
                 for27(ns, g + 3 * (size_t)i, [&](uint32_t j) {
                     SpikyD sd;
-                    if (!spiky_d(at(Pv, PI, j) - self, warg, sd)) return;
+                    if (!spiky_d(at(Pv, PI, j) - self, mk, sd)) return;
                     tm[j] += dot(eb, sd.f());
                     const V3d pdb = scale(sd.H(eb), std::sqrt(dot(at(st.Om, 0, j), at(st.Om, 0, j))));
                     add3(tp, j, pdb);
@@ -565,7 +669,7 @@ void step_adjoint(const orc_params* p, const orc_boundary* b, int iters, const S
                 V3d av = v3<double>(0, 0, 0), apos = v3<double>(0, 0, 0);
                 for27(ns, g + 3 * (size_t)i, [&](uint32_t j) {
                     SpikyD sd;
-                    if (!spiky_d(at(Pv, PI, j) - self, warg, sd)) return;
+                    if (!spiky_d(at(Pv, PI, j) - self, mk, sd)) return;
                     const V3d e = at(st.V, 0, j) - vi;
                     const V3d ebar = scale(cross(sd.f(), ab), beta);
                     add3(tv, j, ebar);
@@ -590,7 +694,7 @@ void step_adjoint(const orc_params* p, const orc_boundary* b, int iters, const S
     }
 
     // Jacobi iterations, last first
-    const double m2r2 = k.m * k.m * k.inv_rho0 * k.inv_rho0;
+    const double m2r2 = mk.m2r2;
     std::vector<double> abar(n3), sbar(nl), aPn(n3);
     for (int it = iters - 1; it >= 0; --it) {
         const size_t PB = (size_t)it * n3;
@@ -611,42 +715,73 @@ void step_adjoint(const orc_params* p, const orc_boundary* b, int iters, const S
             V3d a = vget(aP.data(), i);
             const size_t e0 = st.ev_off[(size_t)it * nl + i], e1 = st.ev_off[(size_t)it * nl + i + 1];
             if (e1 > e0) {
-                V3d sum = v3<double>(0, 0, 0);
-                const uint32_t* lst = &ns.nb[(size_t)i * M];
-                for (uint32_t c = 0; c < ns.nbc[i]; ++c) {
-                    const uint32_t j = lst[c];
-                    PairD pd;
-                    if (!pair_d(self - at(Pv, PB, j), k, pd)) continue;
-                    sum += scale(pd.kg, -(s[i] + s[j] + corr_of(pd.W, s[i], s[j])));
-                }
-                V3d pp = self + scale(sum, beta);
-                struct F {
-                    int kind;
-                    V3d n;
-                    double c0, c1;
-                };
-                std::vector<F> fs;
-                for (size_t q = e0; q < e1; ++q) {
-                    const int kind = st.ev[q] >> 12;
-                    const uint32_t idx = st.ev[q] & 0xFFFu;
-                    if (kind == EV_INNER || kind == EV_OUTER) {
-                        const float* sp = (kind == EV_INNER ? b->inner : b->outer) + 4 * idx;
-                        const bool inner = kind == EV_INNER || p->outer_as_inner != 0;
-                        const double nudge = inner ? KC<double>::nudge_in : KC<double>::nudge_out;
-                        const V3d cd = v3<double>(sp[0], sp[1], sp[2]) - pp;
-                        const double L = std::sqrt(dot(cd, cd));
-                        fs.push_back({0, scale(cd, 1.0 / L), 1.0 - nudge, (double)sp[3] / L});
-                        pp = pp + scale(cd, nudge) - scale(cd, (double)sp[3] / L);
-                    } else {
-                        const float* P0 = b->plane_points + 3 * idx;
-                        const float* nn = b->plane_normals + 3 * idx;
-                        const V3d nv = v3<double>(nn[0], nn[1], nn[2]);
-                        fs.push_back({1, nv, 0, 0});
-                        pp = pp - scale(nv, dot(pp - v3<double>(P0[0], P0[1], P0[2]), nv));
+                std::vector<JacF> fs;
+                if (st.f32) {
+                    // the FP32 step's own delta p (pbf_position_update.cu:131-188)
+                    const Consts<float> kf = make_consts<float>(p);
+                    auto ldf = [&](uint32_t q) {
+                        return v3<float>((float)Pv[PB + 3 * (size_t)q], (float)Pv[PB + 3 * (size_t)q + 1],
+                                         (float)Pv[PB + 3 * (size_t)q + 2]);
+                    };
+                    const V3<float> sf = ldf(i);
+                    const float si = (float)s[i];
+                    const float self_cl = (si < 0.f) ? -si : 0.f;
+                    V3<float> sum = v3<float>(0.f, 0.f, 0.f);
+                    const uint32_t* lst = &ns.nb[(size_t)i * M];
+                    for (uint32_t c = 0; c < ns.nbc[i]; ++c) {
+                        const uint32_t j = lst[c];
+                        float W;
+                        V3<float> kg;
+                        pair_terms<float>(sf, ldf(j), kf, W, kg);
+                        const float sj = (float)s[j];
+                        float corr;
+                        if (kf.mode == ORC_SCORR_REFERENCE) {
+                            const float kr2 = W * W;
+                            const float kr4 = kr2 * kr2;
+                            const float tc = KC<float>::ten * kr4 * kf.inv_k;
+                            const float sup = (sj < 0.f) ? -sj : 0.f;
+                            corr = (self_cl + sup) * tc;
+                        } else {
+                            corr = kf.sk * powi<float>(W * kf.inv_wdq, kf.n);
+                        }
+                        sum += mulsv(-(si + sj + corr), kg);
+                    }
+                    V3<float> dp = v3<float>(sum.x * kf.m * kf.inv_rho0, sum.y * kf.m * kf.inv_rho0,
+                                             sum.z * kf.m * kf.inv_rho0);
+                    collide_factors_f32(p, b, dp, sf, fs);
+                } else {
+                    // FP64 tape: the recorded responses at the FP64 points
+                    V3d sum = v3<double>(0, 0, 0);
+                    const uint32_t* lst = &ns.nb[(size_t)i * M];
+                    for (uint32_t c = 0; c < ns.nbc[i]; ++c) {
+                        const uint32_t j = lst[c];
+                        PairD pd;
+                        if (!pair_d(self - at(Pv, PB, j), k, pd)) continue;
+                        sum += scale(pd.kg, -(s[i] + s[j] + corr_of(pd.W, s[i], s[j])));
+                    }
+                    V3d pp = self + scale(sum, beta);
+                    for (size_t q = e0; q < e1; ++q) {
+                        const int kind = st.ev[q] >> 12;
+                        const uint32_t idx = st.ev[q] & 0xFFFu;
+                        if (kind == EV_INNER || kind == EV_OUTER) {
+                            const float* sp = (kind == EV_INNER ? b->inner : b->outer) + 4 * idx;
+                            const bool inner = kind == EV_INNER || p->outer_as_inner != 0;
+                            const double nudge = inner ? (double)KC<float>::nudge_in : (double)KC<float>::nudge_out;
+                            const V3d cd = v3<double>(sp[0], sp[1], sp[2]) - pp;
+                            const double L = std::sqrt(dot(cd, cd));
+                            fs.push_back(sphere_factor(cd, sp[3], inner));
+                            pp = pp + scale(cd, nudge) - scale(cd, (double)sp[3] / L);
+                        } else {
+                            const float* P0 = b->plane_points + 3 * idx;
+                            const float* nn = b->plane_normals + 3 * idx;
+                            const V3d nv = v3<double>(nn[0], nn[1], nn[2]);
+                            fs.push_back(JacF{1, nv, 0, 0});
+                            pp = pp - scale(nv, dot(pp - v3<double>(P0[0], P0[1], P0[2]), nv));
+                        }
                     }
                 }
                 for (size_t q = fs.size(); q-- > 0;) {
-                    const F& f = fs[q];
+                    const JacF& f = fs[q];
                     if (f.kind == 1)
                         a = a - scale(f.n, dot(f.n, a));
                     else
@@ -711,23 +846,18 @@ void step_adjoint(const orc_params* p, const orc_boundary* b, int iters, const S
                 double* tp = sP.mine();
                 const V3d self = at(Pv, PB, i);
                 const uint32_t* lst = &ns.nb[(size_t)i * M];
-                double ks = 0.0, kg2s = 0.0;
+                double kg2s = 0.0;
                 V3d kgs = v3<double>(0, 0, 0);
                 for (uint32_t c = 0; c < ns.nbc[i]; ++c) {
                     PairD pd;
                     const V3d d = self - at(Pv, PB, lst[c]);
-                    if (!pair_d(d, k, pd)) {
-                        ks += k.a;  // the pair's own term at r = 0: W = a, kg = 0
-                        continue;
-                    }
-                    ks += pd.W;
+                    if (!pair_d(d, k, pd)) continue;  // r = 0: kg = 0
                     kgs += pd.kg;
                     kg2s += dot(pd.kg, pd.kg);
                 }
-                const double C = ks * beta - 1.0;
                 const double D = m2r2 * (dot(kgs, kgs) + kg2s) + k.relax;
                 const double ksb = beta * sbar[i] / D;
-                const double Gb = -m2r2 * sbar[i] * (C / D) / D;
+                const double Gb = -m2r2 * sbar[i] * s[i] / D;
                 V3d apos = v3<double>(0, 0, 0);
                 for (uint32_t c = 0; c < ns.nbc[i]; ++c) {
                     const uint32_t j = lst[c];

@@ -17,6 +17,9 @@
 //   - the particles grouped by final cell (the transposed 27-cell sets of the
 //     vorticity, confinement and XSPH sums), in slot order within a cell.
 // Sums therefore run in a fixed order and results are reproducible run to run.
+// All adjoint arithmetic is FP64 (the tape's FP32 values are widened on
+// load): in exploding flows the map is conditioned up to ~1e7, which turns
+// FP32 rounding into O(1) error over tens of steps.
 //
 // Stages, last first (per step):
 //   XSPH        v_out = Vc + gamma sum_j (Vc_j - Vc_i) W(|p_j - p_i|)
@@ -33,112 +36,172 @@
 namespace pbfx {
 namespace {
 
-constexpr float kSpiky = -14.3239448745f;
-
-__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
-    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+// ------------------------------------------------------------ FP64 vectors
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 md3(double x, double y, double z) {
+    D3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
 }
-__device__ __forceinline__ F3 sc3(F3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ bool zero3(F3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
-__device__ __forceinline__ float4 f4of(F3 a) { return make_float4(a.x, a.y, a.z, 0.f); }
-__device__ __forceinline__ F3 ld3aos(const float* a, uint32_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return md3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return md3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ D3 sc3(D3 a, double s) { return md3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ double dot3(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross3(D3 a, D3 b) {
+    return md3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ bool zero3(D3 a) { return a.x == 0.0 && a.y == 0.0 && a.z == 0.0; }
+__device__ __forceinline__ D3 dz3() { return md3(0.0, 0.0, 0.0); }
+__device__ __forceinline__ D3 wid(float4 f) { return md3(f.x, f.y, f.z); }
+__device__ __forceinline__ D3 wid3(F3 f) { return md3(f.x, f.y, f.z); }
+__device__ __forceinline__ D3 ld(const double4* a, uint32_t i) {
+    const double4 q = a[i];
+    return md3(q.x, q.y, q.z);
+}
+__device__ __forceinline__ void st(double4* a, uint32_t i, D3 v) { a[i] = make_double4(v.x, v.y, v.z, 0.0); }
+__device__ __forceinline__ D3 ld3d(const double* a, uint32_t i) { return md3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+// sorted input position i (SoA, stride cap)
+__device__ __forceinline__ D3 ldX(const float* X, uint32_t cap, uint32_t i) {
+    return md3(X[i], X[(size_t)cap + i], X[2 * (size_t)cap + i]);
+}
+
+__device__ __forceinline__ double powid(double t, int n) {
+    double r = 1.0;
+    for (int k = 0; k < n; ++k) r *= t;
+    return r;
+}
+
+// Constants of the step in FP64: the FP32 step's own (float) constants,
+// widened, so that the derivative is that of the FP32 map.
+struct AdjConst {
+    double beta, gamma, kappa, dpdvc, m2r2, relax, inv_dt, dt, warg, inv_k, inv_wdq, sk;
+    double a, ad, ih, alpha_x, K0;  // memo W / kg, 1/h, XSPH poly6 and spiky-derivative factors
+    int mode, sn;
+};
+__device__ __forceinline__ AdjConst adj_const(const Phys& P) {
+    AdjConst k;
+    k.beta = (double)P.m * (double)P.inv_rho0;
+    k.gamma = (double)P.xsph_c * k.beta;
+    k.dpdvc = (double)P.dtc / (double)P.inv_rho0;
+    k.kappa = (double)P.vc * k.dpdvc;
+    k.m2r2 = P.m2r2;
+    k.relax = P.relax;
+    k.inv_dt = P.inv_dt;
+    k.dt = P.dt;
+    k.warg = P.warg;
+    k.inv_k = P.inv_k;
+    k.inv_wdq = P.inv_wdq;
+    k.sk = P.sk;
+    k.mode = P.mode;
+    k.sn = P.sn;
+    k.a = P.a;
+    k.ad = P.ad;
+    k.ih = P.inv_h;
+    // wpoly6 / wspiky_deriv form these in float (pbf_kernel.inl:34-63)
+    k.alpha_x = 1.56668147065f * P.inv_h * P.inv_h * P.inv_h;
+    const float w2 = P.warg * P.warg;
+    k.K0 = -14.3239448745f * (w2 * w2);
+    return k;
+}
 
 // Memo kernels of a listed pair as smooth functions of d = p_i - p_j
 // (pbf_kernel_memorization.cu:15-56): W = a t^3 with t = 1 - r^2/h^2,
 // kg = kd(r) e with kd = ad (1 - r/h)^2, e = d/r.  grad_d W = gw d and
 // d kg/d d = kdp e e^T + (kd/r)(I - e e^T).
 struct PairAdj {
-    float W, gw, kd, kdp, r;
-    F3 e, kg;
-    __device__ __forceinline__ F3 H(F3 y) const {
-        const float ey = dot3(e, y);
+    double W, gw, kd, kdp, r;
+    D3 e, kg;
+    __device__ __forceinline__ D3 H(D3 y) const {
+        const double ey = dot3(e, y);
         return sc3(e, kdp * ey) + sc3(y - sc3(e, ey), kd / r);
     }
 };
-__device__ __forceinline__ bool pair_adj(F3 d, const Phys& P, PairAdj& o) {
-    const float r2 = dot3(d, d);
-    if (!(r2 > 0.f)) return false;
-    o.r = __builtin_sqrtf(r2);
-    o.e = sc3(d, 1.f / o.r);
-    const float q2 = r2 * P.inv_h * P.inv_h;
-    o.W = 0.f;
-    o.gw = 0.f;
-    if (q2 <= 1.f) {
-        const float t = 1.f - q2;
-        o.W = P.a * t * t * t;
-        o.gw = -6.f * P.a * P.inv_h * P.inv_h * t * t;
+__device__ __forceinline__ bool pair_adj(D3 d, const AdjConst& k, PairAdj& o) {
+    const double r2 = dot3(d, d);
+    if (!(r2 > 0.0)) return false;
+    o.r = sqrt(r2);
+    o.e = sc3(d, 1.0 / o.r);
+    const double q2 = r2 * k.ih * k.ih;
+    o.W = 0.0;
+    o.gw = 0.0;
+    if (q2 <= 1.0) {
+        const double t = 1.0 - q2;
+        o.W = k.a * t * t * t;
+        o.gw = -6.0 * k.a * k.ih * k.ih * t * t;
     }
-    o.kd = 0.f;
-    o.kdp = 0.f;
-    const float q = o.r * P.inv_h;
-    if (q <= 1.f) {
-        const float u = 1.f - q;
-        o.kd = P.ad * u * u;
-        o.kdp = -2.f * P.ad * u * P.inv_h;
+    o.kd = 0.0;
+    o.kdp = 0.0;
+    const double q = o.r * k.ih;
+    if (q <= 1.0) {
+        const double u = 1.0 - q;
+        o.kd = k.ad * u * u;
+        o.kdp = -2.0 * k.ad * u * k.ih;
     }
     o.kg = sc3(o.e, o.kd);
     return true;
 }
 
 // Spiky derivative with argument warg along pd = p_j - p_i
-// (pbf/kernel/cuda/detail/pbf_kernel.inl:50-63): f = K(r) e,
-// df/dpd = Kp e e^T + (K/r)(I - e e^T).
+// (pbf/kernel/cuda/detail/pbf_kernel.inl:50-63): f = K(r) e with
+// K = K0 (1 - r warg)^2; df/dpd = Kp e e^T + (K/r)(I - e e^T).
 struct SpikyAdj {
-    float K, Kp, r;
-    F3 e;
-    __device__ __forceinline__ F3 f() const { return sc3(e, K); }
-    __device__ __forceinline__ F3 H(F3 y) const {
-        const float ey = dot3(e, y);
+    double K, Kp, r;
+    D3 e;
+    __device__ __forceinline__ D3 f() const { return sc3(e, K); }
+    __device__ __forceinline__ D3 H(D3 y) const {
+        const double ey = dot3(e, y);
         return sc3(e, Kp * ey) + sc3(y - sc3(e, ey), K / r);
     }
 };
-__device__ __forceinline__ bool spiky_adj(F3 pd, float warg, SpikyAdj& o) {
-    const float r2 = dot3(pd, pd);
-    if (!(r2 > 0.f)) return false;
-    o.r = __builtin_sqrtf(r2);
-    const float q = o.r * warg;
-    if (q > 1.f) return false;
-    o.e = sc3(pd, 1.f / o.r);
-    const float w2 = warg * warg, w4 = w2 * w2;
-    o.K = kSpiky * w4 * (1.f - q) * (1.f - q);
-    o.Kp = -2.f * kSpiky * w4 * warg * (1.f - q);
+__device__ __forceinline__ bool spiky_adj(D3 pd, const AdjConst& k, SpikyAdj& o) {
+    const double r2 = dot3(pd, pd);
+    if (!(r2 > 0.0)) return false;
+    o.r = sqrt(r2);
+    const double q = o.r * k.warg;
+    if (q > 1.0) return false;
+    o.e = sc3(pd, 1.0 / o.r);
+    o.K = k.K0 * (1.0 - q) * (1.0 - q);
+    o.Kp = -2.0 * k.K0 * k.warg * (1.0 - q);
     return true;
 }
 
 // poly6 with inv_h along pd (the XSPH weight, pbf_kernel.inl:34-48):
-// W = a t^3, t = 1 - r^2/h^2; grad_pd W = gw pd.  False beyond the support.
-__device__ __forceinline__ bool poly6_adj(F3 pd, const Phys& P, float& W, float& gw) {
-    const float q2 = dot3(pd, pd) * P.inv_h * P.inv_h;
-    if (q2 > 1.f) return false;
-    const float t = 1.f - q2;
-    W = P.a * t * t * t;
-    gw = -6.f * P.a * P.inv_h * P.inv_h * t * t;
+// W = alpha t^3, t = 1 - r^2/h^2; grad_pd W = gw pd.  False beyond the support.
+__device__ __forceinline__ bool poly6_adj(D3 pd, const AdjConst& k, double& W, double& gw) {
+    const double q2 = dot3(pd, pd) * k.ih * k.ih;
+    if (q2 > 1.0) return false;
+    const double t = 1.0 - q2;
+    W = k.alpha_x * t * t * t;
+    gw = -6.0 * k.alpha_x * k.ih * k.ih * t * t;
     return true;
 }
 
 // tensile correction of a pair and its derivatives (pbf_position_update.cu:131-188)
-__device__ __forceinline__ float corr_of(const Phys& P, float W, float si, float sj) {
-    if (P.mode == 0) {
-        const float cl = (si < 0.f ? -si : 0.f) + (sj < 0.f ? -sj : 0.f);
-        const float W2 = W * W;
-        return cl * 10.f * W2 * W2 * P.inv_k;
+__device__ __forceinline__ double corr_of(const AdjConst& k, double W, double si, double sj) {
+    if (k.mode == 0) {
+        const double cl = (si < 0.0 ? -si : 0.0) + (sj < 0.0 ? -sj : 0.0);
+        const double W2 = W * W;
+        return cl * 10.0 * W2 * W2 * k.inv_k;
     }
-    return P.sk * powi(W * P.inv_wdq, P.sn);
+    return k.sk * powid(W * k.inv_wdq, k.sn);
 }
 // dcorr/dW
-__device__ __forceinline__ float corr_dW(const Phys& P, float W, float si, float sj) {
-    if (P.mode == 0) {
-        const float cl = (si < 0.f ? -si : 0.f) + (sj < 0.f ? -sj : 0.f);
-        return cl * 40.f * W * W * W * P.inv_k;
+__device__ __forceinline__ double corr_dW(const AdjConst& k, double W, double si, double sj) {
+    if (k.mode == 0) {
+        const double cl = (si < 0.0 ? -si : 0.0) + (sj < 0.0 ? -sj : 0.0);
+        return cl * 40.0 * W * W * W * k.inv_k;
     }
-    return P.sn > 0 ? P.sk * (float)P.sn * powi(W * P.inv_wdq, P.sn - 1) * P.inv_wdq : 0.f;
+    return k.sn > 0 ? k.sk * (double)k.sn * powid(W * k.inv_wdq, k.sn - 1) * k.inv_wdq : 0.0;
 }
-// dcorr/ds of one end whose s is `s` (REFERENCE form only)
-__device__ __forceinline__ float corr_ds(const Phys& P, float W, float s) {
-    if (P.mode != 0 || !(s < 0.f)) return 0.f;
-    const float W2 = W * W;
-    return -10.f * W2 * W2 * P.inv_k;
+// dcorr/ds of the end whose s is `s` (REFERENCE form only)
+__device__ __forceinline__ double corr_ds(const AdjConst& k, double W, double s) {
+    if (k.mode != 0 || !(s < 0.0)) return 0.0;
+    const double W2 = W * W;
+    return -10.0 * W2 * W2 * k.inv_k;
 }
 
 // the 27 search cells around the cell of p, in z, y, x order
@@ -286,34 +349,34 @@ __global__ __launch_bounds__(256) void k_adj_xsph_own(StepTape t, AdjBufs a, Gri
     const uint32_t i = gtid();
     double pc = 0.0;
     if (i < t.n_live) {
-        const float beta = P.m * P.inv_rho0;
-        const float gamma = P.xsph_c * P.m * P.inv_rho0;
-        const F3 self = f3of(t.PI[i]), vi = f3of(t.R[2 * i + 1]);
-        const F3 gi = ld3aos(a.av, i);
-        float sumW = 0.f;
-        F3 acc = mk3(0.f, 0.f, 0.f), apos = mk3(0.f, 0.f, 0.f);
-        for_S(t, G, self, [&](uint32_t j) {
+        const AdjConst k = adj_const(P);
+        const F3 selff = f3of(t.PI[i]);
+        const D3 self = wid3(selff), vi = wid(t.R[2 * i + 1]);
+        const D3 gi = ld3d(a.av, i);
+        double sumW = 0.0;
+        D3 acc = dz3(), apos = dz3();
+        for_S(t, G, selff, [&](uint32_t j) {
             if (j == i) return;
-            const F3 pd = f3of(t.PI[j]) - self;
-            if (dot3(pd, pd) > P.cut2_w) return;
-            float W, gw;
-            if (!poly6_adj(pd, P, W, gw)) return;
-            const F3 e = f3of(t.R[2 * j + 1]) - vi;
+            const D3 pd = wid(t.PI[j]) - self;
+            if (dot3(pd, pd) > (double)P.cut2_w) return;
+            double W, gw;
+            if (!poly6_adj(pd, k, W, gw)) return;
+            const D3 e = wid(t.R[2 * j + 1]) - vi;
             acc = acc + sc3(e, W);
             sumW += W;
-            apos = apos - sc3(pd, gw * gamma * dot3(gi, e));
+            apos = apos - sc3(pd, gw * k.gamma * dot3(gi, e));
         });
         const bool nan = isnan(acc.x) || isnan(acc.y) || isnan(acc.z);
         a.xnan[i] = nan ? 1u : 0u;
-        F3 avc = gi;
+        D3 avc = gi;
         if (nan) {
-            apos = mk3(0.f, 0.f, 0.f);
+            apos = dz3();
         } else {
-            avc = sc3(gi, 1.f - gamma * sumW);
-            pc = (double)beta * (double)dot3(gi, acc);
+            avc = sc3(gi, 1.0 - k.gamma * sumW);
+            pc = k.beta * dot3(gi, acc);
         }
-        a.aVc[i] = f4of(avc);
-        a.aP[i] = f4of(ld3aos(a.ax, i) + apos);
+        st(a.aVc, i, avc);
+        st(a.aP, i, ld3d(a.ax, i) + apos);
     }
     const double tot = block_sum(pc, sh);
     if (threadIdx.x == 0) a.part[blockIdx.x] = tot;
@@ -323,29 +386,29 @@ __global__ __launch_bounds__(256) void k_adj_xsph_own(StepTape t, AdjBufs a, Gri
 __global__ __launch_bounds__(256) void k_adj_xsph_rev(StepTape t, AdjBufs a, Grid G, Phys P) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
-    const float gamma = P.xsph_c * P.m * P.inv_rho0;
-    const F3 pi = f3of(t.PI[i]), vi = f3of(t.R[2 * i + 1]);
-    F3 avc = mk3(0.f, 0.f, 0.f), apos = mk3(0.f, 0.f, 0.f);
+    const AdjConst k = adj_const(P);
+    const D3 pi = wid(t.PI[i]), vi = wid(t.R[2 * i + 1]);
+    D3 avc = dz3(), apos = dz3();
     for_T(a, G, f3of(t.Q[i]), [&](uint32_t l) {
         if (l == i || a.xnan[l]) return;
-        const F3 pd = pi - f3of(t.PI[l]);
-        if (dot3(pd, pd) > P.cut2_w) return;
-        float W, gw;
-        if (!poly6_adj(pd, P, W, gw)) return;
-        const F3 gl = ld3aos(a.av, l);
-        avc = avc + sc3(gl, gamma * W);
-        apos = apos + sc3(pd, gw * gamma * dot3(gl, vi - f3of(t.R[2 * l + 1])));
+        const D3 pd = pi - wid(t.PI[l]);
+        if (dot3(pd, pd) > (double)P.cut2_w) return;
+        double W, gw;
+        if (!poly6_adj(pd, k, W, gw)) return;
+        const D3 gl = ld3d(a.av, l);
+        avc = avc + sc3(gl, k.gamma * W);
+        apos = apos + sc3(pd, gw * k.gamma * dot3(gl, vi - wid(t.R[2 * l + 1])));
     });
-    a.aVc[i] = f4of(f3of(a.aVc[i]) + avc);
-    a.aP[i] = f4of(f3of(a.aP[i]) + apos);
+    st(a.aVc, i, ld(a.aVc, i) + avc);
+    st(a.aP, i, ld(a.aP, i) + apos);
 }
 
-// Without XSPH: v_out = Vc.
+// Without XSPH: v_out = Vc.  Clears the partial sums.
 __global__ __launch_bounds__(256) void k_adj_final_init(StepTape t, AdjBufs a) {
     const uint32_t i = gtid();
     if (i < t.n_live) {
-        a.aVc[i] = f4of(ld3aos(a.av, i));
-        a.aP[i] = f4of(ld3aos(a.ax, i));
+        st(a.aVc, i, ld3d(a.av, i));
+        st(a.aP, i, ld3d(a.ax, i));
     }
     if (i < kAdjMaxBlocks) {
         a.part[i] = 0.0;
@@ -360,28 +423,30 @@ __global__ __launch_bounds__(256) void k_adj_conf_own(StepTape t, AdjBufs a, Gri
     const uint32_t i = gtid();
     double pv = 0.0;
     if (i < t.n_live) {
-        const F3 self = f3of(t.PI[i]);
-        F3 eta = mk3(0.f, 0.f, 0.f);
-        for_S(t, G, self, [&](uint32_t j) {
+        const AdjConst k = adj_const(P);
+        const F3 selff = f3of(t.PI[i]);
+        const D3 self = wid3(selff);
+        D3 eta = dz3();
+        for_S(t, G, selff, [&](uint32_t j) {
             SpikyAdj sd;
-            if (j == i || !spiky_adj(f3of(t.PI[j]) - self, P.warg, sd)) return;
-            eta = eta + sc3(sd.f(), t.OM4[j].w);
+            if (j == i || !spiky_adj(wid(t.PI[j]) - self, k, sd)) return;
+            const D3 om = wid(t.OM4[j]);
+            eta = eta + sc3(sd.f(), sqrt(dot3(om, om)));
         });
-        const F3 avc = f3of(a.aVc[i]);
-        a.aV[i] = f4of(avc);
-        const float nr = __builtin_sqrtf(dot3(eta, eta));
-        F3 aom = mk3(0.f, 0.f, 0.f), eb = mk3(0.f, 0.f, 0.f);
-        if (nr > 0.f) {
-            const F3 Nh = sc3(eta, 1.f / nr), om = f3of(t.OM4[i]);
-            const float kappa = P.vc * P.dtc / P.inv_rho0;
-            pv = (double)dot3(avc, cross3(Nh, om)) * (double)(P.dtc / P.inv_rho0);
-            const F3 y = sc3(avc, kappa);
+        const D3 avc = ld(a.aVc, i);
+        st(a.aV, i, avc);
+        const double nr = sqrt(dot3(eta, eta));
+        D3 aom = dz3(), eb = dz3();
+        if (nr > 0.0) {
+            const D3 Nh = sc3(eta, 1.0 / nr), om = wid(t.OM4[i]);
+            pv = dot3(avc, cross3(Nh, om)) * k.dpdvc;
+            const D3 y = sc3(avc, k.kappa);
             aom = cross3(y, Nh);
-            const F3 nb = cross3(om, y);
-            eb = sc3(nb - sc3(Nh, dot3(Nh, nb)), 1.f / nr);
+            const D3 nb = cross3(om, y);
+            eb = sc3(nb - sc3(Nh, dot3(Nh, nb)), 1.0 / nr);
         }
-        a.aOm[i] = f4of(aom);
-        a.etab[i] = f4of(eb);
+        st(a.aOm, i, aom);
+        st(a.etab, i, eb);
     }
     const double tot = block_sum(pv, sh);
     if (threadIdx.x == 0) a.part[kAdjMaxBlocks + blockIdx.x] = tot;
@@ -392,30 +457,34 @@ __global__ __launch_bounds__(256) void k_adj_conf_own(StepTape t, AdjBufs a, Gri
 __global__ __launch_bounds__(256) void k_adj_conf_eta(StepTape t, AdjBufs a, Grid G, Phys P) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
-    const F3 self = f3of(t.PI[i]);
-    const F3 eb = f3of(a.etab[i]);
-    const float wi = t.OM4[i].w;
-    F3 apos = mk3(0.f, 0.f, 0.f);
+    const AdjConst k = adj_const(P);
+    const F3 selff = f3of(t.PI[i]);
+    const D3 self = wid3(selff);
+    const D3 eb = ld(a.etab, i);
+    const D3 omi = wid(t.OM4[i]);
+    const double wi = sqrt(dot3(omi, omi));
+    D3 apos = dz3();
     if (!zero3(eb))
-        for_S(t, G, self, [&](uint32_t j) {
+        for_S(t, G, selff, [&](uint32_t j) {
             SpikyAdj sd;
-            if (j == i || !spiky_adj(f3of(t.PI[j]) - self, P.warg, sd)) return;
-            apos = apos - sc3(sd.H(eb), t.OM4[j].w);
+            if (j == i || !spiky_adj(wid(t.PI[j]) - self, k, sd)) return;
+            const D3 om = wid(t.OM4[j]);
+            apos = apos - sc3(sd.H(eb), sqrt(dot3(om, om)));
         });
-    float mb = 0.f;
+    double mb = 0.0;
     for_T(a, G, f3of(t.Q[i]), [&](uint32_t l) {
         if (l == i) return;
-        const F3 el = f3of(a.etab[l]);
+        const D3 el = ld(a.etab, l);
         if (zero3(el)) return;
         SpikyAdj sd;
-        if (!spiky_adj(self - f3of(t.PI[l]), P.warg, sd)) return;
+        if (!spiky_adj(self - wid(t.PI[l]), k, sd)) return;
         mb += dot3(el, sd.f());
         apos = apos + sc3(sd.H(el), wi);
     });
-    F3 aom = f3of(a.aOm[i]);
-    if (wi > 0.f) aom = aom + sc3(f3of(t.OM4[i]), mb / wi);
-    a.aOm[i] = f4of(aom);
-    a.aP[i] = f4of(f3of(a.aP[i]) + apos);
+    D3 aom = ld(a.aOm, i);
+    if (wi > 0.0) aom = aom + sc3(omi, mb / wi);
+    st(a.aOm, i, aom);
+    st(a.aP, i, ld(a.aP, i) + apos);
 }
 
 // Vorticity: dL/dV and the position terms (own pairs and transposed pairs),
@@ -423,42 +492,46 @@ __global__ __launch_bounds__(256) void k_adj_conf_eta(StepTape t, AdjBufs a, Gri
 __global__ __launch_bounds__(256) void k_adj_vort(StepTape t, AdjBufs a, Grid G, Phys P, uint32_t cap) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
-    const float beta = P.m * P.inv_rho0;
-    const F3 self = f3of(t.PI[i]);
-    const F3 vi = mulsv(P.inv_dt, self - ld3(t.X, cap, i));
-    const F3 ab = f3of(a.aOm[i]);
-    F3 av = f3of(a.aV[i]), apos = mk3(0.f, 0.f, 0.f);
+    const AdjConst k = adj_const(P);
+    const F3 selff = f3of(t.PI[i]);
+    const D3 self = wid3(selff);
+    // V exactly as the forward formed it (FP32), then widened
+    auto vel = [&](uint32_t q, F3 p) { return wid3(mulsv(P.inv_dt, p - ld3(t.X, cap, q))); };
+    const D3 vi = vel(i, selff);
+    const D3 ab = ld(a.aOm, i);
+    D3 av = ld(a.aV, i), apos = dz3();
     if (!zero3(ab))
-        for_S(t, G, self, [&](uint32_t j) {
+        for_S(t, G, selff, [&](uint32_t j) {
             SpikyAdj sd;
-            const F3 pj = f3of(t.PI[j]);
-            if (j == i || !spiky_adj(pj - self, P.warg, sd)) return;
-            const F3 e = mulsv(P.inv_dt, pj - ld3(t.X, cap, j)) - vi;
-            av = av - sc3(cross3(sd.f(), ab), beta);
-            apos = apos - sd.H(sc3(cross3(ab, e), beta));
+            const F3 pjf = f3of(t.PI[j]);
+            if (j == i || !spiky_adj(wid3(pjf) - self, k, sd)) return;
+            const D3 e = vel(j, pjf) - vi;
+            av = av - sc3(cross3(sd.f(), ab), k.beta);
+            apos = apos - sd.H(sc3(cross3(ab, e), k.beta));
         });
     for_T(a, G, f3of(t.Q[i]), [&](uint32_t l) {
         if (l == i) return;
-        const F3 al = f3of(a.aOm[l]);
+        const D3 al = ld(a.aOm, l);
         if (zero3(al)) return;
         SpikyAdj sd;
-        const F3 pl = f3of(t.PI[l]);
-        if (!spiky_adj(self - pl, P.warg, sd)) return;
-        const F3 e = vi - mulsv(P.inv_dt, pl - ld3(t.X, cap, l));
-        av = av + sc3(cross3(sd.f(), al), beta);
-        apos = apos + sd.H(sc3(cross3(al, e), beta));
+        const F3 plf = f3of(t.PI[l]);
+        if (!spiky_adj(self - wid3(plf), k, sd)) return;
+        const D3 e = vi - vel(l, plf);
+        av = av + sc3(cross3(sd.f(), al), k.beta);
+        apos = apos + sd.H(sc3(cross3(al, e), k.beta));
     });
-    a.aP[i] = f4of(f3of(a.aP[i]) + apos + sc3(av, P.inv_dt));
-    a.aX[i] = f4of(sc3(av, -P.inv_dt));
+    st(a.aP, i, ld(a.aP, i) + apos + sc3(av, k.inv_dt));
+    st(a.aX, i, sc3(av, -k.inv_dt));
 }
 
 // Without vorticity: the update only.
 __global__ __launch_bounds__(256) void k_adj_update(StepTape t, AdjBufs a, Phys P) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
-    const F3 av = f3of(a.aVc[i]);
-    a.aP[i] = f4of(f3of(a.aP[i]) + sc3(av, P.inv_dt));
-    a.aX[i] = f4of(sc3(av, -P.inv_dt));
+    const D3 av = ld(a.aVc, i);
+    const double inv_dt = P.inv_dt;
+    st(a.aP, i, ld(a.aP, i) + sc3(av, inv_dt));
+    st(a.aX, i, sc3(av, -inv_dt));
 }
 
 // ------------------------------------------------------------ Jacobi iteration
@@ -467,11 +540,12 @@ __global__ __launch_bounds__(256) void k_adj_update(StepTape t, AdjBufs a, Phys 
 // planes: pp - ((pp - P).n) n, Jacobian I - n n^T (both cases of
 // pbf_plane_boundary.cu:6-45 reduce to it); spheres (pbf_sphere_boundary.cu:8-64):
 // pp + nudge cd - R cd/|cd| with cd = c - pp, Jacobian
-// (1 - nudge) I + (R/|cd|)(I - u u^T), u = cd/|cd|.  The decisions follow
-// collide() exactly (same arithmetic), J accumulates the product.
-__device__ void collide_jac(const Phys& P, const Boundary& B, F3& dp, F3 pk, float J[9]) {
-    auto apply = [&](const float M[9]) {
-        float T[9];
+// (1 - nudge) I + (R/|cd|)(I - u u^T), u = cd/|cd|.  The decisions and the
+// points follow collide() exactly (FP32, same arithmetic); J (FP64)
+// accumulates the product of the Jacobians of the responses that fired.
+__device__ void collide_jac(const Phys& P, const Boundary& B, F3& dp, F3 pk, double J[9]) {
+    auto apply = [&](const double M[9]) {
+        double T[9];
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) T[3 * r + c] = M[3 * r] * J[c] + M[3 * r + 1] * J[3 + c] + M[3 * r + 2] * J[6 + c];
         for (int q = 0; q < 9; ++q) J[q] = T[q];
@@ -482,34 +556,36 @@ __device__ void collide_jac(const Phys& P, const Boundary& B, F3& dp, F3 pk, flo
         ++sweeps;
         F3 pp = pred_x + dp;
         bool hit = false;
-        for (uint32_t k = 0; k < B.n_inner + B.n_outer; ++k) {
-            const bool is_inner = k < B.n_inner;
-            const float4 s = is_inner ? B.inner[k] : B.outer[k - B.n_inner];
+        for (uint32_t q = 0; q < B.n_inner + B.n_outer; ++q) {
+            const bool is_inner = q < B.n_inner;
+            const float4 s = is_inner ? B.inner[q] : B.outer[q - B.n_inner];
             const bool inner = is_inner || P.outer_as_inner != 0;
-            const F3 cd = mk3(s.x, s.y, s.z) - pp;
-            const float d2 = fdot(cd, cd);
+            const F3 cdf = mk3(s.x, s.y, s.z) - pp;
+            const float d2 = fdot(cdf, cdf);
             if (inner ? (d2 > s.w * s.w) : (d2 < s.w * s.w)) {
-                const float L = __builtin_sqrtf(dot3(cd, cd));
-                const F3 u = sc3(cd, 1.f / L);
-                const float nudge = inner ? 1.0001f : 0.9999f;
-                const float c0 = 1.f - nudge, c1 = s.w / L;
-                const float M[9] = {c0 + c1 * (1.f - u.x * u.x), -c1 * u.x * u.y, -c1 * u.x * u.z,
-                                    -c1 * u.y * u.x, c0 + c1 * (1.f - u.y * u.y), -c1 * u.y * u.z,
-                                    -c1 * u.z * u.x, -c1 * u.z * u.y, c0 + c1 * (1.f - u.z * u.z)};
+                const D3 cd = md3((double)s.x - pp.x, (double)s.y - pp.y, (double)s.z - pp.z);
+                const double L = sqrt(dot3(cd, cd));
+                const D3 u = sc3(cd, 1.0 / L);
+                const double nudge = inner ? (double)1.0001f : (double)0.9999f;
+                const double c0 = 1.0 - nudge, c1 = (double)s.w / L;
+                const double M[9] = {c0 + c1 * (1.0 - u.x * u.x), -c1 * u.x * u.y, -c1 * u.x * u.z,
+                                     -c1 * u.y * u.x, c0 + c1 * (1.0 - u.y * u.y), -c1 * u.y * u.z,
+                                     -c1 * u.z * u.x, -c1 * u.z * u.y, c0 + c1 * (1.0 - u.z * u.z)};
                 apply(M);
             }
             response_sphere(hit, dp, pp, mk3(s.x, s.y, s.z), s.w, inner);
             pp = pred_x + dp;
         }
-        for (uint32_t k = 0; k < B.n_planes; ++k) {
-            const F3 Pp = mk3(B.pts[3 * k], B.pts[3 * k + 1], B.pts[3 * k + 2]);
-            const F3 n = mk3(B.nrm[3 * k], B.nrm[3 * k + 1], B.nrm[3 * k + 2]);
+        for (uint32_t q = 0; q < B.n_planes; ++q) {
+            const F3 Pp = mk3(B.pts[3 * q], B.pts[3 * q + 1], B.pts[3 * q + 2]);
+            const F3 nf = mk3(B.nrm[3 * q], B.nrm[3 * q + 1], B.nrm[3 * q + 2]);
             bool h = false;
-            response_plane(h, dp, old_x, pp, Pp, n);
+            response_plane(h, dp, old_x, pp, Pp, nf);
             if (h) {
                 hit = true;
-                const float M[9] = {1.f - n.x * n.x, -n.x * n.y, -n.x * n.z, -n.y * n.x, 1.f - n.y * n.y,
-                                    -n.y * n.z, -n.z * n.x, -n.z * n.y, 1.f - n.z * n.z};
+                const D3 n = wid3(nf);
+                const double M[9] = {1.0 - n.x * n.x, -n.x * n.y, -n.x * n.z, -n.y * n.x, 1.0 - n.y * n.y,
+                                     -n.y * n.z, -n.z * n.x, -n.z * n.y, 1.0 - n.z * n.z};
                 apply(M);
             }
             pp = pred_x + dp;
@@ -519,12 +595,13 @@ __device__ void collide_jac(const Phys& P, const Boundary& B, F3& dp, F3 pk, flo
 }
 
 // dL/d(pre-collision position) = J^T dL/dP'.  dp is recomputed exactly as
-// k_deltap computes it, so the responses replay the forward's decisions.
+// k_deltap computes it (FP32), so the responses replay the forward's
+// decisions.
 __global__ __launch_bounds__(256) void k_adj_coll(StepTape t, AdjBufs a, const float4* __restrict__ Q, Phys P,
                                                   Boundary B, uint32_t cap) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
-    F3 ag = f3of(a.aP[i]);
+    D3 ag = ld(a.aP, i);
     if (B.n_planes + B.n_inner + B.n_outer > 0) {
         const float4 se = Q[i];
         const F3 self = f3of(se);
@@ -532,8 +609,8 @@ __global__ __launch_bounds__(256) void k_adj_coll(StepTape t, AdjBufs a, const f
         const float self_cl = (si < 0.f) ? -si : 0.f;
         const uint32_t n = t.ncnt[i];
         F3 sum = mk3(0.f, 0.f, 0.f);
-        for (uint32_t k = 0; k < n; ++k) {
-            const float4 pe = Q[t.nlist[(size_t)k * cap + i]];
+        for (uint32_t q = 0; q < n; ++q) {
+            const float4 pe = Q[t.nlist[(size_t)q * cap + i]];
             float W;
             F3 kg;
             pair_terms(self, f3of(pe), P, W, kg);
@@ -553,12 +630,12 @@ __global__ __launch_bounds__(256) void k_adj_coll(StepTape t, AdjBufs a, const f
             acc(sum, mulsv(aa, kg));
         }
         F3 dp = mk3(sum.x * P.m * P.inv_rho0, sum.y * P.m * P.inv_rho0, sum.z * P.m * P.inv_rho0);
-        float J[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+        double J[9] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
         collide_jac(P, B, dp, self, J);
-        ag = mk3(J[0] * ag.x + J[3] * ag.y + J[6] * ag.z, J[1] * ag.x + J[4] * ag.y + J[7] * ag.z,
+        ag = md3(J[0] * ag.x + J[3] * ag.y + J[6] * ag.z, J[1] * ag.x + J[4] * ag.y + J[7] * ag.z,
                  J[2] * ag.x + J[5] * ag.y + J[8] * ag.z);
     }
-    a.abar[i] = f4of(ag);
+    st(a.abar, i, ag);
 }
 
 // delta p = beta sum_j A_ij kg_ij, A_ij = -(s_i + s_j + corr(W_ij)):
@@ -568,57 +645,52 @@ __global__ __launch_bounds__(256) void k_adj_lam(StepTape t, AdjBufs a, const fl
                                                  uint32_t cap) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
-    const float beta = P.m * P.inv_rho0;
+    const AdjConst k = adj_const(P);
     const float4 se = Q[i];
-    const F3 self = f3of(se);
-    const float si = se.w;
-    const F3 ai = f3of(a.abar[i]);
-    float sb = 0.f, ks = 0.f, kg2s = 0.f;
-    F3 kgs = mk3(0.f, 0.f, 0.f), apos = mk3(0.f, 0.f, 0.f);
+    const D3 self = wid(se);
+    const double si = se.w;
+    const D3 ai = ld(a.abar, i);
+    double sb = 0.0, kg2s = 0.0;
+    D3 kgs = dz3(), apos = dz3();
     const uint32_t n = t.ncnt[i];
-    for (uint32_t k = 0; k < n; ++k) {
-        const float4 pe = Q[t.nlist[(size_t)k * cap + i]];
-        const F3 d = self - f3of(pe);
+    for (uint32_t q = 0; q < n; ++q) {
+        const float4 pe = Q[t.nlist[(size_t)q * cap + i]];
+        const D3 d = self - wid(pe);
         PairAdj pa;
-        if (!pair_adj(d, P, pa)) {
-            ks += P.a;  // r = 0: W = a, kg = 0
-            continue;
-        }
-        ks += pa.W;
+        if (!pair_adj(d, k, pa)) continue;  // r = 0: W = a (constant), kg = 0
         kgs = kgs + pa.kg;
         kg2s += dot3(pa.kg, pa.kg);
-        const float sj = pe.w;
-        const float A = -(si + sj + corr_of(P, pa.W, si, sj));
-        const float Ab = beta * dot3(pa.kg, ai);
+        const double sj = pe.w;
+        const double A = -(si + sj + corr_of(k, pa.W, si, sj));
+        const double Ab = k.beta * dot3(pa.kg, ai);
         sb -= Ab;
-        sb -= Ab * corr_ds(P, pa.W, si);
-        const float Wb = -Ab * corr_dW(P, pa.W, si, sj);
-        apos = apos + sc3(pa.H(ai), beta * A) + sc3(d, Wb * pa.gw);
+        sb -= Ab * corr_ds(k, pa.W, si);
+        const double Wb = -Ab * corr_dW(k, pa.W, si, sj);
+        apos = apos + sc3(pa.H(ai), k.beta * A) + sc3(d, Wb * pa.gw);
     }
     const uint32_t r1 = a.roff[i + 1];
     for (uint32_t q = a.roff[i]; q < r1; ++q) {
         const uint32_t l = a.rlist[q];
         const float4 pe = Q[l];
-        const F3 d = f3of(pe) - self;  // the pair (l, i): d = p_l - p_i
+        const D3 d = wid(pe) - self;  // the pair (l, i): d = p_l - p_i
         PairAdj pa;
-        if (!pair_adj(d, P, pa)) continue;
-        const float sl = pe.w;
-        const F3 al = f3of(a.abar[l]);
-        const float A = -(sl + si + corr_of(P, pa.W, sl, si));
-        const float Ab = beta * dot3(pa.kg, al);
+        if (!pair_adj(d, k, pa)) continue;
+        const double sl = pe.w;
+        const D3 al = ld(a.abar, l);
+        const double A = -(sl + si + corr_of(k, pa.W, sl, si));
+        const double Ab = k.beta * dot3(pa.kg, al);
         sb -= Ab;
-        sb -= Ab * corr_ds(P, pa.W, si);
-        const float Wb = -Ab * corr_dW(P, pa.W, sl, si);
-        apos = apos - (sc3(pa.H(al), beta * A) + sc3(d, Wb * pa.gw));
+        sb -= Ab * corr_ds(k, pa.W, si);
+        const double Wb = -Ab * corr_dW(k, pa.W, sl, si);
+        apos = apos - (sc3(pa.H(al), k.beta * A) + sc3(d, Wb * pa.gw));
     }
     // s = C / D, C = beta sum W - 1, D = m2r2 (|sum kg|^2 + sum |kg|^2) + eps
-    const float D = P.m2r2 * (dot3(kgs, kgs) + kg2s) + P.relax;
-    const float ksb = beta * sb / D;
-    const float g2 = -2.f * P.m2r2 * sb * si / D;
-    (void)ks;
-    a.coef[i] = make_float4(ksb, g2 * kgs.x, g2 * kgs.y, g2 * kgs.z);
+    const double D = k.m2r2 * (dot3(kgs, kgs) + kg2s) + k.relax;
+    const double ksb = k.beta * sb / D;
+    const double g2 = -2.0 * k.m2r2 * sb * si / D;
+    a.coef[i] = make_double4(ksb, g2 * kgs.x, g2 * kgs.y, g2 * kgs.z);
     a.g2[i] = g2;
-    a.aQ[i] = f4of(ai + apos);
+    st(a.aQ, i, ai + apos);
 }
 
 // lambda pass: the position terms through W and kg of s_i (own pairs) and
@@ -627,31 +699,32 @@ __global__ __launch_bounds__(256) void k_adj_pos(StepTape t, AdjBufs a, const fl
                                                  uint32_t cap) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
-    const F3 self = f3of(Q[i]);
-    const float4 ci = a.coef[i];
-    const float g2i = a.g2[i];
-    F3 apos = mk3(0.f, 0.f, 0.f);
-    if (ci.x != 0.f || g2i != 0.f) {
+    const AdjConst k = adj_const(P);
+    const D3 self = wid(Q[i]);
+    const double4 ci = a.coef[i];
+    const double g2i = a.g2[i];
+    D3 apos = dz3();
+    if (ci.x != 0.0 || g2i != 0.0) {
         const uint32_t n = t.ncnt[i];
-        for (uint32_t k = 0; k < n; ++k) {
-            const F3 d = self - f3of(Q[t.nlist[(size_t)k * cap + i]]);
+        for (uint32_t q = 0; q < n; ++q) {
+            const D3 d = self - wid(Q[t.nlist[(size_t)q * cap + i]]);
             PairAdj pa;
-            if (!pair_adj(d, P, pa)) continue;
-            apos = apos + sc3(d, ci.x * pa.gw) + pa.H(mk3(ci.y, ci.z, ci.w) + sc3(pa.kg, g2i));
+            if (!pair_adj(d, k, pa)) continue;
+            apos = apos + sc3(d, ci.x * pa.gw) + pa.H(md3(ci.y, ci.z, ci.w) + sc3(pa.kg, g2i));
         }
     }
     const uint32_t r1 = a.roff[i + 1];
     for (uint32_t q = a.roff[i]; q < r1; ++q) {
         const uint32_t l = a.rlist[q];
-        const float4 cl = a.coef[l];
-        const float g2l = a.g2[l];
-        if (cl.x == 0.f && g2l == 0.f) continue;
-        const F3 d = f3of(Q[l]) - self;
+        const double4 cl = a.coef[l];
+        const double g2l = a.g2[l];
+        if (cl.x == 0.0 && g2l == 0.0) continue;
+        const D3 d = wid(Q[l]) - self;
         PairAdj pa;
-        if (!pair_adj(d, P, pa)) continue;
-        apos = apos - (sc3(d, cl.x * pa.gw) + pa.H(mk3(cl.y, cl.z, cl.w) + sc3(pa.kg, g2l)));
+        if (!pair_adj(d, k, pa)) continue;
+        apos = apos - (sc3(d, cl.x * pa.gw) + pa.H(md3(cl.y, cl.z, cl.w) + sc3(pa.kg, g2l)));
     }
-    a.aP[i] = f4of(f3of(a.aQ[i]) + apos);
+    st(a.aP, i, ld(a.aQ, i) + apos);
 }
 
 // ------------------------------------------------------------ predict + sort
@@ -660,11 +733,11 @@ __global__ __launch_bounds__(256) void k_adj_in(StepTape t, AdjBufs a, Phys P) {
     const uint32_t s = gtid();
     if (s >= t.n_in) return;
     const uint32_t i = t.sidx[s];
-    F3 gx = mk3(0.f, 0.f, 0.f), gv = mk3(0.f, 0.f, 0.f);
+    D3 gx = dz3(), gv = dz3();
     if (s < t.n_live) {
-        const F3 ap = f3of(a.aP[s]);
-        gx = ap + f3of(a.aX[s]);
-        gv = sc3(ap, P.dt);
+        const D3 ap = ld(a.aP, s);
+        gx = ap + ld(a.aX, s);
+        gv = sc3(ap, (double)P.dt);
     }
     a.axin[3 * i] = gx.x;
     a.axin[3 * i + 1] = gx.y;
@@ -686,16 +759,21 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjBufs a, uint32_t nb) {
     }
 }
 
+// largest |axin|, |avin| (NaN counts as infinity), as the bits of a double
+// (non-negative doubles order like their bit patterns)
 __global__ __launch_bounds__(256) void k_adj_absmax(AdjBufs a, uint32_t n3) {
     const uint32_t q = gtid();
-    float m = 0.f;
-    if (q < n3) m = fmaxf(fabsf(a.axin[q]), fabsf(a.avin[q]));
-    if (q < n3 && (isnan(a.axin[q]) || isnan(a.avin[q]))) m = __builtin_inff();
-    for (int d = 32; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
-    if ((threadIdx.x & 63u) == 0 && m > 0.f) atomicMax(a.amax, __float_as_uint(m));
+    double m = 0.0;
+    if (q < n3) {
+        m = fmax(fabs(a.axin[q]), fabs(a.avin[q]));
+        if (isnan(a.axin[q]) || isnan(a.avin[q])) m = __builtin_inf();
+    }
+    for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63u) == 0 && m > 0.0)
+        atomicMax(a.amax, (unsigned long long)__double_as_longlong(m));
 }
 
-__global__ __launch_bounds__(256) void k_adj_scale(float* x, float* y, uint32_t n, float f) {
+__global__ __launch_bounds__(256) void k_adj_scale(double* x, double* y, uint32_t n, double f) {
     const uint32_t q = gtid();
     if (q >= n) return;
     if (x) x[q] *= f;
@@ -765,14 +843,14 @@ hipError_t launch_adj_step(const Bufs& b, const Scene& s0, const StepTape& t, co
 }
 
 hipError_t launch_adj_absmax(const AdjBufs& a, uint32_t n, hipStream_t s) {
-    hipError_t e = hipMemsetAsync(a.amax, 0, sizeof(uint32_t), s);
+    hipError_t e = hipMemsetAsync(a.amax, 0, sizeof(*a.amax), s);
     if (e != hipSuccess) return e;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_adj_absmax, dim3(nblk(3 * n)), dim3(256), 0, s, a, 3 * n);
     return hipGetLastError();
 }
 
-hipError_t launch_adj_scale(float* x, float* y, uint32_t n, float f, hipStream_t s) {
+hipError_t launch_adj_scale(double* x, double* y, uint32_t n, double f, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_adj_scale, dim3(nblk(n)), dim3(256), 0, s, x, y, n, f);
     return hipGetLastError();

@@ -32,6 +32,7 @@ struct SceneHost {
 struct AdjState {
     uint32_t max_steps = 0, every = 0, iters = 0, steps = 0;
     uint32_t n0 = 0, n_final = 0;
+    uint32_t stride = 0;              // list entries per particle the tapes hold
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     std::vector<float*> ck_x, ck_v;   // state at steps 0, every, 2*every, ... (device AoS)
     std::vector<uint32_t> ck_n;
@@ -42,7 +43,7 @@ struct AdjState {
     float* xw = nullptr;              // recompute state
     float* vw = nullptr;
     AdjBufs a{};
-    uint32_t* h_amax = nullptr;       // pinned
+    unsigned long long* h_amax = nullptr;  // pinned
     double* h_psum = nullptr;         // pinned [2]
 };
 
@@ -1030,6 +1031,7 @@ pbfx_status pbfx_adj_enable(pbfx_ctx* c, uint32_t max_steps, uint32_t every) {
     c->adj = A;
     A->max_steps = max_steps;
     A->every = every < max_steps ? every : max_steps;
+    A->stride = c->list_stride;
     const size_t cap = c->b.cap, CT = c->b.CT, nT = c->b.nT;
     const uint32_t nck = (max_steps + A->every - 1) / A->every;
     hipError_t e = hipSuccess;
@@ -1099,7 +1101,7 @@ pbfx_status pbfx_adj_enable(pbfx_ctx* c, uint32_t max_steps, uint32_t every) {
     if (e == hipSuccess) e = hipMemset(a.ctflag, 0, (nT + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(a.rcount, 0, ((size_t)a.RT + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(a.rtflag, 0, ((size_t)a.nRT + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&A->h_amax, sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&A->h_amax, sizeof(*A->h_amax), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&A->h_psum, 2 * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_adj(c);
@@ -1129,6 +1131,7 @@ pbfx_status pbfx_adj_forward(pbfx_ctx* c, float* x, float* v, uint32_t* num, con
     if (!c || !c->adj || c->batch || !x || !v || !num || !lo || !hi || iters < 1) return PBFX_INVALID_ARG;
     AdjState* A = c->adj;
     if (steps == 0 || steps > A->max_steps || *num > c->capacity) return PBFX_INVALID_ARG;
+    if (c->list_stride != A->stride) return PBFX_INVALID_ARG;  // max_pair changed since pbfx_adj_enable
     DeviceGuard g(c->device);
     const hipStream_t s = (hipStream_t)stream;
     pbfx_status st = adj_tape_iters(c, (uint32_t)iters);
@@ -1204,16 +1207,16 @@ static pbfx_status adj_record_step(pbfx_ctx* c, StepTape& t, uint32_t* n, hipStr
     return PBFX_OK;
 }
 
-pbfx_status pbfx_adj_backward(pbfx_ctx* c, const float* gx, const float* gv, float* gx0, float* gv0,
+pbfx_status pbfx_adj_backward(pbfx_ctx* c, const double* gx, const double* gv, double* gx0, double* gv0,
                               double gparam[2], int32_t* exponent, void* stream) {
     if (!c || !c->adj || !gx || !gv || !gx0 || !gv0 || !gparam || !exponent) return PBFX_INVALID_ARG;
     AdjState* A = c->adj;
-    if (A->steps == 0) return PBFX_INVALID_ARG;
+    if (A->steps == 0 || c->list_stride != A->stride) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);
     const hipStream_t s = (hipStream_t)stream;
     AdjBufs a = A->a;
-    hipError_t e = hipMemcpyAsync(a.ax, gx, (size_t)A->n_final * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(a.av, gv, (size_t)A->n_final * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+    hipError_t e = hipMemcpyAsync(a.ax, gx, (size_t)A->n_final * 3 * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(a.av, gv, (size_t)A->n_final * 3 * sizeof(double), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(c, e);
     double gp[2] = {0.0, 0.0};
     int32_t E = 0;
@@ -1234,17 +1237,17 @@ pbfx_status pbfx_adj_backward(pbfx_ctx* c, const float* gx, const float* gv, flo
             e = launch_adj_step(c->b, c->last_table[0], T, a, (int)A->iters, s);
             if (e == hipSuccess) e = launch_adj_absmax(a, T.n_in, s);
             if (e == hipSuccess) e = hipMemcpyAsync(A->h_psum, a.psum, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(A->h_amax, a.amax, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(A->h_amax, a.amax, sizeof(*a.amax), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) return hip_fail(c, e);
             gp[0] += A->h_psum[0];
             gp[1] += A->h_psum[1];
-            float m;
+            double m;
             std::memcpy(&m, A->h_amax, sizeof(m));
             // keep the adjoint near 1 once it grows past 2^20: scale by 2^-k
-            if (std::isfinite(m) && m > 1048576.f) {
+            if (std::isfinite(m) && m > 1048576.0) {
                 const int k = std::ilogb(m);
-                const float f = std::ldexp(1.f, -k);
+                const double f = std::ldexp(1.0, -k);
                 e = launch_adj_scale(a.axin, a.avin, 3 * T.n_in, f, s);
                 if (e != hipSuccess) return hip_fail(c, e);
                 gp[0] = std::ldexp(gp[0], -k);
@@ -1255,8 +1258,8 @@ pbfx_status pbfx_adj_backward(pbfx_ctx* c, const float* gx, const float* gv, flo
             std::swap(a.av, a.avin);
         }
     }
-    e = hipMemcpyAsync(gx0, a.ax, (size_t)A->n0 * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(gv0, a.av, (size_t)A->n0 * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+    e = hipMemcpyAsync(gx0, a.ax, (size_t)A->n0 * 3 * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(gv0, a.av, (size_t)A->n0 * 3 * sizeof(double), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e);
     gparam[0] = gp[0];

@@ -196,24 +196,25 @@ struct StepTape {
     float4* OM4;        // [cap]   (omega, |omega|) when vorticity ran
 };
 
-// Work arrays of the adjoint of one step.  Adjoints are FP32 and carry a
-// common power-of-two scale kept by the host.
+// Work arrays of the adjoint of one step, in FP64: the map's conditioning
+// (up to ~1e7 in exploding flows) would turn FP32 rounding into O(1) error.
+// The adjoint carries a common power-of-two scale kept by the host.
 struct AdjBufs {
     uint32_t cap;
-    float* ax;          // [3*cap] dL/d(step output x), output order
-    float* av;          // [3*cap] dL/d(step output v)
-    float* axin;        // [3*cap] dL/d(step input x), input order
-    float* avin;        // [3*cap] dL/d(step input v)
-    float4* aP;         // dL/d positions of the current stage
-    float4* aQ;         // dL/d positions, partial (pre-collision path + first pass)
-    float4* aV;         // dL/d velocity before confinement
-    float4* aVc;        // dL/d velocity after confinement
-    float4* aOm;        // dL/d vorticity
-    float4* etab;       // dL/d confinement location vector
-    float4* abar;       // dL/d pre-collision positions
-    float4* coef;       // (dL/d sum W, 2 dL/dG * sum kg) per particle
-    float* g2;          // 2 dL/dG per particle
-    float4* aX;         // dL/d sorted input positions
+    double* ax;         // [3*cap] dL/d(step output x), output order
+    double* av;         // [3*cap] dL/d(step output v)
+    double* axin;       // [3*cap] dL/d(step input x), input order
+    double* avin;       // [3*cap] dL/d(step input v)
+    double4* aP;        // dL/d positions of the current stage
+    double4* aQ;        // dL/d positions, partial (pre-collision path + first pass)
+    double4* aV;        // dL/d velocity before confinement
+    double4* aVc;       // dL/d velocity after confinement
+    double4* aOm;       // dL/d vorticity
+    double4* etab;      // dL/d confinement location vector
+    double4* abar;      // dL/d pre-collision positions
+    double4* coef;      // (dL/d sum W, 2 dL/dG * sum kg) per particle
+    double* g2;         // 2 dL/dG per particle
+    double4* aX;        // dL/d sorted input positions
     uint32_t* xnan;     // XSPH NaN guard per slot
     // particles grouped by final cell: the transposed 27-cell sets
     uint32_t *ckey, *crank, *ctmp, *cperm, *ccount, *coffset, *ctsum, *ctflag, *ctprefix;
@@ -223,7 +224,7 @@ struct AdjBufs {
     uint32_t RT, lgRT, nRT;   // padded slot space of the reverse-count scan
     double* part;       // [2 * kAdjMaxBlocks] per-workgroup partial sums of dL/dc, dL/d eps
     double* psum;       // [2]
-    uint32_t* amax;     // [1] largest |adjoint| of the step input (float bits)
+    unsigned long long* amax;  // [1] largest |adjoint| of the step input, as the bits of a double
 };
 constexpr uint32_t kAdjMaxBlocks = 4096;
 
@@ -258,10 +259,10 @@ void size_unit_grids(Tuning& T, int num_cu);
 // a.psum, for the recorded step t of the single scene s0.
 hipError_t launch_adj_step(const Bufs& b, const Scene& s0, const StepTape& t, const AdjBufs& a, int iters,
                            hipStream_t s);
-// a.amax = max |a.axin|, |a.avin| over n rows (as float bits).
+// a.amax = max |a.axin|, |a.avin| over n rows (as double bits).
 hipError_t launch_adj_absmax(const AdjBufs& a, uint32_t n, hipStream_t s);
 // x[0..n) *= f for each of the arrays (may be NULL).
-hipError_t launch_adj_scale(float* x, float* y, uint32_t n, float f, hipStream_t s);
+hipError_t launch_adj_scale(double* x, double* y, uint32_t n, double f, hipStream_t s);
 // id_out[q] = id_in[sidx[q]] for q < n; id_in NULL: the identity.
 hipError_t launch_adj_ident(const uint32_t* id_in, const uint32_t* sidx, uint32_t* id_out, uint32_t n, hipStream_t s);
 

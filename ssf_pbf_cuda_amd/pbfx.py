@@ -358,9 +358,9 @@ class Solver:
         _check(self._lib.pbfx_adj_identity(self._ctx, _ptr(out), _stream(stream)), "pbfx_adj_identity")
 
     def adj_backward(self, gx, gv, gx0, gv0, stream=None):
-        """dL/d(final x, v) -> dL/d(initial x, v) into gx0, gv0; returns
-        (dL/dc, dL/d vorticity eps, exponent): every gradient is the value
-        returned times 2**exponent."""
+        """dL/d(final x, v) -> dL/d(initial x, v) into gx0, gv0 (device
+        float64, AoS); returns (dL/dc, dL/d vorticity eps, exponent): every
+        gradient is the value returned times 2**exponent."""
         gp = (ctypes.c_double * 2)()
         ex = ctypes.c_int32(0)
         _check(self._lib.pbfx_adj_backward(self._ctx, _ptr(gx), _ptr(gv), _ptr(gx0), _ptr(gv0), gp, ctypes.byref(ex),

@@ -81,22 +81,23 @@ def autograd_fn():
             n = roll.solver.adj_forward(x, v, x.shape[0], sc.area_lo, sc.area_hi, sc.iters, roll.steps)
             ident = torch.empty(n, dtype=torch.int32, device=x.device)
             roll.solver.adj_identity(ident)
-            out = torch.full_like(x0, float("nan"))
-            out[ident.long()] = x[:n]
+            # float64, so that dL/d(output) reaches the FP64 adjoint unrounded
+            out = torch.full(x0.shape, float("nan"), dtype=torch.float64, device=x0.device)
+            out[ident.long()] = x[:n].double()
             ctx.roll, ctx.ident, ctx.n = roll, ident, n
-            ctx.shape = x0.shape
+            ctx.shape, ctx.dtype = x0.shape, x0.dtype
             return out
 
         @staticmethod
         def backward(ctx, g):
             roll, ident = ctx.roll, ctx.ident.long()
-            gx = g[ident].contiguous().float()
+            gx = g[ident].contiguous().double()
             gv = torch.zeros_like(gx)
-            gx0 = torch.empty(ctx.shape, dtype=torch.float32, device=g.device)
+            gx0 = torch.empty(ctx.shape, dtype=torch.float64, device=g.device)
             gv0 = torch.empty_like(gx0)
             gc, ge, e = roll.solver.adj_backward(gx, gv, gx0, gv0)
             s = 2.0 ** e
-            return (gx0.double() * s).to(g.dtype), (gv0.double() * s).to(g.dtype), \
+            return (gx0 * s).to(ctx.dtype), (gv0 * s).to(ctx.dtype), \
                 torch.tensor(gc * s, dtype=torch.float64), torch.tensor(ge * s, dtype=torch.float64), None
 
     return PbfRollout

@@ -1,8 +1,8 @@
 """The differentiable form on the GPU (pbfx_adj_*): the forward it records is
-the oracle's step bit for bit, and its gradients -- recomputed window by
+the oracle's step bit for bit, and its FP64 gradients -- recomputed window by
 window from checkpoints -- agree with the oracle's FP64 adjoint evaluated at
-the same recorded values.  Gradients are compared as (values, power-of-two
-exponent) pairs, normwise."""
+the same recorded values (the two differ in summation order only).
+Gradients are compared as (values, power-of-two exponent) pairs, normwise."""
 from __future__ import annotations
 
 import dataclasses
@@ -15,9 +15,9 @@ from ssf_pbf_cuda_amd import scenes
 from tests.adjoint_cases import Rollout, autograd_fn, cases, linear_loss_weights, rel, rel_norm
 
 CASES = cases()
-# FP32 adjoint vs FP64 adjoint on identical values, normwise
-TOL = {"calm": 1e-4, "reference_scorr": 1e-4, "spheres": 1e-4, "no_final_passes": 1e-4, "vorticity_only": 1e-4,
-       "exploding_1step": 1e-2}
+# GPU FP64 adjoint vs oracle FP64 adjoint on identical values, normwise
+# (they differ in summation order only)
+TOL = 1e-9
 
 
 def _gpu_grad(sc, steps, every, wx_fn):
@@ -31,13 +31,13 @@ def _gpu_grad(sc, steps, every, wx_fn):
     roll.solver.adj_identity(ident)
     xs, vs = x[:n].cpu().numpy(), v[:n].cpu().numpy()
     wx, wv = wx_fn(xs, ident.cpu().numpy().astype(np.int64))
-    gx = torch.from_numpy(np.ascontiguousarray(wx, np.float32)).cuda()
-    gv = torch.from_numpy(np.ascontiguousarray(wv, np.float32)).cuda()
-    gx0 = torch.empty_like(x)
-    gv0 = torch.empty_like(v)
+    gx = torch.from_numpy(np.ascontiguousarray(wx, np.float64)).cuda()
+    gv = torch.from_numpy(np.ascontiguousarray(wv, np.float64)).cuda()
+    gx0 = torch.empty(x.shape, dtype=torch.float64, device="cuda")
+    gv0 = torch.empty_like(gx0)
     gc, ge, e = roll.solver.adj_backward(gx, gv, gx0, gv0)
-    out = dict(x=xs, v=vs, ident=ident.cpu().numpy(), gx0=gx0.cpu().numpy().astype(np.float64),
-               gv0=gv0.cpu().numpy().astype(np.float64), gc=gc, ge=ge, e=e, wx=wx, wv=wv)
+    out = dict(x=xs, v=vs, ident=ident.cpu().numpy(), gx0=gx0.cpu().numpy(), gv0=gv0.cpu().numpy(), gc=gc, ge=ge,
+               e=e, wx=wx, wv=wv)
     roll.solver.close()
     return out
 
@@ -76,7 +76,7 @@ def test_gpu_adjoint_matches_fp64_adjoint(name):
     assert np.array_equal(g["ident"], c["ident"])
     err = _compare(name, g, c)
     for k, e in err.items():
-        assert e < TOL[name], (name, k, e)
+        assert e < TOL, (name, k, e)
 
 
 @pytest.mark.gpu
@@ -114,7 +114,7 @@ def test_autograd_wrapper_against_fp64_adjoint():
     c = torch.tensor(float(np.float32(sc.params.xsph_parameter)), dtype=torch.float64, requires_grad=True)
     eps = torch.tensor(float(np.float32(sc.params.vc_parameter)), dtype=torch.float64, requires_grad=True)
     xf = autograd_fn().apply(x0, v0, c, eps, roll)
-    loss = ((xf - torch.from_numpy(target).cuda()) ** 2).sum() / sc.n
+    loss = ((xf - torch.from_numpy(target).double().cuda()) ** 2).sum() / sc.n
     loss.backward()
     roll.solver.close()
     # the same loss through the FP64 adjoint
@@ -126,8 +126,10 @@ def test_autograd_wrapper_against_fp64_adjoint():
     errs = {"x0": rel_norm(x0.grad.cpu().numpy(), gx0), "v0": rel_norm(v0.grad.cpu().numpy(), gv0),
             "c": rel(c.grad.item(), gc), "eps": rel(eps.grad.item(), ge)}
     print("autograd", f"loss {loss.item():.4e}", {k: f"{e:.2e}" for k, e in errs.items()}, flush=True)
+    # x0.grad and v0.grad are stored as float32 (the dtype of x0, v0)
+    tol = {"x0": 1e-6, "v0": 1e-6, "c": 1e-9, "eps": 1e-9}
     for k, e in errs.items():
-        assert e < 1e-4, (k, e)
+        assert e < tol[k], (k, e)
 
 
 @pytest.mark.gpu
@@ -155,5 +157,9 @@ def test_workload_65k_40_steps_against_fp64_adjoint():
     c = _cpu_grad(sc, steps, g["wx"], g["wv"])
     assert np.array_equal(g["x"].view(np.uint32), c["x"].view(np.uint32))
     err = _compare("workload", g, c)
-    assert abs(g["e"] - c["e"]) <= 1
     assert np.all(np.isfinite(g["gx0"])) and np.all(np.isfinite(g["gv0"]))
+    assert abs(g["e"] - c["e"]) <= 1
+    # two FP64 evaluations in different summation orders agree to ~1e-9 on
+    # this workload (its conditioning is ~1e7)
+    for k, e in err.items():
+        assert e < 1e-5, (k, e)

@@ -5,8 +5,10 @@ A step is one full pbf::one_step over the whole particle set: predict,
 hashed-grid neighbour search, I Jacobi iterations, velocity update,
 vorticity confinement and XSPH.  Default workload: the 1,048,576-particle
 dam break (128x128x64 lattice, spacing 0.5h, h = 0.05, 256x128x128 cells, 4
-Jacobi iterations); the 8,388,608-particle double dam break (3 iterations) is
-reported beside it.  Inputs are resident in HBM before the timed region.
+Jacobi iterations); the 8,388,608-particle double dam break (3 iterations),
+the 256-scene batch and the differentiable 65,536-particle workload (40
+steps forward + backward) are reported beside it.  Inputs are resident in HBM
+before the timed region.
 
 Prints ONE JSON line.
 """
@@ -57,6 +59,8 @@ def parse():
     ap.add_argument("--steps-8m", type=int, default=200, help="timed steps of the 8M workload (0: skip)")
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps of the per-kernel event pass")
     ap.add_argument("--batch-steps", type=int, default=10, help="timed steps of the 256-scene batch (0: skip)")
+    ap.add_argument("--adjoint-runs", type=int, default=2,
+                    help="timed forward+backward runs of the differentiable workload (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--tile-cap", type=int, default=None, help="LDS tile (particles); default: autotuned")
     ap.add_argument("--unit-width", type=int, default=16)
@@ -224,6 +228,104 @@ def run_batch(scs, steps, warmup):
             "tuning": {"tile_cap": tune[0], "unit_width": tune[1]}}
 
 
+def run_adjoint(runs, steps=40, every=10):
+    """The differentiable workload: 65,536 particles (64x32x32, h = 0.1,
+    v0 ~ N(0, 0.1^2)), `steps` steps with a checkpoint every `every`; loss =
+    mean squared distance by particle to the run with XSPH c = 0.02 and
+    vorticity eps = 0.0008; gradients with respect to x0, v0, c and eps.
+    Times pbfx_adj_forward and pbfx_adj_backward (window recomputation from
+    the checkpoints included)."""
+    import dataclasses
+
+    import torch
+
+    from ssf_pbf_cuda_amd import pbfx, scenes
+
+    sc = scenes.config4()
+    tgt = dataclasses.replace(sc, params=dataclasses.replace(sc.params, xsph_parameter=0.02, vc_parameter=0.0008))
+    x0 = torch.from_numpy(sc.x).cuda()
+    v0 = torch.from_numpy(sc.v).cuda()
+
+    def forward(solver, x, v):
+        n = solver.adj_forward(x, v, sc.n, sc.area_lo, sc.area_hi, sc.iters, steps)
+        ident = torch.empty(n, dtype=torch.int32, device="cuda")
+        solver.adj_identity(ident)
+        return n, ident.long()
+
+    st = pbfx.Solver.for_scene(tgt)
+    st.adj_enable(steps, every)
+    xt, vt = x0.clone(), v0.clone()
+    nt, it = forward(st, xt, vt)
+    target = torch.empty_like(x0)
+    target[it] = xt[:nt]
+    st.close()
+    solver = pbfx.Solver.for_scene(sc)
+    solver.adj_enable(steps, every)
+    gx0 = torch.empty(x0.shape, dtype=torch.float64, device="cuda")
+    gv0 = torch.empty_like(gx0)
+    t_f, t_b, res = [], [], None
+    for r in range(runs + 1):  # the first run warms up
+        x, v = x0.clone(), v0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n, ident = forward(solver, x, v)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        d = (x[:n].double() - target[ident].double())
+        loss = float((d ** 2).sum() / sc.n)
+        gx = (2.0 / sc.n) * d
+        gv = torch.zeros_like(gx)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        gc, ge, e = solver.adj_backward(gx, gv, gx0, gv0)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if r:
+            t_f.append(t1 - t0)
+            t_b.append(t3 - t2)
+        res = (loss, gc, ge, e)
+        log(f"adjoint run {r}: forward {1e3 * (t1 - t0):.1f} ms, backward {1e3 * (t3 - t2):.1f} ms")
+    solver.close()
+    tf, tb = min(t_f), min(t_b)
+    loss, gc, ge, e = res
+    return {
+        "workload": f"{sc.name}: {sc.n} particles, {steps} steps, checkpoints every {every}, {sc.iters} "
+                    f"iterations; loss = mean |x - x_target|^2 (target: c = 0.02, eps = 0.0008); gradients "
+                    f"w.r.t. x0, v0, c, eps (FP64 adjoint at the FP32 forward's values)",
+        "value": round(sc.n * steps / (tf + tb) / 1e6, 3),
+        "unit": "M particle-steps/s (forward + backward)",
+        "forward_ms": round(1e3 * tf, 2),
+        "backward_ms": round(1e3 * tb, 2),
+        "backward_over_forward": round(tb / tf, 2),
+        "loss": loss,
+        "grad_exponent": e,
+        "dL_dc": f"{gc:.6e} * 2^{e}",
+        "dL_deps": f"{ge:.6e} * 2^{e}",
+        "runs": runs,
+    }
+
+
+def cpu_adjoint(steps):
+    """The oracle's FP64 adjoint (host C++, OpenMP) on the same workload,
+    over its first `steps` steps: FP32 forward with tape + FP64 backward."""
+    from oracle import pyoracle as orc
+    from ssf_pbf_cuda_amd import scenes
+
+    sc = scenes.config4()
+    T = orc.Tape(sc)
+    t0 = time.perf_counter()
+    x, _ = T.run(steps)
+    gx = np.zeros((x.shape[0], 3), np.float64)
+    gx[:, 0] = 1.0 / sc.n
+    T.backward(gx, np.zeros_like(gx))
+    el = time.perf_counter() - t0
+    T.close()
+    threads = orc.get_threads()
+    return {"value": round(sc.n * steps / el / 1e6, 4), "unit": "M particle-steps/s (forward + backward)",
+            "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of {sc.name} forward + FP64 adjoint, {el:.1f} s, OpenMP threads={threads}"}
+
+
 def kernel_report(kt, n, cells):
     rows = {}
     for k, (tot, cnt) in kt.items():
@@ -351,9 +453,13 @@ def main():
             "tuning": rb["tuning"],
             "steps": a.batch_steps,
         }
+    if a.adjoint_runs:
+        out["differentiable"] = run_adjoint(a.adjoint_runs)
     if a.cpu_seconds > 0:
         log(f"cpu baseline ({a.cpu_seconds:.0f} s budget)")
         out["cpu_baseline"] = cpu_baseline(sc, a.cpu_seconds)
+        if a.adjoint_runs:
+            out["differentiable"]["cpu_baseline"] = cpu_adjoint(10)
     if a.calibrate_copy_mb:
         src = torch.ones(a.calibrate_copy_mb * (1 << 18), dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

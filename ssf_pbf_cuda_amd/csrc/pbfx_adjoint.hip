@@ -204,6 +204,8 @@ __device__ __forceinline__ double corr_ds(const AdjConst& k, double W, double s)
     return -10.0 * W2 * W2 * k.inv_k;
 }
 
+__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
 // the 27 search cells around the cell of p, in z, y, x order
 template <class F>
 __device__ __forceinline__ void for_cells(const Grid& G, F3 p, F&& f) {
@@ -214,36 +216,110 @@ __device__ __forceinline__ void for_cells(const Grid& G, F3 p, F&& f) {
             for (int dx = -1; dx <= 1; ++dx) f(grid_hash(g[0] + dx, g[1] + dy, g[2] + dz, G), dx, dy, dz);
 }
 
+// How a thread takes part in a pass over a particle's candidate set: the
+// light kernel runs one thread per particle not flagged heavy (bit), the
+// heavy kernel one wave per listed particle, its lanes taking every 64th
+// candidate and reducing their sums in a fixed order.
+struct Role {
+    uint32_t i, lane, stride;
+    bool on;
+};
+template <bool kHeavy>
+__device__ __forceinline__ Role role(const StepTape& t, const AdjBufs& a, uint32_t bit) {
+    Role r;
+    if (!kHeavy) {
+        r.i = gtid();
+        r.lane = 0;
+        r.stride = 1;
+        r.on = r.i < t.n_live && !(a.hflag[r.i] & bit);
+    } else {
+        const uint32_t w = gtid() >> 6;
+        const uint32_t set = (uint32_t)__builtin_ctz(bit);
+        r.lane = threadIdx.x & 63u;
+        r.stride = 64;
+        r.on = w < a.nh[set];
+        r.i = r.on ? a.hlist[(size_t)set * a.cap + w] : 0u;
+    }
+    return r;
+}
+__device__ __forceinline__ void mark_heavy(const AdjBufs& a, uint32_t i, uint32_t set) {
+    a.hlist[(size_t)set * a.cap + atomicAdd(&a.nh[set], 1u)] = i;
+}
+// sum over the wave (the heavy kernels; every lane holds the total)
+__device__ __forceinline__ double wsum(double v) {
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ __forceinline__ D3 wsum3(D3 v) { return md3(wsum(v.x), wsum(v.y), wsum(v.z)); }
+
 // S(i): the slots of the 27 cells around the final cell of p (the forward
-// sum's candidates, pbf_contribution.inl:7-57)
+// sum's candidates, pbf_contribution.inl:7-57); this thread's share.
 template <class F>
-__device__ __forceinline__ void for_S(const StepTape& t, const Grid& G, F3 p, F&& f) {
+__device__ __forceinline__ void for_S(const StepTape& t, const Grid& G, F3 p, const Role& r, F&& f) {
     for_cells(G, p, [&](uint32_t c, int, int, int) {
         const uint32_t e = t.offset[c + 1];
-        for (uint32_t j = t.offset[c]; j < e; ++j) f(j);
+        for (uint32_t j = t.offset[c] + r.lane; j < e; j += r.stride) f(j);
     });
 }
 // T(i) = {l : i in S(l)}: the particles whose final cell is one of the 27
-// cells around the search cell of i (p0 = i's predicted position)
+// cells around the search cell of i (p0 = i's predicted position); this
+// thread's share.  f gets the position q in final-cell order (slot
+// l = cperm[q]); the pass's data of l sit at gP[q], gV[q], gD[q].
 template <class F>
-__device__ __forceinline__ void for_T(const AdjBufs& a, const Grid& G, F3 p0, F&& f) {
+__device__ __forceinline__ void for_T(const AdjBufs& a, const Grid& G, F3 p0, const Role& r, F&& f) {
     for_cells(G, p0, [&](uint32_t c, int, int, int) {
         const uint32_t e = a.coffset[c + 1];
-        for (uint32_t q = a.coffset[c]; q < e; ++q) f(a.cperm[q]);
+        for (uint32_t q = a.coffset[c] + r.lane; q < e; q += r.stride) f(q);
     });
 }
 
-__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// Candidate counts of the final passes' sets and of the search cells;
+// particles above kAdjHeavy are flagged and listed for the heavy kernels
+// (the reverse-list bit is set by k_adj_rev).  Also clears the per-particle
+// shares of the parameter gradients.
+__global__ __launch_bounds__(256) void k_adj_classify(StepTape t, AdjBufs a, Grid G) {
+    const uint32_t i = gtid();
+    if (i >= t.n_live) return;
+    uint32_t cs = 0, ct = 0, cn = 0;
+    for_cells(G, f3of(t.PI[i]), [&](uint32_t c, int, int, int) { cs += t.offset[c + 1] - t.offset[c]; });
+    for_cells(G, f3of(t.Q[i]), [&](uint32_t c, int, int, int) {
+        ct += a.coffset[c + 1] - a.coffset[c];
+        cn += t.offset[c + 1] - t.offset[c];
+    });
+    uint32_t f = 0;
+    const uint32_t cnt[3] = {cs, ct, cn};
+    for (uint32_t set = 0; set < 3; ++set)
+        if (cnt[set] > kAdjHeavy) {
+            f |= 1u << set;
+            mark_heavy(a, i, set);
+        }
+    a.hflag[i] = f;
+    a.pc[i] = 0.0;
+    a.pe[i] = 0.0;
+}
 
-// sum of v over the workgroup (256 threads), valid in thread 0
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
+// Gathers what a transposed pass reads of each l into final-cell order:
+// gP = (final position, slot bits) and, by pass,
+//   0 XSPH:        gV = (velocity after confinement, NaN guard), gD = dL/dv_out
+//   1 confinement: gD = dL/deta
+//   2 vorticity:   gV = velocity before confinement (as the forward forms it), gD = dL/domega
+__global__ __launch_bounds__(256) void k_adj_gather(StepTape t, AdjBufs a, Phys P, uint32_t cap, int pass) {
+    const uint32_t q = gtid();
+    if (q >= t.n_live) return;
+    const uint32_t l = a.cperm[q];
+    const float4 p = t.PI[l];
+    a.gP[q] = make_float4(p.x, p.y, p.z, __uint_as_float(l));
+    if (pass == 0) {
+        const float4 vc = t.R[2 * l + 1];
+        a.gV[q] = make_float4(vc.x, vc.y, vc.z, a.xnan[l] ? 1.f : 0.f);
+        a.gD[q] = make_double4(a.av[3 * l], a.av[3 * l + 1], a.av[3 * l + 2], 0.0);
+    } else if (pass == 1) {
+        a.gD[q] = a.etab[l];
+    } else {
+        const F3 v = mulsv(P.inv_dt, f3of(p) - ld3(t.X, cap, l));
+        a.gV[q] = make_float4(v.x, v.y, v.z, 0.f);
+        a.gD[q] = a.aOm[l];
     }
-    return sh[0];
 }
 
 // ------------------------------------------------------------ grouping by final cell
@@ -307,228 +383,301 @@ __global__ __launch_bounds__(256) void k_adj_last(StepTape t, AdjBufs a, Grid G,
 // The slots l != j whose frozen list holds j, in j's 27-cell scan order
 // (pbf_neighbor_search_util.cu:386-408,428-452: l lists the candidates with
 // r^2 < h^2 in its own scan order until the list is full).  fill = false:
-// count into rcount; fill = true: write rlist.
-template <bool fill>
+// count into rcount (and flag reverse lists above kAdjHeavy); fill = true:
+// write rlist.  A heavy wave takes the candidates 64 at a time and keeps the
+// scan order with a ballot.
+template <bool fill, bool kHeavy>
 __global__ __launch_bounds__(256) void k_adj_rev(StepTape t, AdjBufs a, Grid G, Phys P) {
-    const uint32_t j = gtid();
-    if (j >= t.n_live) return;
+    const Role r = role<kHeavy>(t, a, 4u);
+    if (!r.on) return;
+    const uint32_t j = r.i;
     const F3 self = f3of(t.Q[j]);
     const uint32_t full = (uint32_t)P.cap;
     uint32_t cnt = 0;
     uint32_t* out = fill ? a.rlist + a.roff[j] : nullptr;
     for_cells(G, self, [&](uint32_t c, int dx, int dy, int dz) {
-        const uint32_t e = t.offset[c + 1];
+        const uint32_t b0 = t.offset[c], e = t.offset[c + 1];
         const uint32_t pos = (uint32_t)((1 - dz) * 9 + (1 - dy) * 3 + (1 - dx));  // j seen from l
-        for (uint32_t l = t.offset[c]; l < e; ++l) {
-            if (l == j) continue;
-            const F3 d = f3of(t.Q[l]) - self;  // the search's d = p_l - p_j
-            const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
-            if (!(r2 < P.h2)) continue;
-            const uint32_t n = t.ncnt[l];
-            bool listed = n > 0 && n < full;
-            if (n == full) {
-                const uint2 L = a.last[l];
-                listed = pos < L.x || (pos == L.x && j <= L.y);
+        for (uint32_t base = b0; base < e; base += r.stride) {
+            const uint32_t l = base + r.lane;
+            bool listed = false;
+            if (l < e && l != j) {
+                const F3 d = f3of(t.Q[l]) - self;  // the search's d = p_l - p_j
+                const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+                if (r2 < P.h2) {
+                    const uint32_t n = t.ncnt[l];
+                    listed = n > 0 && n < full;
+                    if (n == full) {
+                        const uint2 L = a.last[l];
+                        listed = pos < L.x || (pos == L.x && j <= L.y);
+                    }
+                }
             }
-            if (listed) {
-                if (fill) out[cnt] = l;
-                ++cnt;
+            if (!kHeavy) {
+                if (listed) {
+                    if (fill) out[cnt] = l;
+                    ++cnt;
+                }
+            } else {
+                const uint64_t m = __ballot(listed);
+                if (fill && listed) out[cnt + (uint32_t)__popcll(m & ((1ull << r.lane) - 1ull))] = l;
+                cnt += (uint32_t)__popcll(m);
             }
         }
     });
-    if (!fill) {
+    if (!fill && r.lane == 0) {
         a.rcount[j] = cnt;
         if (cnt) mark_tile(a.rtflag, j >> a.lgRT);
+        if (cnt > kAdjHeavy) {
+            atomicOr(&a.hflag[j], 8u);
+            mark_heavy(a, j, 3);
+        }
     }
 }
 
 // ------------------------------------------------------------ final stage
-// XSPH, own pairs: dL/dVc_i, the position terms of i's own sum, dL/dc.
-__global__ __launch_bounds__(256) void k_adj_xsph_own(StepTape t, AdjBufs a, Grid G, Phys P) {
-    __shared__ double sh[256];
-    const uint32_t i = gtid();
-    double pc = 0.0;
-    if (i < t.n_live) {
-        const AdjConst k = adj_const(P);
-        const F3 selff = f3of(t.PI[i]);
-        const D3 self = wid3(selff), vi = wid(t.R[2 * i + 1]);
-        const D3 gi = ld3d(a.av, i);
-        double sumW = 0.0;
-        D3 acc = dz3(), apos = dz3();
-        for_S(t, G, selff, [&](uint32_t j) {
-            if (j == i) return;
-            const D3 pd = wid(t.PI[j]) - self;
-            if (dot3(pd, pd) > (double)P.cut2_w) return;
-            double W, gw;
-            if (!poly6_adj(pd, k, W, gw)) return;
-            const D3 e = wid(t.R[2 * j + 1]) - vi;
-            acc = acc + sc3(e, W);
-            sumW += W;
-            apos = apos - sc3(pd, gw * k.gamma * dot3(gi, e));
-        });
-        const bool nan = isnan(acc.x) || isnan(acc.y) || isnan(acc.z);
-        a.xnan[i] = nan ? 1u : 0u;
-        D3 avc = gi;
-        if (nan) {
-            apos = dz3();
-        } else {
-            avc = sc3(gi, 1.0 - k.gamma * sumW);
-            pc = k.beta * dot3(gi, acc);
-        }
-        st(a.aVc, i, avc);
-        st(a.aP, i, ld3d(a.ax, i) + apos);
-    }
-    const double tot = block_sum(pc, sh);
-    if (threadIdx.x == 0) a.part[blockIdx.x] = tot;
-}
+// Every pass below has a light (one thread per particle) and a heavy (one
+// wave per particle) instantiation over the same body.
 
-// XSPH, transposed pairs (l, i), i in S(l).
-__global__ __launch_bounds__(256) void k_adj_xsph_rev(StepTape t, AdjBufs a, Grid G, Phys P) {
+// Clears dL/dVc = dL/dv_out, dL/dP = dL/dx_out (the stage's starting point).
+__global__ __launch_bounds__(256) void k_adj_final_init(StepTape t, AdjBufs a) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
+    st(a.aVc, i, ld3d(a.av, i));
+    st(a.aP, i, ld3d(a.ax, i));
+}
+
+// XSPH, own pairs: dL/dVc_i, the position terms of i's own sum, dL/dc
+// (v_out = Vc + gamma sum_j (Vc_j - Vc_i) W(|p_j - p_i|)).
+template <bool kHeavy>
+__global__ __launch_bounds__(256) void k_adj_xsph_own(StepTape t, AdjBufs a, Grid G, Phys P) {
+    const Role r = role<kHeavy>(t, a, 1u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
     const AdjConst k = adj_const(P);
-    const D3 pi = wid(t.PI[i]), vi = wid(t.R[2 * i + 1]);
-    D3 avc = dz3(), apos = dz3();
-    for_T(a, G, f3of(t.Q[i]), [&](uint32_t l) {
-        if (l == i || a.xnan[l]) return;
-        const D3 pd = pi - wid(t.PI[l]);
+    const F3 selff = f3of(t.PI[i]);
+    const D3 self = wid3(selff), vi = wid(t.R[2 * i + 1]);
+    const D3 gi = ld3d(a.av, i);
+    double sumW = 0.0;
+    D3 acc = dz3(), apos = dz3();
+    for_S(t, G, selff, r, [&](uint32_t j) {
+        if (j == i) return;
+        const D3 pd = wid(t.PI[j]) - self;
         if (dot3(pd, pd) > (double)P.cut2_w) return;
         double W, gw;
         if (!poly6_adj(pd, k, W, gw)) return;
-        const D3 gl = ld3d(a.av, l);
-        avc = avc + sc3(gl, k.gamma * W);
-        apos = apos + sc3(pd, gw * k.gamma * dot3(gl, vi - wid(t.R[2 * l + 1])));
+        const D3 e = wid(t.R[2 * j + 1]) - vi;
+        acc = acc + sc3(e, W);
+        sumW += W;
+        apos = apos - sc3(pd, gw * k.gamma * dot3(gi, e));
     });
+    if (kHeavy) {
+        acc = wsum3(acc);
+        sumW = wsum(sumW);
+        apos = wsum3(apos);
+        if (r.lane) return;
+    }
+    const bool nan = isnan(acc.x) || isnan(acc.y) || isnan(acc.z);
+    a.xnan[i] = nan ? 1u : 0u;
+    if (nan) return;  // velocity kept: v_out = Vc
+    st(a.aVc, i, sc3(gi, 1.0 - k.gamma * sumW));
+    st(a.aP, i, ld(a.aP, i) + apos);
+    a.pc[i] = k.beta * dot3(gi, acc);
+}
+
+// XSPH, transposed pairs (l, i), i in S(l).
+template <bool kHeavy>
+__global__ __launch_bounds__(256) void k_adj_xsph_rev(StepTape t, AdjBufs a, Grid G, Phys P) {
+    const Role r = role<kHeavy>(t, a, 2u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
+    const AdjConst k = adj_const(P);
+    const D3 pi = wid(t.PI[i]), vi = wid(t.R[2 * i + 1]);
+    D3 avc = dz3(), apos = dz3();
+    for_T(a, G, f3of(t.Q[i]), r, [&](uint32_t q) {
+        const float4 pl = a.gP[q];
+        const float4 vl = a.gV[q];
+        if (__float_as_uint(pl.w) == i || vl.w != 0.f) return;
+        const D3 pd = pi - wid(pl);
+        if (dot3(pd, pd) > (double)P.cut2_w) return;
+        double W, gw;
+        if (!poly6_adj(pd, k, W, gw)) return;
+        const double4 g4 = a.gD[q];
+        const D3 gl = md3(g4.x, g4.y, g4.z);
+        avc = avc + sc3(gl, k.gamma * W);
+        apos = apos + sc3(pd, gw * k.gamma * dot3(gl, vi - wid(vl)));
+    });
+    if (kHeavy) {
+        avc = wsum3(avc);
+        apos = wsum3(apos);
+        if (r.lane) return;
+    }
     st(a.aVc, i, ld(a.aVc, i) + avc);
     st(a.aP, i, ld(a.aP, i) + apos);
 }
 
-// Without XSPH: v_out = Vc.  Clears the partial sums.
-__global__ __launch_bounds__(256) void k_adj_final_init(StepTape t, AdjBufs a) {
-    const uint32_t i = gtid();
-    if (i < t.n_live) {
-        st(a.aVc, i, ld3d(a.av, i));
-        st(a.aP, i, ld3d(a.ax, i));
-    }
-    if (i < kAdjMaxBlocks) {
-        a.part[i] = 0.0;
-        a.part[kAdjMaxBlocks + i] = 0.0;
-    }
-}
-
-// Confinement, direct terms: dL/domega_i through N x omega_i, dL/deta_i,
-// dL/d epsilon; dL/dV starts as dL/dVc.
+// Confinement, direct terms (Vc = V + kappa (N x omega_i), N = eta/|eta|,
+// eta_i = sum_j |omega_j| K(r) e_ij): dL/domega_i, dL/deta_i, dL/d eps;
+// dL/dV starts as dL/dVc.
+template <bool kHeavy>
 __global__ __launch_bounds__(256) void k_adj_conf_own(StepTape t, AdjBufs a, Grid G, Phys P) {
-    __shared__ double sh[256];
-    const uint32_t i = gtid();
-    double pv = 0.0;
-    if (i < t.n_live) {
-        const AdjConst k = adj_const(P);
-        const F3 selff = f3of(t.PI[i]);
-        const D3 self = wid3(selff);
-        D3 eta = dz3();
-        for_S(t, G, selff, [&](uint32_t j) {
-            SpikyAdj sd;
-            if (j == i || !spiky_adj(wid(t.PI[j]) - self, k, sd)) return;
-            const D3 om = wid(t.OM4[j]);
-            eta = eta + sc3(sd.f(), sqrt(dot3(om, om)));
-        });
-        const D3 avc = ld(a.aVc, i);
-        st(a.aV, i, avc);
-        const double nr = sqrt(dot3(eta, eta));
-        D3 aom = dz3(), eb = dz3();
-        if (nr > 0.0) {
-            const D3 Nh = sc3(eta, 1.0 / nr), om = wid(t.OM4[i]);
-            pv = dot3(avc, cross3(Nh, om)) * k.dpdvc;
-            const D3 y = sc3(avc, k.kappa);
-            aom = cross3(y, Nh);
-            const D3 nb = cross3(om, y);
-            eb = sc3(nb - sc3(Nh, dot3(Nh, nb)), 1.0 / nr);
-        }
-        st(a.aOm, i, aom);
-        st(a.etab, i, eb);
-    }
-    const double tot = block_sum(pv, sh);
-    if (threadIdx.x == 0) a.part[kAdjMaxBlocks + blockIdx.x] = tot;
-}
-
-// Confinement through eta: the position terms and dL/d|omega_i| (own pairs
-// and transposed pairs).
-__global__ __launch_bounds__(256) void k_adj_conf_eta(StepTape t, AdjBufs a, Grid G, Phys P) {
-    const uint32_t i = gtid();
-    if (i >= t.n_live) return;
+    const Role r = role<kHeavy>(t, a, 1u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
     const AdjConst k = adj_const(P);
     const F3 selff = f3of(t.PI[i]);
     const D3 self = wid3(selff);
-    const D3 eb = ld(a.etab, i);
-    const D3 omi = wid(t.OM4[i]);
-    const double wi = sqrt(dot3(omi, omi));
-    D3 apos = dz3();
-    if (!zero3(eb))
-        for_S(t, G, selff, [&](uint32_t j) {
-            SpikyAdj sd;
-            if (j == i || !spiky_adj(wid(t.PI[j]) - self, k, sd)) return;
-            const D3 om = wid(t.OM4[j]);
-            apos = apos - sc3(sd.H(eb), sqrt(dot3(om, om)));
-        });
-    double mb = 0.0;
-    for_T(a, G, f3of(t.Q[i]), [&](uint32_t l) {
-        if (l == i) return;
-        const D3 el = ld(a.etab, l);
-        if (zero3(el)) return;
+    D3 eta = dz3();
+    for_S(t, G, selff, r, [&](uint32_t j) {
         SpikyAdj sd;
-        if (!spiky_adj(self - wid(t.PI[l]), k, sd)) return;
-        mb += dot3(el, sd.f());
-        apos = apos + sc3(sd.H(el), wi);
+        if (j == i || !spiky_adj(wid(t.PI[j]) - self, k, sd)) return;
+        const D3 om = wid(t.OM4[j]);
+        eta = eta + sc3(sd.f(), sqrt(dot3(om, om)));
     });
-    D3 aom = ld(a.aOm, i);
-    if (wi > 0.0) aom = aom + sc3(omi, mb / wi);
+    if (kHeavy) {
+        eta = wsum3(eta);
+        if (r.lane) return;
+    }
+    const D3 avc = ld(a.aVc, i);
+    st(a.aV, i, avc);
+    const double nr = sqrt(dot3(eta, eta));
+    D3 aom = dz3(), eb = dz3();
+    if (nr > 0.0) {
+        const D3 Nh = sc3(eta, 1.0 / nr), om = wid(t.OM4[i]);
+        a.pe[i] = dot3(avc, cross3(Nh, om)) * k.dpdvc;
+        const D3 y = sc3(avc, k.kappa);
+        aom = cross3(y, Nh);
+        const D3 nb = cross3(om, y);
+        eb = sc3(nb - sc3(Nh, dot3(Nh, nb)), 1.0 / nr);
+    }
     st(a.aOm, i, aom);
+    st(a.etab, i, eb);
+}
+
+// Confinement through eta, own pairs: the position terms of eta_i.
+template <bool kHeavy>
+__global__ __launch_bounds__(256) void k_adj_eta_own(StepTape t, AdjBufs a, Grid G, Phys P) {
+    const Role r = role<kHeavy>(t, a, 1u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
+    const D3 eb = ld(a.etab, i);
+    if (zero3(eb)) return;  // uniform over a heavy wave
+    const AdjConst k = adj_const(P);
+    const F3 selff = f3of(t.PI[i]);
+    const D3 self = wid3(selff);
+    D3 apos = dz3();
+    for_S(t, G, selff, r, [&](uint32_t j) {
+        SpikyAdj sd;
+        if (j == i || !spiky_adj(wid(t.PI[j]) - self, k, sd)) return;
+        const D3 om = wid(t.OM4[j]);
+        apos = apos - sc3(sd.H(eb), sqrt(dot3(om, om)));
+    });
+    if (kHeavy) {
+        apos = wsum3(apos);
+        if (r.lane) return;
+    }
     st(a.aP, i, ld(a.aP, i) + apos);
 }
 
-// Vorticity: dL/dV and the position terms (own pairs and transposed pairs),
-// then the update V = (P - X)/dt.
-__global__ __launch_bounds__(256) void k_adj_vort(StepTape t, AdjBufs a, Grid G, Phys P, uint32_t cap) {
-    const uint32_t i = gtid();
-    if (i >= t.n_live) return;
+// Confinement through eta, transposed pairs (l, i): dL/d|omega_i| and the
+// position terms.
+template <bool kHeavy>
+__global__ __launch_bounds__(256) void k_adj_eta_rev(StepTape t, AdjBufs a, Grid G, Phys P) {
+    const Role r = role<kHeavy>(t, a, 2u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
+    const AdjConst k = adj_const(P);
+    const D3 self = wid(t.PI[i]);
+    const D3 omi = wid(t.OM4[i]);
+    const double wi = sqrt(dot3(omi, omi));
+    double mb = 0.0;
+    D3 apos = dz3();
+    for_T(a, G, f3of(t.Q[i]), r, [&](uint32_t q) {
+        const float4 pl = a.gP[q];
+        if (__float_as_uint(pl.w) == i) return;
+        const D3 el = ld(a.gD, q);
+        if (zero3(el)) return;
+        SpikyAdj sd;
+        if (!spiky_adj(self - wid(pl), k, sd)) return;
+        mb += dot3(el, sd.f());
+        apos = apos + sc3(sd.H(el), wi);
+    });
+    if (kHeavy) {
+        mb = wsum(mb);
+        apos = wsum3(apos);
+        if (r.lane) return;
+    }
+    if (wi > 0.0) st(a.aOm, i, ld(a.aOm, i) + sc3(omi, mb / wi));
+    st(a.aP, i, ld(a.aP, i) + apos);
+}
+
+// Vorticity (omega_i = beta sum_j (V_j - V_i) x (K(r) e_ij)), own pairs:
+// dL/dV and the position terms.  V as the forward forms it (FP32).
+template <bool kHeavy>
+__global__ __launch_bounds__(256) void k_adj_vort_own(StepTape t, AdjBufs a, Grid G, Phys P, uint32_t cap) {
+    const Role r = role<kHeavy>(t, a, 1u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
+    const D3 ab = ld(a.aOm, i);
+    if (zero3(ab)) return;  // uniform over a heavy wave
     const AdjConst k = adj_const(P);
     const F3 selff = f3of(t.PI[i]);
     const D3 self = wid3(selff);
-    // V exactly as the forward formed it (FP32), then widened
-    auto vel = [&](uint32_t q, F3 p) { return wid3(mulsv(P.inv_dt, p - ld3(t.X, cap, q))); };
-    const D3 vi = vel(i, selff);
-    const D3 ab = ld(a.aOm, i);
-    D3 av = ld(a.aV, i), apos = dz3();
-    if (!zero3(ab))
-        for_S(t, G, selff, [&](uint32_t j) {
-            SpikyAdj sd;
-            const F3 pjf = f3of(t.PI[j]);
-            if (j == i || !spiky_adj(wid3(pjf) - self, k, sd)) return;
-            const D3 e = vel(j, pjf) - vi;
-            av = av - sc3(cross3(sd.f(), ab), k.beta);
-            apos = apos - sd.H(sc3(cross3(ab, e), k.beta));
-        });
-    for_T(a, G, f3of(t.Q[i]), [&](uint32_t l) {
-        if (l == i) return;
-        const D3 al = ld(a.aOm, l);
+    const D3 vi = wid3(mulsv(P.inv_dt, selff - ld3(t.X, cap, i)));
+    D3 av = dz3(), apos = dz3();
+    for_S(t, G, selff, r, [&](uint32_t j) {
+        SpikyAdj sd;
+        const F3 pjf = f3of(t.PI[j]);
+        if (j == i || !spiky_adj(wid3(pjf) - self, k, sd)) return;
+        const D3 e = wid3(mulsv(P.inv_dt, pjf - ld3(t.X, cap, j))) - vi;
+        av = av - sc3(cross3(sd.f(), ab), k.beta);
+        apos = apos - sd.H(sc3(cross3(ab, e), k.beta));
+    });
+    if (kHeavy) {
+        av = wsum3(av);
+        apos = wsum3(apos);
+        if (r.lane) return;
+    }
+    st(a.aV, i, ld(a.aV, i) + av);
+    st(a.aP, i, ld(a.aP, i) + apos);
+}
+
+// Vorticity, transposed pairs (l, i).
+template <bool kHeavy>
+__global__ __launch_bounds__(256) void k_adj_vort_rev(StepTape t, AdjBufs a, Grid G, Phys P, uint32_t cap) {
+    const Role r = role<kHeavy>(t, a, 2u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
+    const AdjConst k = adj_const(P);
+    const F3 selff = f3of(t.PI[i]);
+    const D3 self = wid3(selff);
+    const D3 vi = wid3(mulsv(P.inv_dt, selff - ld3(t.X, cap, i)));
+    D3 av = dz3(), apos = dz3();
+    for_T(a, G, f3of(t.Q[i]), r, [&](uint32_t q) {
+        const float4 pl = a.gP[q];
+        if (__float_as_uint(pl.w) == i) return;
+        const D3 al = ld(a.gD, q);
         if (zero3(al)) return;
         SpikyAdj sd;
-        const F3 plf = f3of(t.PI[l]);
-        if (!spiky_adj(self - wid3(plf), k, sd)) return;
-        const D3 e = vi - vel(l, plf);
+        if (!spiky_adj(self - wid(pl), k, sd)) return;
+        const D3 e = vi - wid(a.gV[q]);
         av = av + sc3(cross3(sd.f(), al), k.beta);
         apos = apos + sd.H(sc3(cross3(al, e), k.beta));
     });
-    st(a.aP, i, ld(a.aP, i) + apos + sc3(av, k.inv_dt));
-    st(a.aX, i, sc3(av, -k.inv_dt));
+    if (kHeavy) {
+        av = wsum3(av);
+        apos = wsum3(apos);
+        if (r.lane) return;
+    }
+    st(a.aV, i, ld(a.aV, i) + av);
+    st(a.aP, i, ld(a.aP, i) + apos);
 }
 
-// Without vorticity: the update only.
-__global__ __launch_bounds__(256) void k_adj_update(StepTape t, AdjBufs a, Phys P) {
+// The update V = (P - X)/dt: dL/dP += dL/dV / dt, dL/dX = -dL/dV / dt.
+__global__ __launch_bounds__(256) void k_adj_update(StepTape t, AdjBufs a, Phys P, int vort) {
     const uint32_t i = gtid();
     if (i >= t.n_live) return;
-    const D3 av = ld(a.aVc, i);
+    const D3 av = ld(vort ? a.aV : a.aVc, i);
     const double inv_dt = P.inv_dt;
     st(a.aP, i, ld(a.aP, i) + sc3(av, inv_dt));
     st(a.aX, i, sc3(av, -inv_dt));
@@ -641,10 +790,13 @@ __global__ __launch_bounds__(256) void k_adj_coll(StepTape t, AdjBufs a, const f
 // delta p = beta sum_j A_ij kg_ij, A_ij = -(s_i + s_j + corr(W_ij)):
 // dL/ds_i and the position terms through kg and corr, own pairs (i, j) and
 // transposed pairs (l, i); then the coefficients of the lambda pass.
+// Particles on long reverse lists run one wave each.
+template <bool kHeavy>
 __global__ __launch_bounds__(256) void k_adj_lam(StepTape t, AdjBufs a, const float4* __restrict__ Q, Phys P,
                                                  uint32_t cap) {
-    const uint32_t i = gtid();
-    if (i >= t.n_live) return;
+    const Role r = role<kHeavy>(t, a, 8u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
     const AdjConst k = adj_const(P);
     const float4 se = Q[i];
     const D3 self = wid(se);
@@ -653,7 +805,7 @@ __global__ __launch_bounds__(256) void k_adj_lam(StepTape t, AdjBufs a, const fl
     double sb = 0.0, kg2s = 0.0;
     D3 kgs = dz3(), apos = dz3();
     const uint32_t n = t.ncnt[i];
-    for (uint32_t q = 0; q < n; ++q) {
+    for (uint32_t q = r.lane; q < n; q += r.stride) {
         const float4 pe = Q[t.nlist[(size_t)q * cap + i]];
         const D3 d = self - wid(pe);
         PairAdj pa;
@@ -669,7 +821,7 @@ __global__ __launch_bounds__(256) void k_adj_lam(StepTape t, AdjBufs a, const fl
         apos = apos + sc3(pa.H(ai), k.beta * A) + sc3(d, Wb * pa.gw);
     }
     const uint32_t r1 = a.roff[i + 1];
-    for (uint32_t q = a.roff[i]; q < r1; ++q) {
+    for (uint32_t q = a.roff[i] + r.lane; q < r1; q += r.stride) {
         const uint32_t l = a.rlist[q];
         const float4 pe = Q[l];
         const D3 d = wid(pe) - self;  // the pair (l, i): d = p_l - p_i
@@ -684,6 +836,13 @@ __global__ __launch_bounds__(256) void k_adj_lam(StepTape t, AdjBufs a, const fl
         const double Wb = -Ab * corr_dW(k, pa.W, sl, si);
         apos = apos - (sc3(pa.H(al), k.beta * A) + sc3(d, Wb * pa.gw));
     }
+    if (kHeavy) {
+        sb = wsum(sb);
+        kg2s = wsum(kg2s);
+        kgs = wsum3(kgs);
+        apos = wsum3(apos);
+        if (r.lane) return;
+    }
     // s = C / D, C = beta sum W - 1, D = m2r2 (|sum kg|^2 + sum |kg|^2) + eps
     const double D = k.m2r2 * (dot3(kgs, kgs) + kg2s) + k.relax;
     const double ksb = k.beta * sb / D;
@@ -695,10 +854,12 @@ __global__ __launch_bounds__(256) void k_adj_lam(StepTape t, AdjBufs a, const fl
 
 // lambda pass: the position terms through W and kg of s_i (own pairs) and
 // s_l (transposed pairs); dL/dP of the iteration's input.
+template <bool kHeavy>
 __global__ __launch_bounds__(256) void k_adj_pos(StepTape t, AdjBufs a, const float4* __restrict__ Q, Phys P,
                                                  uint32_t cap) {
-    const uint32_t i = gtid();
-    if (i >= t.n_live) return;
+    const Role r = role<kHeavy>(t, a, 8u);
+    if (!r.on) return;
+    const uint32_t i = r.i;
     const AdjConst k = adj_const(P);
     const D3 self = wid(Q[i]);
     const double4 ci = a.coef[i];
@@ -706,7 +867,7 @@ __global__ __launch_bounds__(256) void k_adj_pos(StepTape t, AdjBufs a, const fl
     D3 apos = dz3();
     if (ci.x != 0.0 || g2i != 0.0) {
         const uint32_t n = t.ncnt[i];
-        for (uint32_t q = 0; q < n; ++q) {
+        for (uint32_t q = r.lane; q < n; q += r.stride) {
             const D3 d = self - wid(Q[t.nlist[(size_t)q * cap + i]]);
             PairAdj pa;
             if (!pair_adj(d, k, pa)) continue;
@@ -714,7 +875,7 @@ __global__ __launch_bounds__(256) void k_adj_pos(StepTape t, AdjBufs a, const fl
         }
     }
     const uint32_t r1 = a.roff[i + 1];
-    for (uint32_t q = a.roff[i]; q < r1; ++q) {
+    for (uint32_t q = a.roff[i] + r.lane; q < r1; q += r.stride) {
         const uint32_t l = a.rlist[q];
         const double4 cl = a.coef[l];
         const double g2l = a.g2[l];
@@ -723,6 +884,10 @@ __global__ __launch_bounds__(256) void k_adj_pos(StepTape t, AdjBufs a, const fl
         PairAdj pa;
         if (!pair_adj(d, k, pa)) continue;
         apos = apos - (sc3(d, cl.x * pa.gw) + pa.H(md3(cl.y, cl.z, cl.w) + sc3(pa.kg, g2l)));
+    }
+    if (kHeavy) {
+        apos = wsum3(apos);
+        if (r.lane) return;
     }
     st(a.aP, i, ld(a.aQ, i) + apos);
 }
@@ -747,12 +912,24 @@ __global__ __launch_bounds__(256) void k_adj_in(StepTape t, AdjBufs a, Phys P) {
     a.avin[3 * i + 2] = gv.z;
 }
 
-// psum[0] = sum of part[0..nb), psum[1] = sum of part[M..M+nb), in order
-__global__ __launch_bounds__(256) void k_adj_reduce(AdjBufs a, uint32_t nb) {
+// sum of v over the workgroup (256 threads), valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// psum = (sum of pc, sum of pe) over the live slots, in a fixed order
+__global__ __launch_bounds__(256) void k_adj_reduce(StepTape t, AdjBufs a) {
     __shared__ double sh[256];
     for (int w = 0; w < 2; ++w) {
+        const double* src = w ? a.pe : a.pc;
         double v = 0.0;
-        for (uint32_t q = threadIdx.x; q < nb; q += blockDim.x) v += a.part[w * kAdjMaxBlocks + q];
+        for (uint32_t q = threadIdx.x; q < t.n_live; q += blockDim.x) v += src[q];
         const double tot = block_sum(v, sh);
         if (threadIdx.x == 0) a.psum[w] = tot;
         __syncthreads();
@@ -793,52 +970,80 @@ inline uint32_t nblk(uint32_t n) { return (n + 255u) / 256u; }
 }  // namespace
 
 hipError_t launch_adj_step(const Bufs& b, const Scene& s0, const StepTape& t, const AdjBufs& a, int iters,
-                           hipStream_t s) {
+                           hipStream_t s, uint32_t* h_nh) {
     const Grid& G = s0.G;
     const Phys& P = s0.P;
     const uint32_t nl = t.n_live;
     const dim3 blk(256);
     const dim3 gl(nblk(nl > 0 ? nl : 1));
-    if (gl.x > kAdjMaxBlocks) return hipErrorInvalidValue;
     const uint32_t cap = b.cap;
+    hipError_t e = hipSuccess;
     if (nl > 0) {
         // particles grouped by final cell
         hipLaunchKernelGGL(k_adj_ckey, gl, blk, 0, s, t, a, G, b.lgT);
-        hipError_t e = launch_count_scan(a.ccount, a.coffset, a.ctsum, a.ctflag, a.ctprefix, b.nT, b.lgT, b.CT, s);
+        e = launch_count_scan(a.ccount, a.coffset, a.ctsum, a.ctflag, a.ctprefix, b.nT, b.lgT, b.CT, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_adj_cscatter, gl, blk, 0, s, t, a);
         hipLaunchKernelGGL(k_adj_corder, gl, blk, 0, s, t, a);
+        // heavy lists of the final passes and the search cells: their
+        // lengths size the heavy launches
+        e = hipMemsetAsync(a.nh, 0, 4 * sizeof(uint32_t), s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_adj_classify, gl, blk, 0, s, t, a, G);
+        e = hipMemcpyAsync(h_nh, a.nh, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        auto ghv = [](uint32_t n) { return dim3(nblk(n * 64u > 0 ? n * 64u : 1u)); };
+        const uint32_t hs = h_nh[0], ht = h_nh[1], hn = h_nh[2];
         // reverse lists
         hipLaunchKernelGGL(k_adj_last, gl, blk, 0, s, t, a, G, P, cap);
-        hipLaunchKernelGGL(k_adj_rev<false>, gl, blk, 0, s, t, a, G, P);
+        hipLaunchKernelGGL((k_adj_rev<false, false>), gl, blk, 0, s, t, a, G, P);
+        if (hn) hipLaunchKernelGGL((k_adj_rev<false, true>), ghv(hn), blk, 0, s, t, a, G, P);
         e = launch_count_scan(a.rcount, a.roff, a.rtsum, a.rtflag, a.rtprefix, a.nRT, a.lgRT, a.RT, s);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_adj_rev<true>, gl, blk, 0, s, t, a, G, P);
+        hipLaunchKernelGGL((k_adj_rev<true, false>), gl, blk, 0, s, t, a, G, P);
+        if (hn) hipLaunchKernelGGL((k_adj_rev<true, true>), ghv(hn), blk, 0, s, t, a, G, P);
+        e = hipMemcpyAsync(h_nh + 3, a.nh + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        const uint32_t hr = h_nh[3];
+        const dim3 ghS = ghv(hs), ghT = ghv(ht);
         // final stage
-        hipLaunchKernelGGL(k_adj_final_init, dim3(nblk(kAdjMaxBlocks > nl ? kAdjMaxBlocks : nl)), blk, 0, s, t, a);
+        hipLaunchKernelGGL(k_adj_final_init, gl, blk, 0, s, t, a);
         if (P.xsph_on) {
-            hipLaunchKernelGGL(k_adj_xsph_own, gl, blk, 0, s, t, a, G, P);
-            hipLaunchKernelGGL(k_adj_xsph_rev, gl, blk, 0, s, t, a, G, P);
+            hipLaunchKernelGGL(k_adj_xsph_own<false>, gl, blk, 0, s, t, a, G, P);
+            if (hs) hipLaunchKernelGGL(k_adj_xsph_own<true>, ghS, blk, 0, s, t, a, G, P);
+            hipLaunchKernelGGL(k_adj_gather, gl, blk, 0, s, t, a, P, cap, 0);
+            hipLaunchKernelGGL(k_adj_xsph_rev<false>, gl, blk, 0, s, t, a, G, P);
+            if (ht) hipLaunchKernelGGL(k_adj_xsph_rev<true>, ghT, blk, 0, s, t, a, G, P);
         }
         if (P.vort_on) {
-            hipLaunchKernelGGL(k_adj_conf_own, gl, blk, 0, s, t, a, G, P);
-            hipLaunchKernelGGL(k_adj_conf_eta, gl, blk, 0, s, t, a, G, P);
-            hipLaunchKernelGGL(k_adj_vort, gl, blk, 0, s, t, a, G, P, cap);
-        } else {
-            hipLaunchKernelGGL(k_adj_update, gl, blk, 0, s, t, a, P);
+            hipLaunchKernelGGL(k_adj_conf_own<false>, gl, blk, 0, s, t, a, G, P);
+            if (hs) hipLaunchKernelGGL(k_adj_conf_own<true>, ghS, blk, 0, s, t, a, G, P);
+            hipLaunchKernelGGL(k_adj_eta_own<false>, gl, blk, 0, s, t, a, G, P);
+            if (hs) hipLaunchKernelGGL(k_adj_eta_own<true>, ghS, blk, 0, s, t, a, G, P);
+            hipLaunchKernelGGL(k_adj_gather, gl, blk, 0, s, t, a, P, cap, 1);
+            hipLaunchKernelGGL(k_adj_eta_rev<false>, gl, blk, 0, s, t, a, G, P);
+            if (ht) hipLaunchKernelGGL(k_adj_eta_rev<true>, ghT, blk, 0, s, t, a, G, P);
+            hipLaunchKernelGGL(k_adj_gather, gl, blk, 0, s, t, a, P, cap, 2);
+            hipLaunchKernelGGL(k_adj_vort_own<false>, gl, blk, 0, s, t, a, G, P, cap);
+            if (hs) hipLaunchKernelGGL(k_adj_vort_own<true>, ghS, blk, 0, s, t, a, G, P, cap);
+            hipLaunchKernelGGL(k_adj_vort_rev<false>, gl, blk, 0, s, t, a, G, P, cap);
+            if (ht) hipLaunchKernelGGL(k_adj_vort_rev<true>, ghT, blk, 0, s, t, a, G, P, cap);
         }
+        hipLaunchKernelGGL(k_adj_update, gl, blk, 0, s, t, a, P, P.vort_on);
         // Jacobi iterations, last first
         for (int it = iters - 1; it >= 0; --it) {
             const float4* Q = t.Q + (size_t)it * cap;
             hipLaunchKernelGGL(k_adj_coll, gl, blk, 0, s, t, a, Q, P, s0.B, cap);
-            hipLaunchKernelGGL(k_adj_lam, gl, blk, 0, s, t, a, Q, P, cap);
-            hipLaunchKernelGGL(k_adj_pos, gl, blk, 0, s, t, a, Q, P, cap);
+            hipLaunchKernelGGL(k_adj_lam<false>, gl, blk, 0, s, t, a, Q, P, cap);
+            if (hr) hipLaunchKernelGGL(k_adj_lam<true>, ghv(hr), blk, 0, s, t, a, Q, P, cap);
+            hipLaunchKernelGGL(k_adj_pos<false>, gl, blk, 0, s, t, a, Q, P, cap);
+            if (hr) hipLaunchKernelGGL(k_adj_pos<true>, ghv(hr), blk, 0, s, t, a, Q, P, cap);
         }
-    } else {
-        hipLaunchKernelGGL(k_adj_final_init, dim3(nblk(kAdjMaxBlocks)), blk, 0, s, t, a);
     }
     if (t.n_in > 0) hipLaunchKernelGGL(k_adj_in, dim3(nblk(t.n_in)), blk, 0, s, t, a, P);
-    hipLaunchKernelGGL(k_adj_reduce, dim3(1), blk, 0, s, a, gl.x);
+    hipLaunchKernelGGL(k_adj_reduce, dim3(1), blk, 0, s, t, a);
     return hipGetLastError();
 }
 

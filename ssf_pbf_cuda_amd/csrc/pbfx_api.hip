@@ -45,6 +45,7 @@ struct AdjState {
     AdjBufs a{};
     unsigned long long* h_amax = nullptr;  // pinned
     double* h_psum = nullptr;         // pinned [2]
+    uint32_t* h_nh = nullptr;         // pinned [4]
 };
 
 struct pbfx_ctx {
@@ -217,13 +218,14 @@ void free_adj(pbfx_ctx* c) {
     AdjBufs& a = A->a;
     void* ap[] = {A->id[0], A->id[1], A->xw, A->vw, a.ax, a.av, a.axin, a.avin, a.aP, a.aQ, a.aV, a.aVc, a.aOm,
                   a.etab, a.abar, a.coef, a.g2, a.aX, a.xnan, a.ckey, a.crank, a.ctmp, a.cperm, a.ccount,
-                  a.coffset, a.ctsum, a.ctflag, a.ctprefix, a.last, a.rcount, a.roff, a.rtsum, a.rtflag,
-                  a.rtprefix, a.rlist, a.part, a.psum, a.amax};
+                  a.coffset, a.ctsum, a.ctflag, a.ctprefix, a.gP, a.gV, a.gD, a.hflag, a.hlist, a.nh, a.last,
+                  a.rcount, a.roff, a.rtsum, a.rtflag, a.rtprefix, a.rlist, a.pc, a.pe, a.psum, a.amax};
     for (void* p : ap) ptrs.push_back(p);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (A->h_amax) (void)hipHostFree(A->h_amax);
     if (A->h_psum) (void)hipHostFree(A->h_psum);
+    if (A->h_nh) (void)hipHostFree(A->h_nh);
     delete A;
     c->adj = nullptr;
 }
@@ -1085,6 +1087,12 @@ pbfx_status pbfx_adj_enable(pbfx_ctx* c, uint32_t max_steps, uint32_t every) {
     PBFX_ALLOC(a.ctsum, nT + 1);
     PBFX_ALLOC(a.ctflag, nT + 1);
     PBFX_ALLOC(a.ctprefix, nT + 1);
+    PBFX_ALLOC(a.gP, cap);
+    PBFX_ALLOC(a.gV, cap);
+    PBFX_ALLOC(a.gD, cap);
+    PBFX_ALLOC(a.hflag, cap);
+    PBFX_ALLOC(a.hlist, 4 * cap);
+    PBFX_ALLOC(a.nh, 4);
     PBFX_ALLOC(a.last, cap);
     a.RT = pad_tiles((uint32_t)cap, a.lgRT, a.nRT);
     PBFX_ALLOC(a.rcount, (size_t)a.RT + 1);
@@ -1093,7 +1101,8 @@ pbfx_status pbfx_adj_enable(pbfx_ctx* c, uint32_t max_steps, uint32_t every) {
     PBFX_ALLOC(a.rtflag, (size_t)a.nRT + 1);
     PBFX_ALLOC(a.rtprefix, (size_t)a.nRT + 1);
     PBFX_ALLOC(a.rlist, cap * c->list_stride);
-    PBFX_ALLOC(a.part, 2 * (size_t)kAdjMaxBlocks);
+    PBFX_ALLOC(a.pc, cap);
+    PBFX_ALLOC(a.pe, cap);
     PBFX_ALLOC(a.psum, 2);
     PBFX_ALLOC(a.amax, 1);
 #undef PBFX_ALLOC
@@ -1103,6 +1112,7 @@ pbfx_status pbfx_adj_enable(pbfx_ctx* c, uint32_t max_steps, uint32_t every) {
     if (e == hipSuccess) e = hipMemset(a.rtflag, 0, ((size_t)a.nRT + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc((void**)&A->h_amax, sizeof(*A->h_amax), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&A->h_psum, 2 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&A->h_nh, 4 * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_adj(c);
         return hip_fail(c, e);
@@ -1234,7 +1244,7 @@ pbfx_status pbfx_adj_backward(pbfx_ctx* c, const double* gx, const double* gv, d
         }
         for (uint32_t t = t1; t-- > t0;) {
             const StepTape& T = A->tape[t - t0];
-            e = launch_adj_step(c->b, c->last_table[0], T, a, (int)A->iters, s);
+            e = launch_adj_step(c->b, c->last_table[0], T, a, (int)A->iters, s, A->h_nh);
             if (e == hipSuccess) e = launch_adj_absmax(a, T.n_in, s);
             if (e == hipSuccess) e = hipMemcpyAsync(A->h_psum, a.psum, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipMemcpyAsync(A->h_amax, a.amax, sizeof(*a.amax), hipMemcpyDeviceToHost, s);

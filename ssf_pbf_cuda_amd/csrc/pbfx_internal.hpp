@@ -218,15 +218,28 @@ struct AdjBufs {
     uint32_t* xnan;     // XSPH NaN guard per slot
     // particles grouped by final cell: the transposed 27-cell sets
     uint32_t *ckey, *crank, *ctmp, *cperm, *ccount, *coffset, *ctsum, *ctflag, *ctprefix;
+    // per-kernel copies in final-cell order (position q holds slot cperm[q]):
+    // (final position, slot bits), a float4 and a double4 of the pass
+    float4* gP;
+    float4* gV;
+    double4* gD;
+    // particles with more than kAdjHeavy candidates run one wave each, per
+    // set: bit 0 the final passes' 27-cell set, bit 1 its transpose, bit 2
+    // the search cells (reverse-list construction), bit 3 the reverse list
+    uint32_t* hflag;    // [cap]
+    uint32_t* hlist;    // [4*cap] the heavy particles of each set
+    uint32_t* nh;       // [4] list lengths
     // reverse lists: the particles that list each slot
     uint2* last;        // (27-cell index, slot) of a full list's last entry; (255, 0) if not full
     uint32_t *rcount, *roff, *rtsum, *rtflag, *rtprefix, *rlist;
     uint32_t RT, lgRT, nRT;   // padded slot space of the reverse-count scan
-    double* part;       // [2 * kAdjMaxBlocks] per-workgroup partial sums of dL/dc, dL/d eps
+    double* pc;         // [cap] each particle's share of dL/dc
+    double* pe;         // [cap] each particle's share of dL/d eps
     double* psum;       // [2]
     unsigned long long* amax;  // [1] largest |adjoint| of the step input, as the bits of a double
 };
-constexpr uint32_t kAdjMaxBlocks = 4096;
+constexpr uint32_t kAdjMaxBlocks = 4096;   // workgroups of a per-particle adjoint launch
+constexpr uint32_t kAdjHeavy = 256;        // candidates above which a particle gets a wave
 
 // Launchers (pbfx_kernels.hip).  All enqueue on `s` and never synchronise.
 // n_in: particles entering (all scenes); uw: cells per work unit along x,
@@ -256,9 +269,10 @@ void size_unit_grids(Tuning& T, int num_cu);
 
 // Adjoint launchers (pbfx_adjoint.hip).  launch_adj_step: dL/d(step input)
 // into a.axin / a.avin from a.ax / a.av, and the step's dL/dc, dL/d eps into
-// a.psum, for the recorded step t of the single scene s0.
+// a.psum, for the recorded step t of the single scene s0.  It waits on `s`
+// twice, to read the heavy-list lengths into h_nh (pinned, 4 words).
 hipError_t launch_adj_step(const Bufs& b, const Scene& s0, const StepTape& t, const AdjBufs& a, int iters,
-                           hipStream_t s);
+                           hipStream_t s, uint32_t* h_nh);
 // a.amax = max |a.axin|, |a.avin| over n rows (as double bits).
 hipError_t launch_adj_absmax(const AdjBufs& a, uint32_t n, hipStream_t s);
 // x[0..n) *= f for each of the arrays (may be NULL).

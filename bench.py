@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=10, help="timed steps of the 256-scene batch (0: skip)")
     ap.add_argument("--adjoint-runs", type=int, default=2,
                     help="timed forward+backward runs of the differentiable workload (0: skip)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the differentiable workload (for profiling it alone)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--tile-cap", type=int, default=None, help="LDS tile (particles); default: autotuned")
     ap.add_argument("--unit-width", type=int, default=16)
@@ -369,6 +371,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm device"
     pbfx.lib()
+    if a.adjoint_only:
+        d = run_adjoint(max(a.adjoint_runs, 1))
+        print(json.dumps({"metric": "M particle-steps/s (forward + backward), differentiable workload",
+                          "value": d["value"], "unit": d["unit"], "n_gpus": 1, "higher_is_better": True,
+                          "dtype": "f32 forward, f64 adjoint", "data": "synthetic",
+                          "config": {"workload": d["workload"]}, "differentiable": d}), flush=True)
+        return
     sc = scenes.CONFIGS[a.config]()
     r = run_gpu(sc, a.steps, a.warmup, a.kernel_steps, a.tile_cap, a.unit_width)
     n, C, I = sc.n, sc.cells, sc.iters

@@ -7,6 +7,7 @@
 //   pbf_buffer                                                    (pbf/pbf_type.h:88-100)
 //   pbf_boundary                                                  (pbf/pbf_boundary.h:4-19)
 //   pbf::one_step, with and without a boundary                    (pbf/solver/pbf_solve.h:57-70)
+//   pbf::cuda::addParticle                                        (pbf/util/cuda/pbf_add.cu:12-28)
 // so a caller such as pbf_sphere_sim::simulateOneStep
 // (simulator/sphere/pbf_sphere_sim.cpp:384-389) recompiles against this header
 // and links libpbfx.so instead of the CUDA objects.
@@ -283,6 +284,26 @@ inline void one_step(pbf_particle& simulatee, pbf_buffer& buffer, const std::pai
     simulatee.phase.num = num;
 }
 
+namespace cuda {
+
+// pbf/util/cuda/pbf_add.cu:12-28: appends adding_num particles (device
+// arrays) after the live ones and raises phase.num (no capacity check in the
+// reference; here exceeding the capacity exits like its allocation errors).
+inline void addParticle(pbf_particle& particle, const dom_dim* adding_position, const dom_dim* adding_velocity,
+                        uint32_t adding_num) {
+    pbfx_ctx* c = particle.ns.get();
+    if (!c) pbf_compat_detail::fail("addParticle", PBFX_INVALID_ARG);
+    uint32_t num = particle.phase.num;
+    pbf_compat_detail::check("addParticle",
+                             pbfx_add_particles(c, reinterpret_cast<float*>(particle.phase.x),
+                                                reinterpret_cast<float*>(particle.phase.v), &num,
+                                                reinterpret_cast<const float*>(adding_position),
+                                                reinterpret_cast<const float*>(adding_velocity), adding_num,
+                                                (void*)pbf_compat_options().stream));
+    particle.phase.num = num;
+}
+
+}  // namespace cuda
 }  // namespace pbf
 
 #endif  // PBF_COMPAT_HPP

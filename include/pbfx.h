@@ -131,6 +131,15 @@ pbfx_status pbfx_step(pbfx_ctx* ctx, float* x, float* v, uint32_t* num_inout, co
 pbfx_status pbfx_step_async(pbfx_ctx* ctx, float* x, float* v, uint32_t num, const float lo[3],
                             const float hi[3], int iters, void* stream);
 
+/* Appends add_n particles after the live ones: x[num..num+add_n) = add_x,
+ * v[num..num+add_n) = add_v (device AoS float3, copied on `stream`) and
+ * *num_inout += add_n.  Replaces pbf::cuda::addParticle
+ * (pbf/util/cuda/pbf_add.cu:12-28), the emission of the reference's jet
+ * (simulator/sphere/pbf_sphere_sim.cpp:84-86,386-387).  PBFX_CAPACITY if the
+ * count would exceed the context's capacity (nothing is copied then). */
+pbfx_status pbfx_add_particles(pbfx_ctx* ctx, float* x, float* v, uint32_t* num_inout, const float* add_x,
+                               const float* add_v, uint32_t add_n, void* stream);
+
 /* Live particle count after the last step on `stream` (synchronises it). */
 pbfx_status pbfx_get_num(pbfx_ctx* ctx, void* stream, uint32_t* num);
 

@@ -762,6 +762,21 @@ pbfx_status pbfx_batch_step(pbfx_ctx* c, float* x, float* v, uint32_t* counts, i
     return step_status(c);
 }
 
+pbfx_status pbfx_add_particles(pbfx_ctx* c, float* x, float* v, uint32_t* num_inout, const float* add_x,
+                               const float* add_v, uint32_t add_n, void* stream) {
+    if (!c || c->batch || !x || !v || !num_inout || (add_n && (!add_x || !add_v))) return PBFX_INVALID_ARG;
+    if ((uint64_t)*num_inout + add_n > c->capacity) return PBFX_CAPACITY;
+    if (add_n == 0) return PBFX_OK;
+    DeviceGuard g(c->device);
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t off = (size_t)*num_inout * 3, bytes = (size_t)add_n * 3 * sizeof(float);
+    hipError_t e = hipMemcpyAsync(x + off, add_x, bytes, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(v + off, add_v, bytes, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return hip_fail(c, e);
+    *num_inout += add_n;
+    return PBFX_OK;
+}
+
 pbfx_status pbfx_get_num(pbfx_ctx* c, void* stream, uint32_t* num) {
     if (!c || !num) return PBFX_INVALID_ARG;
     DeviceGuard g(c->device);

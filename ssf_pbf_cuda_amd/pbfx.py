@@ -147,6 +147,7 @@ def lib():
             "pbfx_selftest_arith": ([vp, vp, vp, u32, vp], ctypes.c_int),
             "pbfx_set_kernel_timing": ([vp, ctypes.c_int], ctypes.c_int),
             "pbfx_kernel_timing_read": ([vp, vp, ctypes.POINTER(KernelTime), u32, u32p], ctypes.c_int),
+            "pbfx_add_particles": ([vp, vp, vp, u32p, vp, vp, u32, vp], ctypes.c_int),
             "pbfx_adj_enable": ([vp, u32, u32], ctypes.c_int),
             "pbfx_adj_forward": ([vp, vp, vp, u32p, f32p, f32p, ctypes.c_int, u32, vp], ctypes.c_int),
             "pbfx_adj_identity": ([vp, vp, vp], ctypes.c_int),
@@ -295,6 +296,13 @@ class Solver:
     def step_async(self, x, v, num: int, lo, hi, iters: int, stream=None) -> None:
         _check(self._lib.pbfx_step_async(self._ctx, _ptr(x), _ptr(v), int(num), _f3(lo), _f3(hi), int(iters),
                                          _stream(stream)), "pbfx_step_async")
+
+    def add_particles(self, x, v, num: int, add_x, add_v, add_n: int, stream=None) -> int:
+        """Appends add_n particles after the live ones (pbf::cuda::addParticle); returns the new count."""
+        n = ctypes.c_uint32(num)
+        _check(self._lib.pbfx_add_particles(self._ctx, _ptr(x), _ptr(v), ctypes.byref(n), _ptr(add_x), _ptr(add_v),
+                                            int(add_n), _stream(stream)), "pbfx_add_particles")
+        return int(n.value)
 
     def num(self, stream=None) -> int:
         n = ctypes.c_uint32()

@@ -331,4 +331,173 @@ def small_dam(nx: int = 12, ny: int = 10, nz: int = 8, h: float = 0.1, box=(1.6,
                      **overrides)
 
 
+# ---------------------------------------------------------------------------
+# The reference's arrangement generators (pbf/util/pbf_arrangement.h) and the
+# initial conditions of its two simulators, in FP32 as it computes them
+# (dom_dim = glm::vec3, scalar_t = float; pbf/pbf_type.h:18-27).
+
+_EPS10 = 10.0 * np.finfo(np.float64).eps
+
+
+def _count(rng, space) -> int:
+    """static_cast<int>(range / space + epsilon * 10.0) + 1 (float division,
+    double sum)."""
+    return int(float(np.float32(np.float32(rng) / np.float32(space))) + _EPS10) + 1
+
+
+def cartesian_volume_range(origin, end, space) -> np.ndarray:
+    """cartesian_volume(pos, origin, end, space) (pbf_arrangement.h:82-100):
+    points origin + space*(x, y, z) up to end, x fastest."""
+    o = np.asarray(origin, np.float32)
+    e = np.asarray(end, np.float32)
+    n = [_count(np.float32(e[a] - o[a]), space) for a in range(3)]
+    return cartesian_volume(o, n, space)
+
+
+def cartesian_line_inclusive(origin, end, space) -> np.ndarray:
+    """cartesian_line_inclusive (pbf_arrangement.h:8-18): floor(|end -
+    origin| / space) + 1 points from origin towards end, spaced `space`."""
+    o = np.asarray(origin, np.float32)
+    e = np.asarray(end, np.float32)
+    d = (e - o).astype(np.float32)
+    length = np.float32(np.sqrt(np.float32(np.sum(d.astype(np.float32) * d, dtype=np.float32))))
+    step = (d / length).astype(np.float32) * np.float32(space)
+    num = int(math.floor(np.float32(length / np.float32(space)))) + 1
+    return np.stack([(o + step * np.float32(i)).astype(np.float32) for i in range(num)]).astype(np.float32)
+
+
+def cartesian_rectangle(axis: str, axis_value, origin, end, space) -> np.ndarray:
+    """cartesian_rectangle (pbf_arrangement.h:123-162): a lattice in the
+    plane axis = axis_value; origin/end are 2-D in (y, z) for "x", (z, x) for
+    "y" and (x, y) for "z", with the reference's loop orders."""
+    s = np.float32(space)
+    av = np.float32(axis_value)
+    o = np.asarray(origin, np.float32)
+    e = np.asarray(end, np.float32)
+    n0, n1 = _count(np.float32(e[0] - o[0]), s), _count(np.float32(e[1] - o[1]), s)
+    pts = []
+    if axis == "x":
+        for z in range(n1):
+            for y in range(n0):
+                pts.append((av, o[0] + s * np.float32(y), o[1] + s * np.float32(z)))
+    elif axis == "y":
+        for z in range(n0):
+            for x in range(n1):
+                pts.append((o[1] + s * np.float32(x), av, o[0] + s * np.float32(z)))
+    elif axis == "z":
+        for x in range(n0):
+            for y in range(n1):
+                pts.append((o[0] + s * np.float32(x), o[1] + s * np.float32(y), av))
+    else:
+        raise ValueError(axis)
+    return np.asarray(pts, np.float32).reshape(-1, 3)
+
+
+def cartesian_cylinder(axis: str, axis_value, origin, radius, height, space) -> np.ndarray:
+    """cartesian_cylinder (pbf_arrangement.h:210-241): rings of n points,
+    theta = 2 asin(space / 2r), n = floor(2 pi / theta), every `space` along
+    the axis up to `height` (FP32 angles and accumulation; cos/sin in FP32)."""
+    r = np.float32(radius)
+    s = np.float32(space)
+    theta = np.float32(2.0 * math.asin(float(s) / (2.0 * float(r))))
+    pi_f = np.float32(math.pi)
+    n = np.float32(math.floor(2.0 * float(pi_f) / float(theta)))
+    theta_hat = np.float32(2.0 * float(pi_f) / float(n))
+    av = np.float32(axis_value)
+    o = np.asarray(origin, np.float32)
+    pts = []
+    h = np.float32(0.0)
+    while h < np.float32(height):
+        i = np.float32(0.0)
+        while i < n:
+            a = np.float32(i * theta_hat)
+            c, sn = np.float32(r * np.cos(a)), np.float32(r * np.sin(a))
+            if axis == "x":
+                pts.append((av + h, o[0] + c, o[1] + sn))
+            elif axis == "y":
+                pts.append((o[1] + sn, av + h, o[0] + c))
+            else:
+                pts.append((o[0] + c, o[1] + sn, av + h))
+            i = np.float32(i + np.float32(1.0))
+        h = np.float32(h + s)
+    return np.asarray(pts, np.float32).reshape(-1, 3)
+
+
+def reference_params(space: float, time_step: float, relaxation_scale: float, **overrides) -> StepParams:
+    """The parameter derivation of the reference's simulators
+    (simulator/dam/pbf_dam_init_cond.cpp:11-27,
+    simulator/sphere/pbf_sphere_init_cond.cpp:11-27): rho0 = 1000,
+    m = space^3 * rho0 (double), h = 1.4 space, relaxation = scale / space^2,
+    XSPH c = 0.1, vorticity eps = 0.005; the step as shipped (REFERENCE
+    s_corr, no XSPH or vorticity)."""
+    s = np.float32(space)
+    m = np.float32(float(s) ** 3 * 1000.0)
+    p = StepParams(
+        stable_density=1000.0,
+        stable_distance=float(s),
+        particle_mass=float(m),
+        smoothing_length=float(np.float32(np.float32(1.4) * s)),
+        time_step=float(np.float32(time_step)),
+        relaxation=float(np.float32(np.float32(relaxation_scale) / np.float32(s * s))),
+        xsph_parameter=float(np.float32(0.1)),
+        vc_parameter=float(np.float32(0.005)),
+        scorr_mode=SCORR_REFERENCE,
+        xsph_on=0,
+        vort_on=0,
+    )
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def reference_dam(space: float = 0.16) -> Scene:
+    """The reference's dam break (simulator/dam/pbf_dam_sim.cpp:10-66 with
+    pbf_dam_init_cond.cpp): the fluid block (2.25)..(5.65) as a lattice of
+    `space`, domain (1)..(10), the no-boundary step (the fixed sphere at
+    (3,3,3), radius 3, pbf_constraint.cu:262-315), 3 iterations, dt 0.015.
+    The simulation area is the domain: the shipped sim passes a zero-size
+    area (a shadowed member), which would delete every particle."""
+    p = reference_params(space, 0.015, 1.0)
+    x = cartesian_volume_range((2.25, 2.25, 2.25), (5.65, 5.65, 5.65), space)
+    lo = np.full(3, 1.0, np.float32)
+    hi = np.full(3, 10.0, np.float32)
+    return Scene(name=f"reference_dam_s{space}", x=x, v=np.zeros_like(x), params=p, iters=3,
+                 box_lo=lo, box_hi=hi, area_lo=lo, area_hi=hi,
+                 grid=calc_grid_size(2.0 * float(np.float32(p.smoothing_length)), hi),
+                 plane_points=np.zeros((0, 3), np.float32), plane_normals=np.zeros((0, 3), np.float32),
+                 inner_spheres=np.array([[3.0, 3.0, 3.0, 3.0]], np.float32))
+
+
+JET_CAPACITY = 60000
+
+
+def reference_sphere_jet(space: float = 0.1) -> Tuple[Scene, np.ndarray, np.ndarray]:
+    """The reference application's scene (main.cpp:81, pbf_sphere_sim(0.1);
+    simulator/sphere/pbf_sphere_sim.cpp:281-327, pbf_sphere_init_cond.cpp):
+    an inner sphere (3,3,3), radius 3, as the boundary, domain (0.05)..(7),
+    dt 0.007, relaxation 2/space^2, 3 iterations, and a jet: the rectangle
+    x = 1.25, (y, z) in (3.5, 2.5)..(4.5, 3.5) at `space`, velocity
+    (2.8, 0, 0).  Returns the scene (holding one jet batch, the state after
+    construction), and the jet's positions and velocities; see
+    jet_emits()."""
+    p = reference_params(space, 0.007, 2.0)
+    jx = cartesian_rectangle("x", 1.25, (3.5, 2.5), (4.5, 3.5), space)
+    jv = np.tile(np.array([2.8, 0.0, 0.0], np.float32), (jx.shape[0], 1))
+    lo = np.full(3, 0.05, np.float32)
+    hi = np.full(3, 7.0, np.float32)
+    sc = Scene(name=f"reference_sphere_jet_s{space}", x=jx.copy(), v=jv.copy(), params=p, iters=3,
+               box_lo=lo, box_hi=hi, area_lo=lo, area_hi=hi,
+               grid=calc_grid_size(2.0 * float(np.float32(p.smoothing_length)), hi),
+               plane_points=np.zeros((0, 3), np.float32), plane_normals=np.zeros((0, 3), np.float32),
+               inner_spheres=np.array([[3.0, 3.0, 3.0, 3.0]], np.float32))
+    return sc, jx, jv
+
+
+def jet_emits(step: int, num: int, capacity: int = JET_CAPACITY) -> bool:
+    """Whether the jet adds a batch before step `step` (1-based: the counter
+    is raised before the test, pbf_sphere_sim.cpp:357-387): every 8th step
+    while fewer than 80% of the capacity are live."""
+    return step % 8 == 0 and num < capacity * 0.8
+
+
 CONFIGS = {0: config0, 1: config1, 2: config2, 4: config4}

@@ -346,6 +346,15 @@ pbfx_status pbfx_adj_backward(pbfx_ctx* ctx, const double* gx, const double* gv,
  * Device pointers. */
 pbfx_status pbfx_selftest_arith(const float* a, const float* b, float* out, uint32_t n, void* stream);
 
+/* Self-test of the solver's 1/sqrt: over every float whose bits lie in
+ * [lo_bits, hi_bits], compares the kernels' inv_sqrt (v_sqrt_f32 and v_rcp_f32
+ * with correction steps) with 1.f / sqrtf in IEEE arithmetic, adds the number
+ * of inputs whose results differ in any bit to *mismatches and, if there are
+ * any, stores one such input's bits in *first_bits.  Device pointers; the
+ * caller zeroes *mismatches.  No counterpart in the reference (diagnostic). */
+pbfx_status pbfx_selftest_inv_sqrt(uint32_t lo_bits, uint32_t hi_bits, unsigned long long* mismatches,
+                                   uint32_t* first_bits, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

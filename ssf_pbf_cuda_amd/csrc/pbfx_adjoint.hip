@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_adj_ckey(StepTape t, AdjBufs a, Grid G,
         grid_pos(f3of(t.PI[s]), G.cw, g);
         k = grid_hash(g[0], g[1], g[2], G);
     }
-    const uint32_t r = wave_rank(a.ccount, k, active);
+    const uint32_t r = wave_rank(a.ccount, k, active, k);
     if (active) {
         mark_tile(a.ctflag, k >> lgT);
         a.ckey[s] = k;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_adj_last(StepTape t, AdjBufs a, Grid G,
     const uint32_t n = t.ncnt[l];
     uint2 out = make_uint2(255u, 0u);
     if (n == (uint32_t)P.cap && n > 0) {
-        const uint32_t jl = t.nlist[(size_t)(n - 1u) * cap + l];
+        const uint32_t jl = t.nlist[list_at(cap, l, n - 1u)];
         int gl[3], gj[3];
         grid_pos(f3of(t.Q[l]), G.cw, gl);
         grid_pos(f3of(t.Q[jl]), G.cw, gj);
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(256) void k_adj_coll(StepTape t, AdjBufs a, const f
         const uint32_t n = t.ncnt[i];
         F3 sum = mk3(0.f, 0.f, 0.f);
         for (uint32_t q = 0; q < n; ++q) {
-            const float4 pe = Q[t.nlist[(size_t)q * cap + i]];
+            const float4 pe = Q[t.nlist[list_at(cap, i, q)]];
             float W;
             F3 kg;
             pair_terms(self, f3of(pe), P, W, kg);
@@ -806,7 +806,7 @@ __global__ __launch_bounds__(256) void k_adj_lam(StepTape t, AdjBufs a, const fl
     D3 kgs = dz3(), apos = dz3();
     const uint32_t n = t.ncnt[i];
     for (uint32_t q = r.lane; q < n; q += r.stride) {
-        const float4 pe = Q[t.nlist[(size_t)q * cap + i]];
+        const float4 pe = Q[t.nlist[list_at(cap, i, q)]];
         const D3 d = self - wid(pe);
         PairAdj pa;
         if (!pair_adj(d, k, pa)) continue;  // r = 0: W = a (constant), kg = 0
@@ -868,7 +868,7 @@ __global__ __launch_bounds__(256) void k_adj_pos(StepTape t, AdjBufs a, const fl
     if (ci.x != 0.0 || g2i != 0.0) {
         const uint32_t n = t.ncnt[i];
         for (uint32_t q = r.lane; q < n; q += r.stride) {
-            const D3 d = self - wid(Q[t.nlist[(size_t)q * cap + i]]);
+            const D3 d = self - wid(Q[t.nlist[list_at(cap, i, q)]]);
             PairAdj pa;
             if (!pair_adj(d, k, pa)) continue;
             apos = apos + sc3(d, ci.x * pa.gw) + pa.H(md3(ci.y, ci.z, ci.w) + sc3(pa.kg, g2i));

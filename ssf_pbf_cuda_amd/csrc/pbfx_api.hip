@@ -164,7 +164,7 @@ Phys make_phys(const pbfx_params& p) {
     P.vc = p.vc_parameter;
     P.warg = p.vort_h_as_inv_h ? P.h : P.inv_h;
     P.cap = p.max_pair - 1;
-    P.stride = P.cap;
+    P.stride = (P.cap + 3) & ~3;  // whole groups of four list entries
     P.max_sweeps = p.max_sweeps;
     P.outer_as_inner = p.outer_as_inner;
     P.cut2_w = support_bound2(P.inv_h);
@@ -1291,6 +1291,13 @@ pbfx_status pbfx_adj_backward(pbfx_ctx* c, const double* gx, const double* gv, d
     gparam[1] = gp[1];
     *exponent = E;
     return PBFX_OK;
+}
+
+pbfx_status pbfx_selftest_inv_sqrt(uint32_t lo_bits, uint32_t hi_bits, unsigned long long* mismatches,
+                                   uint32_t* first_bits, void* stream) {
+    if (!mismatches || !first_bits || lo_bits > hi_bits) return PBFX_INVALID_ARG;
+    hipError_t e = launch_selftest_inv_sqrt(lo_bits, hi_bits, mismatches, first_bits, (hipStream_t)stream);
+    return e == hipSuccess ? PBFX_OK : PBFX_HIP_ERROR;
 }
 
 pbfx_status pbfx_selftest_arith(const float* a, const float* b, float* out, uint32_t n, void* stream) {

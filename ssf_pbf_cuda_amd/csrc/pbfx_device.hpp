@@ -162,12 +162,56 @@ __device__ __forceinline__ float powi(float t, int n) {
     return result;
 }
 
+// Element index of entry k of slot i's frozen list.  Lists are stored in
+// groups of four entries: entries 4q..4q+3 of every slot form the 16-byte
+// word q * cap + i, so that one wave reads the next four entries of its 64
+// slots as one coalesced 1 KiB load.
+__device__ __forceinline__ size_t list_at(uint32_t cap, uint32_t i, uint32_t k) {
+    return (((size_t)(k >> 2) * cap + i) << 2) | (k & 3u);
+}
+
+// The word holding entries k..k+3 (k a multiple of 4) of slot i's list.
+__device__ __forceinline__ uint4 list_word(const uint32_t* __restrict__ nl, uint32_t cap, uint32_t i, uint32_t k) {
+    return reinterpret_cast<const uint4*>(nl)[(size_t)(k >> 2) * cap + i];
+}
+
+// The indices of word e = entries k..k+3 (k < n); entries at or past n are
+// replaced by entry k, so every index returned is a written one.
+__device__ __forceinline__ void list_group(uint4 e, uint32_t k, uint32_t n, uint32_t j[4]) {
+    j[0] = e.x;
+    j[1] = (k + 1u < n) ? e.y : e.x;
+    j[2] = (k + 2u < n) ? e.z : e.x;
+    j[3] = (k + 3u < n) ? e.w : e.x;
+}
+
+// 1.f / sqrtf(r2) with both operations correctly rounded -- the IEEE
+// result -- in about a third of the instructions of the IEEE sequences.  On
+// [2^-96, FLT_MAX]: v_sqrt_f32, then the two-sided fma test of its
+// neighbours (s - 1 ulp is taken when fma(-(s - ulp), s, r2) <= 0, s + 1 ulp
+// when fma(-(s + ulp), s, r2) > 0), then v_rcp_f32 and one Newton step
+// y + y * fma(-s, y, 1).  Outside that range (0, below 2^-96, inf, NaN) the
+// IEEE operations themselves.  pbfx_selftest_inv_sqrt compares this function
+// with the IEEE form on every float of the range.
+__device__ __forceinline__ float inv_sqrt(float r2) {
+    if (__builtin_expect(r2 >= 0x1p-96f && r2 <= 0x1.fffffep+127f, 1)) {
+        const float s0 = __builtin_amdgcn_sqrtf(r2);
+        const uint32_t si = __float_as_uint(s0);
+        const float sd = __uint_as_float(si - 1u), su = __uint_as_float(si + 1u);
+        const float vd = __builtin_fmaf(-sd, s0, r2), vu = __builtin_fmaf(-su, s0, r2);
+        float s = (vd <= 0.f) ? sd : s0;
+        s = (vu > 0.f) ? su : s;
+        const float y = __builtin_amdgcn_rcpf(s);
+        return __builtin_fmaf(__builtin_fmaf(-s, y, 1.f), y, y);
+    }
+    return 1.f / __builtin_sqrtf(r2);
+}
+
 // pbf/solver/cuda/pbf_kernel_memorization.cu:15-56 (memorizeKernelPair):
 // W and grad W of one listed pair (rsqrtf restated as 1/sqrtf).
 __device__ __forceinline__ void pair_terms(F3 self, F3 pj, const Phys& P, float& W, F3& kg) {
     F3 d = self - pj;
     float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
-    float inv_r = 1.f / __builtin_sqrtf(r2);
+    float inv_r = inv_sqrt(r2);
     float q2 = r2 * P.inv_h * P.inv_h;
     W = 0.f;
     if (q2 <= 1.f) {
@@ -339,12 +383,13 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
     return off + x - v;
 }
 
-// atomicAdd(&ctr[k], 1) per active lane, returning a unique rank per lane
-// (ranks among equal keys are arbitrary; the reorder makes them stable).
-// Runs of equal keys in consecutive lanes share one atomic, issued by the
-// run's first lane; all runs' atomics are in flight together.  The active
-// lanes must be a prefix of the wave.
-__device__ __forceinline__ uint32_t wave_rank(uint32_t* ctr, uint32_t k, bool active) {
+// atomicAdd(&ctr[slot], 1) per active lane, where slot is where key k's
+// counter is stored, returning a unique rank per lane (ranks among equal keys
+// are arbitrary; the reorder makes them stable).  Runs of equal keys in
+// consecutive lanes share one atomic, issued by the run's first lane; all
+// runs' atomics are in flight together.  The active lanes must be a prefix of
+// the wave.
+__device__ __forceinline__ uint32_t wave_rank(uint32_t* ctr, uint32_t k, bool active, uint32_t slot) {
     const uint32_t lane = lane_id();
     const uint32_t nact = (uint32_t)__popcll(__ballot(active));
     const uint32_t kp = (uint32_t)__shfl_up((int)k, 1, 64);
@@ -355,7 +400,7 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t* ctr, uint32_t k, bool ac
     if (head) {
         const uint64_t above = heads & ~upto;
         const uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : nact;
-        base = atomicAdd(&ctr[k], end - lane);
+        base = atomicAdd(&ctr[slot], end - lane);
     }
     const int hl = 63 - __clzll((unsigned long long)(heads & upto));
     base = (uint32_t)__shfl((int)base, hl < 0 ? 0 : hl, 64);

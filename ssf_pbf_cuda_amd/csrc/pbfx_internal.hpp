@@ -7,8 +7,9 @@
 //   X                              f32[3*cap]    SoA sorted old positions
 //   R                              f32x4[2*cap]  per slot: (x*, 0), (v, 0) -- the final passes' own record
 //   OM4                            f32x4[cap]    per slot: (omega, |omega|)
-//   nlist, ncnt                    u32[cap*kcap], u32[cap]  frozen neighbour lists as sorted
-//                                                 slots, column-major (entry k of slot i at k*cap + i)
+//   nlist, ncnt                    u32[cap*stride], u32[cap]  frozen neighbour lists as sorted
+//                                                 slots in groups of four: entries 4q..4q+3 of slot
+//                                                 i are the 16-byte word q*cap + i (list_at)
 //   count                          u32[C+1]      cell histogram (cell C = deleted)
 //   offset                         u32[C+2]      exclusive scan: cell c owns [off[c], off[c+1])
 //   tsum, tflag, tprefix           u32[nT+1]     per-4096-cell-tile totals, occupied flags, scan
@@ -16,7 +17,7 @@
 //   ckey, crank, cperm, ccount, coffset, ctsum, ctprefix, cunits
 //                                                 the same sort keyed by each particle's final cell
 //   flist, fcnt                    u32[64*cap], u32[cap]  in-range pairs of the final passes by
-//                                                 final-order position (column-major), counts
+//                                                 final-order position (groups of four, list_at), counts
 //   ctr                            u32[16*65]    allocation counters, then 64 copies of the
 //                                                 statistics counters of the last step
 #pragma once
@@ -77,7 +78,7 @@ struct Phys {
     float g[3];
     float xsph_c, vc, warg; // warg: inv_h, or h to reproduce the vorticity argument mix-up
     int32_t cap;            // neighbours kept per particle (max_pair - 1)
-    int32_t stride;         // list entries allocated per particle (= cap)
+    int32_t stride;         // list entries allocated per particle (cap rounded up to a multiple of 4)
     int32_t max_sweeps;
     int32_t outer_as_inner;
     float cut2_w, cut2_wd;  // squared support bounds of W (arg inv_h) and of W' (arg warg): pairs beyond add 0
@@ -139,7 +140,7 @@ struct Bufs {
     uint32_t* ctflag;
     uint32_t* ctprefix;
     uint2* cunits;
-    uint32_t* flist;   // final pair lists, column-major [kFinalK][cap]
+    uint32_t* flist;   // final pair lists, kFinalK entries per position in groups of four (list_at)
     uint32_t* fcnt;    // pairs per final-order position
     uint32_t* ctr;
     uint32_t lgT;
@@ -262,6 +263,8 @@ hipError_t launch_neighbors_view(const Bufs& b, uint32_t* out, uint32_t* counts,
 hipError_t launch_cell_ranges(const Bufs& b, uint32_t C, uint32_t* start, uint32_t* end, hipStream_t s);
 hipError_t launch_copy_lambda(const Bufs& b, float* out, hipStream_t s);
 hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t n, hipStream_t s);
+hipError_t launch_selftest_inv_sqrt(uint32_t lo, uint32_t hi, unsigned long long* mismatches, uint32_t* first,
+                                    hipStream_t s);
 hipError_t launch_density(const Bufs& b, const Scene& s0, const float* x, float* rho, hipStream_t s);
 hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float* xo, float* vo, hipStream_t s);
 // Fills T.grid[] (and T.grid_units) for num_cu compute units.

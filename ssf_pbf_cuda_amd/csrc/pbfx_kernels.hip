@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void k_predict_hash(const float* __restrict__ 
             S.hi[2] < q[2])
             k = b.CT;
     });
-    const uint32_t r = wave_rank(count, k, active);
+    const uint32_t r = wave_rank(count, k, active, k);
     if (active) {
         mark_tile(tflag, k >> lgT);
         key[i] = k;
@@ -66,12 +66,21 @@ __global__ __launch_bounds__(256) void k_predict_hash(const float* __restrict__ 
     }
 }
 
+// Where the counter of key c is stored.  lgnc = 0: at c.  Otherwise keys are
+// (run << lgnc | copy) and copy j of every run lives in region j of C >> lgnc
+// words, so that the copies of one run sit on different cache lines: a run
+// that gathers particles from the whole domain (a wall or corner pile) is
+// counted through nc separate lines instead of one.
+__device__ __forceinline__ uint32_t count_slot(uint32_t c, uint32_t C, uint32_t lgnc) {
+    return (c & ((1u << lgnc) - 1u)) * (C >> lgnc) + (c >> lgnc);
+}
+
 // ------------------------------------------------- K2a: per-tile totals
 // Totals of the cell counts of each occupied scan tile (unoccupied tiles are
 // all zero); tile nT is the deleted bucket (cell C).
 __global__ __launch_bounds__(256) void k_tile_reduce(const uint32_t* __restrict__ count,
                                                      const uint32_t* __restrict__ tflag, uint32_t* __restrict__ tsum,
-                                                     uint32_t nT, uint32_t lgT, uint32_t C) {
+                                                     uint32_t nT, uint32_t lgT, uint32_t C, uint32_t lgnc) {
     __shared__ uint32_t wsum[kWaves];
     const uint32_t t = blockIdx.x;
     if (t == nT) {
@@ -84,7 +93,7 @@ __global__ __launch_bounds__(256) void k_tile_reduce(const uint32_t* __restrict_
     }
     const uint32_t T = 1u << lgT, c0 = t << lgT;
     uint32_t s = 0;
-    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) s += count[c0 + j];
+    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) s += count[count_slot(c0 + j, C, lgnc)];
     uint32_t total;
     block_excl_scan(s, wsum, total);
     if (threadIdx.x == 0) tsum[t] = total;
@@ -131,7 +140,7 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict_
 __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
                                                     const uint32_t* __restrict__ tsum, uint32_t* __restrict__ tflag,
                                                     const uint32_t* __restrict__ tprefix, uint32_t nT,
-                                                    uint32_t lgT, uint32_t C) {
+                                                    uint32_t lgT, uint32_t C, uint32_t lgnc) {
     __shared__ uint32_t wsum[kWaves];
     const uint32_t t = blockIdx.x;
     if (t == nT) {
@@ -155,14 +164,15 @@ __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count
         return;
     }
     uint32_t s = 0;
-    for (uint32_t j = 0; j < myn; ++j) s += count[my0 + j];
+    for (uint32_t j = 0; j < myn; ++j) s += count[count_slot(my0 + j, C, lgnc)];
     uint32_t total;
     uint32_t run = base + block_excl_scan(s, wsum, total);
     for (uint32_t j = 0; j < myn; ++j) {
-        uint32_t c = count[my0 + j];
+        const uint32_t cs = count_slot(my0 + j, C, lgnc);
+        uint32_t c = count[cs];
         offset[my0 + j] = run;
         run += c;
-        count[my0 + j] = 0;
+        count[cs] = 0;
     }
     if (threadIdx.x == 0) tflag[t] = 0;
 }
@@ -398,8 +408,8 @@ __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, 
 // detectNeighborsCUDA (pbf_neighbor_search_util.cu:361-482): per particle the
 // 27 cells in z,y,x order, slots ascending, keep r2 < h^2 while fewer than
 // cap are kept; the list is frozen for all Jacobi iterations.  Stored as
-// sorted slots, column-major: nlist[k * cap_particles + slot], with the
-// count in ncnt[slot].  Candidates come from the unit's staged tile, four
+// sorted slots in groups of four (list_at), with the count in
+// ncnt[slot].  Candidates come from the unit's staged tile, four
 // at a time.
 template <bool kB>
 __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t tile_cap) {
@@ -445,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t t
                             const F3 d = self - pj[i];
                             const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
                             if (!trunc && t + i < hi && r2 < h2) {
-                                if (cnt < ncap) nl[(size_t)cnt++ * cap + g] = gb + t + i;
+                                if (cnt < ncap) nl[list_at(cap, g, cnt++)] = gb + t + i;
                                 else trunc = true;
                             }
                         }
@@ -481,10 +491,14 @@ __global__ __launch_bounds__(256) void k_lambda(Bufs b, Scenes<kB> SC) {
         const uint32_t n = b.ncnt[i];
         float ks = 0.f, kg2s = 0.f;
         F3 kgs = mk3(0.f, 0.f, 0.f);
+        uint4 e = n ? list_word(nl, cap, i, 0u) : make_uint4(0u, 0u, 0u, 0u);
         for (uint32_t k = 0; k < n; k += 4u) {
+            uint32_t j[4];
+            list_group(e, k, n, j);
             F3 pj[4];
 #pragma unroll
-            for (uint32_t m = 0; m < 4u; ++m) pj[m] = f3of(Pf[nl[(size_t)min(k + m, n - 1u) * cap + i]]);
+            for (uint32_t m = 0; m < 4u; ++m) pj[m] = f3of(Pf[j[m]]);
+            e = list_word(nl, cap, i, min(k + 4u, (n - 1u) & ~3u));  // next group, in flight during this one
 #pragma unroll
             for (uint32_t m = 0; m < 4u; ++m)
                 if (k + m < n) {
@@ -526,10 +540,14 @@ __global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
         const float self_cl = (si < 0.f) ? -si : 0.f;
         const uint32_t n = b.ncnt[i];
         F3 sum = mk3(0.f, 0.f, 0.f);
+        uint4 e = n ? list_word(nl, cap, i, 0u) : make_uint4(0u, 0u, 0u, 0u);
         for (uint32_t k = 0; k < n; k += 4u) {
+            uint32_t j[4];
+            list_group(e, k, n, j);
             float4 pe[4];
 #pragma unroll
-            for (uint32_t m = 0; m < 4u; ++m) pe[m] = Q[nl[(size_t)min(k + m, n - 1u) * cap + i]];
+            for (uint32_t m = 0; m < 4u; ++m) pe[m] = Q[j[m]];
+            e = list_word(nl, cap, i, min(k + 4u, (n - 1u) & ~3u));  // next group, in flight during this one
 #pragma unroll
             for (uint32_t m = 0; m < 4u; ++m)
                 if (k + m < n) {
@@ -579,7 +597,8 @@ __global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
 // v = inv_dt*(x* - x_sorted) (updateVelocityCUDA, pbf_update.cu:25-39), so
 // that a particle visited in run order costs one cache line.
 template <bool kB>
-__global__ __launch_bounds__(256) void k_cur_hash(Bufs b, Scenes<kB> SC, uint32_t lguw, uint32_t lgnc, uint32_t lgT) {
+__global__ __launch_bounds__(256) void k_cur_hash(Bufs b, Scenes<kB> SC, uint32_t lguw, uint32_t lgnc, uint32_t lgT,
+                                                  uint32_t CR) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = s < b.offset[b.CT];
     uint32_t k = 0;
@@ -594,7 +613,7 @@ __global__ __launch_bounds__(256) void k_cur_hash(Bufs b, Scenes<kB> SC, uint32_
         grid_pos(p, G.cw, gp);
         k = ((scene_key(gp[0], gp[1], gp[2], G) >> lguw) << lgnc) | (blockIdx.x & ((1u << lgnc) - 1u));
     });
-    const uint32_t r = wave_rank(b.ccount, k, active);
+    const uint32_t r = wave_rank(b.ccount, k, active, count_slot(k, CR, lgnc));
     if (active) {
         mark_tile(b.ctflag, k >> lgT);
         b.ckey[s] = k;
@@ -788,7 +807,7 @@ __device__ __forceinline__ float4 pos_slot(const float4* __restrict__ R, uint32_
 
 // ------------------------------------------------- K9: final pair lists
 // For each particle (by position q of the final-cell order) the slots of its
-// in-range pairs, column-major: flist[k * cap + q], k < kFinalK; fcnt[q] =
+// in-range pairs in groups of four (list_at(cap, q, k), k < kFinalK); fcnt[q] =
 // the full count (above kFinalK: the passes fall back to the direct sum).
 template <bool kB>
 __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, uint32_t tile_cap, uint32_t lguw, uint32_t lgnc) {
@@ -804,7 +823,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         const F3 self = f3of(R[2 * g]);
         uint32_t n = 0;
         auto emit = [&](uint32_t j, bool first) {
-            if (n < kFinalK) fl[(size_t)n * cap + q] = j | (first ? 0x80000000u : 0u);
+            if (n < kFinalK) fl[list_at(cap, q, n)] = j | (first ? 0x80000000u : 0u);
             ++n;
         };
         if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, emit);
@@ -850,13 +869,14 @@ template <class ElemF, class TermF>
 __device__ __forceinline__ F3 sum_list(const uint32_t* __restrict__ fl, uint32_t cap, uint32_t q, uint32_t n,
                                        ElemF&& elem, TermF&& term) {
     F3 sum = mk3(0.f, 0.f, 0.f), cs = mk3(0.f, 0.f, 0.f);
+    uint4 w = n ? list_word(fl, cap, q, 0u) : make_uint4(0u, 0u, 0u, 0u);
     for (uint32_t k = 0; k < n; k += 4u) {
         uint32_t e[4];
         float4 el[4];
-#pragma unroll
-        for (uint32_t i = 0; i < 4u; ++i) e[i] = fl[(size_t)min(k + i, n - 1u) * cap + q];
+        list_group(w, k, n, e);
 #pragma unroll
         for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(e[i] & 0x7FFFFFFFu);
+        w = list_word(fl, cap, q, min(k + 4u, (n - 1u) & ~3u));  // next group, in flight during this one
 #pragma unroll
         for (uint32_t i = 0; i < 4u; ++i)
             if (k + i < n) {
@@ -1044,7 +1064,7 @@ __global__ __launch_bounds__(256) void k_neighbors_view(Bufs b, uint32_t* __rest
     const uint32_t cnt = b.ncnt[g];
     counts[g] = cnt;
     for (uint32_t c = 0; c < max_pair; ++c)
-        out[(size_t)g * max_pair + c] = c < cnt ? b.nlist[(size_t)c * b.cap + g] : 0xFFFFFFFFu;
+        out[(size_t)g * max_pair + c] = c < cnt ? b.nlist[list_at(b.cap, g, c)] : 0xFFFFFFFFu;
 }
 
 __global__ void k_cell_ranges(const uint32_t* __restrict__ offset, uint32_t C, uint32_t* __restrict__ start,
@@ -1076,7 +1096,7 @@ __global__ void k_density(Bufs b, Scenes<kB> SC, const float* __restrict__ x, fl
     const uint32_t n = b.ncnt[i];
     float sum = 0.f;
     for (uint32_t k = 0; k < n; ++k) {
-        const uint32_t j = b.nlist[(size_t)k * b.cap + i];
+        const uint32_t j = b.nlist[list_at(b.cap, i, k)];
         const F3 d = self - mk3(x[3 * j], x[3 * j + 1], x[3 * j + 2]);
         sum += wpoly6(glm_length(d), P.inv_h) * P.m;
     }
@@ -1113,6 +1133,27 @@ __global__ void k_selftest(const float* __restrict__ a, const float* __restrict_
     out[4 * i + 1] = __builtin_sqrtf(x);
     out[4 * i + 2] = __builtin_fmaf(x, y, x);
     out[4 * i + 3] = floorf(x / y);
+}
+
+// inv_sqrt against 1.f / sqrtf on every bit pattern in [lo, hi] (grid-stride):
+// counts the mismatches and records the first one's input bits (any one, if
+// several race).
+__global__ __launch_bounds__(256) void k_selftest_inv_sqrt(uint32_t lo, uint32_t hi,
+                                                           unsigned long long* __restrict__ mismatches,
+                                                           uint32_t* __restrict__ first) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t bad = 0, which = 0;
+    for (uint64_t u = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; u <= hi; u += stride) {
+        const float r2 = __uint_as_float((uint32_t)u);
+        if (__float_as_uint(inv_sqrt(r2)) != __float_as_uint(1.f / __builtin_sqrtf(r2))) {
+            if (!bad) which = (uint32_t)u;
+            ++bad;
+        }
+    }
+    if (bad) {
+        atomicAdd(mismatches, (unsigned long long)bad);
+        atomicExch(first, which);
+    }
 }
 
 
@@ -1190,13 +1231,13 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     }
     {
         Mark m(tm, s, K_SCAN_TILES);
-        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.tflag, b.tsum, b.nT, b.lgT, CT);
+        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.tflag, b.tsum, b.nT, b.lgT, CT, 0u);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.tsum, b.tprefix, b.nT + 1, b.ctr, 1u);
     }
     {
         Mark m(tm, s, K_SCAN_CELLS);
         hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.offset, b.tsum, b.tflag,
-                           b.tprefix, b.nT, b.lgT, CT);
+                           b.tprefix, b.nT, b.lgT, CT, 0u);
     }
     if (n == 0) return hipGetLastError();
     {
@@ -1237,11 +1278,12 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     const uint32_t nTr = CR >> lgTr;
     {
         Mark m(tm, s, K_CUR_SORT);
-        hipLaunchKernelGGL(k_cur_hash<kB>, dim3(nb), dim3(256), 0, s, b, SC, lguw, lgnc, lgTr);
-        hipLaunchKernelGGL(k_tile_reduce, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, nTr, lgTr, CR);
+        hipLaunchKernelGGL(k_cur_hash<kB>, dim3(nb), dim3(256), 0, s, b, SC, lguw, lgnc, lgTr, CR);
+        hipLaunchKernelGGL(k_tile_reduce, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, nTr, lgTr, CR,
+                           lgnc);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.ctsum, b.ctprefix, nTr + 1, b.ctr, 0u);
         hipLaunchKernelGGL(k_scan_cells, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
-                           b.ctprefix, nTr, lgTr, CR);
+                           b.ctprefix, nTr, lgTr, CR, lgnc);
         hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, CT, b.ckey, b.crank, b.coffset,
                            b.cperm, uw, lguw, lgnc, b.cunits, b.ctr);
     }
@@ -1282,9 +1324,9 @@ hipError_t launch_step(const Bufs& b, const Scene& s0, const Tuning& T, float* x
 
 hipError_t launch_count_scan(uint32_t* count, uint32_t* offset, uint32_t* tsum, uint32_t* tflag, uint32_t* tprefix,
                              uint32_t nT, uint32_t lgT, uint32_t C, hipStream_t s) {
-    hipLaunchKernelGGL(k_tile_reduce, dim3(nT + 1), dim3(256), 0, s, count, tflag, tsum, nT, lgT, C);
+    hipLaunchKernelGGL(k_tile_reduce, dim3(nT + 1), dim3(256), 0, s, count, tflag, tsum, nT, lgT, C, 0u);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tsum, tprefix, nT + 1, (uint32_t*)nullptr, 0u);
-    hipLaunchKernelGGL(k_scan_cells, dim3(nT + 1), dim3(256), 0, s, count, offset, tsum, tflag, tprefix, nT, lgT, C);
+    hipLaunchKernelGGL(k_scan_cells, dim3(nT + 1), dim3(256), 0, s, count, offset, tsum, tflag, tprefix, nT, lgT, C, 0u);
     return hipGetLastError();
 }
 
@@ -1324,6 +1366,12 @@ hipError_t launch_density(const Bufs& b, const Scene& s0, const float* x, float*
 
 hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float* xo, float* vo, hipStream_t s) {
     hipLaunchKernelGGL(k_restore, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b, xs, vs, xo, vo);
+    return hipGetLastError();
+}
+
+hipError_t launch_selftest_inv_sqrt(uint32_t lo, uint32_t hi, unsigned long long* mismatches, uint32_t* first,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(k_selftest_inv_sqrt, dim3(16384), dim3(256), 0, s, lo, hi, mismatches, first);
     return hipGetLastError();
 }
 

@@ -145,6 +145,7 @@ def lib():
             "pbfx_last_hip_error": ([vp], ctypes.c_int),
             "pbfx_status_string": ([ctypes.c_int], ctypes.c_char_p),
             "pbfx_selftest_arith": ([vp, vp, vp, u32, vp], ctypes.c_int),
+            "pbfx_selftest_inv_sqrt": ([u32, u32, vp, vp, vp], ctypes.c_int),
             "pbfx_set_kernel_timing": ([vp, ctypes.c_int], ctypes.c_int),
             "pbfx_kernel_timing_read": ([vp, vp, ctypes.POINTER(KernelTime), u32, u32p], ctypes.c_int),
             "pbfx_add_particles": ([vp, vp, vp, u32p, vp, vp, u32, vp], ctypes.c_int),
@@ -379,6 +380,20 @@ class Solver:
 def selftest_arith(a, b, out, stream=None) -> None:
     n = int(a.numel())
     _check(lib().pbfx_selftest_arith(_ptr(a), _ptr(b), _ptr(out), n, _stream(stream)), "pbfx_selftest_arith")
+
+
+def selftest_inv_sqrt(lo_bits: int, hi_bits: int, stream=None) -> tuple[int, int]:
+    """Compares the kernels' 1/sqrt with IEEE 1.f/sqrtf on every float whose
+    bits lie in [lo_bits, hi_bits]; returns (mismatches, bits of one
+    mismatching input or 0)."""
+    import torch
+
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    first = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _check(lib().pbfx_selftest_inv_sqrt(lo_bits, hi_bits, _ptr(cnt), _ptr(first), _stream(stream)),
+           "pbfx_selftest_inv_sqrt")
+    torch.cuda.synchronize()
+    return int(cnt.item()), int(first.item()) & 0xFFFFFFFF
 
 
 _hip = None

@@ -53,6 +53,17 @@ def test_device_arithmetic_is_ieee():
         assert bit_mismatches(got[mask, k], ref[mask, k]) == 0, name
 
 
+def test_kernel_inv_sqrt_is_ieee_on_every_float():
+    """The pair terms' 1/sqrt (v_sqrt_f32 and v_rcp_f32 with correction steps
+    on [2^-96, FLT_MAX], the IEEE operations elsewhere) equals IEEE
+    1.f / sqrtf bit for bit on every non-negative float, infinity and a NaN."""
+    from ssf_pbf_cuda_amd import pbfx
+
+    for lo, hi in ((0x00000000, 0x7F800000), (0x7FC00000, 0x7FC00000)):
+        bad, first = pbfx.selftest_inv_sqrt(lo, hi)
+        assert bad == 0, (hex(lo), hex(hi), bad, hex(first))
+
+
 @pytest.mark.parametrize("shape", [(12, 10, 8), (7, 5, 3), (1, 1, 1), (33, 9, 4)])
 def test_neighbor_search_bit_exact(shape):
     sc = small(*shape)

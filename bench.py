@@ -111,6 +111,7 @@ def run_gpu(scene, steps, warmup, kernel_steps, tile_cap, unit_width):
     xs = [x0.clone() for _ in range(nbuf)]
     vs = [v0.clone() for _ in range(nbuf)]
     if tile_cap is None:
+        log(f"{scene.name}: autotuning the LDS tile and unit width")
         tile_cap, unit_width = solver.autotune(x0, v0, n, lo, hi, iters, xs[0], vs[0], stream=stream)
         xs[0].copy_(x0)
         vs[0].copy_(v0)

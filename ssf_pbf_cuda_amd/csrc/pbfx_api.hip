@@ -54,7 +54,7 @@ struct pbfx_ctx {
     bool batch = false;                  // created by pbfx_batch_create
     uint32_t capacity = 0;               // all scenes
     uint32_t csum = 0;                   // cells of all scenes (kbase[n])
-    uint32_t uw_max = 16;                // narrowest grid row of any scene
+    uint32_t uw_max = kMaxW;             // narrowest grid row of any scene
     Bufs b{};
     Tuning T{};
     int num_cu = 0;
@@ -893,7 +893,7 @@ pbfx_status pbfx_autotune(pbfx_ctx* c, const float* x, const float* v, uint32_t 
     uint32_t best_tile = saved.tile_cap, best_uw = saved.unit_width;
     float best = 3.4e38f;
     const uint32_t tiles[] = {1024, 2048, 3072, 4096};
-    const uint32_t widths[] = {4, 8, 16};
+    const uint32_t widths[] = {4, 8, 16, 32};
     pbfx_status st = PBFX_OK;
     for (uint32_t tc : tiles)
         for (uint32_t uw : widths) {

@@ -29,7 +29,7 @@ namespace pbfx {
 
 constexpr int kBlock = 256;               // threads per work-unit workgroup
 constexpr int kWaves = kBlock / 64;
-constexpr uint32_t kMaxW = 16;            // widest work unit along x (cells)
+constexpr uint32_t kMaxW = 32;            // widest work unit along x (cells)
 constexpr uint32_t kMaxCells = 9 * (kMaxW + 2);
 constexpr uint32_t kTileCapMax = 4096;
 constexpr uint32_t kTileIndexLimit = 65535;  // tile indices are stored as u16

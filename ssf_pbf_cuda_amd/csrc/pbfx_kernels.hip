@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ke
 // estimated 27-cell neighbourhood (9 rows of W+2 cells at the run's mean
 // density) exceeds `budget` particles is split into narrower units, and a
 // unit with more than kChunk own particles into several items that share its
-// neighbourhood (item = (first cell, W | chunk << 5)).
+// neighbourhood (item = (first cell, W | chunk << 6)).
 __device__ __forceinline__ void emit_units(const uint32_t* __restrict__ offset, uint32_t k, uint32_t f, uint32_t uw,
                                            uint32_t budget, uint32_t min_own, uint2* __restrict__ units,
                                            uint32_t* __restrict__ nunits) {
@@ -208,7 +208,7 @@ __device__ __forceinline__ void emit_units(const uint32_t* __restrict__ offset, 
         if (own) {
             const uint32_t nch = (own + kChunk - 1u) / kChunk;
             const uint32_t u = atomicAdd(nunits, nch);
-            for (uint32_t c = 0; c < nch; ++c) units[u + c] = make_uint2(kq, W | (c << 5));
+            for (uint32_t c = 0; c < nch; ++c) units[u + c] = make_uint2(kq, W | (c << 6));
         }
     }
 }
@@ -326,41 +326,53 @@ __device__ __forceinline__ void unit_range(uint32_t nU, uint32_t& u0, uint32_t& 
 // Builds the neighbourhood table of a unit: 9 rows (dz outer, dy inner -- the
 // reference's z,y order) of W+2 cells (x0-1 .. x0+W, wrapped), their tile
 // starts and sorted slots, and the runs of cells whose particles are
-// contiguous in memory (one per row unless the row wraps).  Contains
-// barriers: call uniformly.
+// contiguous in memory (one per row unless the row wraps).  Thread t owns
+// cells 2t and 2t+1 (up to 2 * kBlock cells).  Contains barriers: call
+// uniformly.
+static_assert(kMaxCells <= 2u * kBlock, "unit_setup gives each thread two neighbourhood cells");
 __device__ UnitInfo unit_setup(const Grid& G, const uint32_t* __restrict__ offset, uint2 unit, uint32_t tile_cap,
                                UnitShared& S, Tally& tl) {
     UnitInfo I;
-    I.W = unit.y & 31u;
-    const uint32_t chunk = unit.y >> 5;
+    I.W = unit.y & 63u;
+    const uint32_t chunk = unit.y >> 6;
     I.ncell = 9u * (I.W + 2u);
     const uint32_t kb = unit.x - G.base;  // key within the scene's grid
     I.xb = kb & (G.gx - 1u);
     I.y = (kb >> G.lgx) & (G.gy - 1u);
     I.z = kb >> (G.lgx + G.lgy);
-    const uint32_t e = threadIdx.x;
-    uint32_t cnt = 0;
-    if (e < I.ncell) {
-        const uint32_t r = e / (I.W + 2u), kk = e - r * (I.W + 2u);
-        const uint32_t cx = (I.xb + kk - 1u) & (G.gx - 1u);
-        const uint32_t cy = (I.y + (r % 3u) - 1u) & (G.gy - 1u);
-        const uint32_t cz = (I.z + (r / 3u) - 1u) & (G.gz - 1u);
-        const uint32_t key = G.base + (cz * G.gy + cy) * G.gx + cx;
-        const uint32_t gs = offset[key];
-        cnt = offset[key + 1] - gs;
-        S.gstart[e] = gs;
-        S.cnt[e] = cnt;
+    const uint32_t e0 = 2u * threadIdx.x;
+    uint32_t cnt[2] = {0u, 0u};
+#pragma unroll
+    for (uint32_t h = 0; h < 2u; ++h) {
+        const uint32_t e = e0 + h;
+        if (e < I.ncell) {
+            const uint32_t r = e / (I.W + 2u), kk = e - r * (I.W + 2u);
+            const uint32_t cx = (I.xb + kk - 1u) & (G.gx - 1u);
+            const uint32_t cy = (I.y + (r % 3u) - 1u) & (G.gy - 1u);
+            const uint32_t cz = (I.z + (r / 3u) - 1u) & (G.gz - 1u);
+            const uint32_t key = G.base + (cz * G.gy + cy) * G.gx + cx;
+            const uint32_t gs = offset[key];
+            cnt[h] = offset[key + 1] - gs;
+            S.gstart[e] = gs;
+            S.cnt[e] = cnt[h];
+        }
     }
     uint32_t total;
-    const uint32_t ex = block_excl_scan(cnt, S.wsum, total);
-    if (e < I.ncell) S.tstart[e] = ex;
-    if (e == 0) S.tstart[I.ncell] = total;
-    uint32_t flag = 0;
-    if (e < I.ncell) flag = (e == 0 || S.gstart[e] != S.gstart[e - 1] + S.cnt[e - 1]) ? 1u : 0u;
+    const uint32_t ex = block_excl_scan(cnt[0] + cnt[1], S.wsum, total);
+    if (e0 < I.ncell) S.tstart[e0] = ex;
+    if (e0 + 1u < I.ncell) S.tstart[e0 + 1u] = ex + cnt[0];
+    if (threadIdx.x == 0) S.tstart[I.ncell] = total;
+    uint32_t flag[2] = {0u, 0u};
+#pragma unroll
+    for (uint32_t h = 0; h < 2u; ++h) {
+        const uint32_t e = e0 + h;
+        if (e < I.ncell) flag[h] = (e == 0 || S.gstart[e] != S.gstart[e - 1] + S.cnt[e - 1]) ? 1u : 0u;
+    }
     uint32_t nruns;
-    const uint32_t ri = block_excl_scan(flag, S.wsum, nruns);
-    if (flag) S.run_e[ri] = e;
-    if (e == 0) S.run_e[nruns] = I.ncell;
+    const uint32_t ri = block_excl_scan(flag[0] + flag[1], S.wsum, nruns);
+    if (flag[0]) S.run_e[ri] = e0;
+    if (flag[1]) S.run_e[ri + flag[0]] = e0 + 1u;
+    if (threadIdx.x == 0) S.run_e[nruns] = I.ncell;
     __syncthreads();
     const uint32_t base = 4u * (I.W + 2u);
     I.total = total;
@@ -409,8 +421,50 @@ __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, 
 // 27 cells in z,y,x order, slots ascending, keep r2 < h^2 while fewer than
 // cap are kept; the list is frozen for all Jacobi iterations.  Stored as
 // sorted slots in groups of four (list_at), with the count in
-// ncnt[slot].  Candidates come from the unit's staged tile, four
-// at a time.
+// ncnt[slot].  Candidates come from the unit's staged tile, four at a time.
+
+// The filter of one own particle (column kc of the unit, slot g): candidate
+// tile index t is read at src[sb + t] -- the tile itself (kStaged) or the
+// sorted positions, with sb the cell's slot base.  Returns the number kept
+// (at most ncap); trunc is set when an (ncap+1)-th pair exists.  Each group
+// of four candidates gives a 4-bit hit mask whose pairs are stored in order.
+template <bool kStaged>
+__device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, uint32_t kc, F3 self, float h2,
+                                              uint32_t ncap, const float4* __restrict__ src,
+                                              uint32_t* __restrict__ nl, uint32_t cap, uint32_t g, bool& trunc) {
+    uint32_t cnt = 0;
+    for (uint32_t c = 0; c < 27u; ++c) {
+        const uint32_t e = (c / 3u) * (W + 2u) + kc - 1u + (c % 3u);
+        const uint32_t lo = S.tstart[e], hi = S.tstart[e + 1];
+        const uint32_t gb = S.gstart[e] - lo;  // slot = gb + tile index
+        const uint32_t sb = kStaged ? 0u : gb;
+        for (uint32_t t = lo; t < hi; t += 4u) {
+            F3 pj[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) pj[i] = f3of(src[sb + min(t + i, hi - 1u)]);
+            uint32_t m = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) {
+                const F3 d = self - pj[i];
+                const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+                m |= (r2 < h2) ? (1u << i) : 0u;
+            }
+            m &= (hi - t >= 4u) ? 0xFu : ((1u << (hi - t)) - 1u);
+            while (m) {
+                const uint32_t i = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                if (cnt == ncap) {
+                    trunc = true;
+                    return cnt;
+                }
+                nl[list_at(cap, g, cnt)] = gb + t + i;
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
 template <bool kB>
 __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t tile_cap) {
     __shared__ UnitShared S;
@@ -436,31 +490,10 @@ __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t t
             for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
                 const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
                 const uint32_t kc = own_column(S, I.W, ts);
-                const F3 self = f3of(I.staged ? tile[ts] : Pf[g]);
-                uint32_t cnt = 0;
                 bool trunc = false;
-                for (uint32_t c = 0; c < 27u && !trunc; ++c) {
-                    const uint32_t e = (c / 3u) * (I.W + 2u) + kc - 1u + (c % 3u);
-                    const uint32_t lo = S.tstart[e], hi = S.tstart[e + 1];
-                    const uint32_t gb = S.gstart[e] - lo;  // slot = gb + tile index
-                    for (uint32_t t = lo; t < hi && !trunc; t += 4u) {
-                        F3 pj[4];
-#pragma unroll
-                        for (uint32_t i = 0; i < 4u; ++i) {
-                            const uint32_t ti = min(t + i, hi - 1u);
-                            pj[i] = f3of(I.staged ? tile[ti] : Pf[gb + ti]);
-                        }
-#pragma unroll
-                        for (uint32_t i = 0; i < 4u; ++i) {
-                            const F3 d = self - pj[i];
-                            const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
-                            if (!trunc && t + i < hi && r2 < h2) {
-                                if (cnt < ncap) nl[list_at(cap, g, cnt++)] = gb + t + i;
-                                else trunc = true;
-                            }
-                        }
-                    }
-                }
+                const uint32_t cnt =
+                    I.staged ? ns_filter<true>(S, I.W, kc, f3of(tile[ts]), h2, ncap, tile, nl, cap, g, trunc)
+                             : ns_filter<false>(S, I.W, kc, f3of(Pf[g]), h2, ncap, Pf, nl, cap, g, trunc);
                 b.ncnt[g] = cnt;
                 tl.trunc += trunc ? 1u : 0u;
             }
@@ -642,7 +675,7 @@ __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict_
         if (own >= kSparse) {
             const uint32_t nch = (own + kChunk - 1u) / kChunk;
             const uint32_t u = atomicAdd(&ctr[CTR_CUNITS], nch);
-            for (uint32_t c = 0; c < nch; ++c) cunits[u + c] = make_uint2(run << lguw, uw | (c << 5));
+            for (uint32_t c = 0; c < nch; ++c) cunits[u + c] = make_uint2(run << lguw, uw | (c << 6));
         }
     }
 }
@@ -672,13 +705,19 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
             float4 el[4];
 #pragma unroll
             for (uint32_t i = 0; i < 4u; ++i) el[i] = tp[min(t + i, hi - 1u)];
+            uint32_t m = 0;
 #pragma unroll
             for (uint32_t i = 0; i < 4u; ++i) {
                 const F3 d = f3of(el[i]) - self;
-                if (t + i < hi && !(glm_dot(d, d) > cut2)) {
-                    emit(__float_as_uint(el[i].w), first);
-                    first = false;
-                }
+                m |= !(glm_dot(d, d) > cut2) ? (1u << i) : 0u;
+            }
+            m &= (hi - t >= 4u) ? 0xFu : ((1u << (hi - t)) - 1u);
+            while (m) {
+                const uint32_t i = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                const float w = i == 0u ? el[0].w : (i == 1u ? el[1].w : (i == 2u ? el[2].w : el[3].w));
+                emit(__float_as_uint(w), first);
+                first = false;
             }
         }
     }
@@ -712,13 +751,18 @@ __device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __res
                 F3 pj[4];
 #pragma unroll
                 for (uint32_t i = 0; i < 4u; ++i) pj[i] = pos(min(j + i, en[dx] - 1u));
+                uint32_t m = 0;
 #pragma unroll
                 for (uint32_t i = 0; i < 4u; ++i) {
                     const F3 d = pj[i] - self;
-                    if (j + i < en[dx] && !(glm_dot(d, d) > cut2)) {
-                        emit(j + i, first);
-                        first = false;
-                    }
+                    m |= !(glm_dot(d, d) > cut2) ? (1u << i) : 0u;
+                }
+                m &= (en[dx] - j >= 4u) ? 0xFu : ((1u << (en[dx] - j)) - 1u);
+                while (m) {
+                    const uint32_t i = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1u;
+                    emit(j + i, first);
+                    first = false;
                 }
             }
         }
@@ -786,7 +830,7 @@ struct OwnCur {
 };
 __device__ __forceinline__ OwnCur own_cur(const Bufs& b, uint2 unit, uint32_t lguw, uint32_t lgnc) {
     OwnCur o;
-    const uint32_t run = unit.x >> lguw, skip = (unit.y >> 5) * kChunk;
+    const uint32_t run = unit.x >> lguw, skip = (unit.y >> 6) * kChunk;
     o.c0 = b.coffset[run << lgnc] + skip;
     o.n = min(kChunk, b.coffset[(run + 1u) << lgnc] - o.c0);
     return o;

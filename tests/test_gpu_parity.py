@@ -120,13 +120,33 @@ def test_step_variants(variant):
     compare_state(f"variant {variant}", gpu_run(sc), oracle_run(sc))
 
 
-@pytest.mark.parametrize("tile_cap,unit_width", [(0, 16), (64, 16), (2048, 1), (2048, 2), (2048, 8), (4096, 16)])
+@pytest.mark.parametrize("tile_cap,unit_width", [(0, 16), (64, 16), (2048, 1), (2048, 2), (2048, 8), (4096, 16), (4096, 32)])
 def test_tiling_does_not_change_results(tile_cap, unit_width):
     """The on-chip tile, its global-memory fallback and every unit width give
     the oracle's results."""
     sc = small(20, 12, 6)
     compare_state(f"tile {tile_cap}/{unit_width}", gpu_run(sc, tile_cap=tile_cap, unit_width=unit_width),
                   oracle_run(sc))
+
+
+def test_wide_units_bit_exact():
+    """32-cell units (neighbourhoods of 9 x 34 = 306 cells, more than one per
+    thread of the workgroup) on a grid 64 cells long, with the lists and the
+    final passes: the oracle's results, and fewer units than at width 16."""
+    sc = small(96, 8, 6, box=(6.4, 0.8, 0.8))
+    assert sc.grid[0] == 64
+    wide = gpu_run(sc, tile_cap=4096, unit_width=32, with_ns=True)
+    narrow = gpu_run(sc, tile_cap=4096, unit_width=16)
+    ref = oracle_run(sc, want_ns=True)
+    nl = ref["num"]
+    for tag, g in (("4096/32", wide), ("4096/16", narrow)):
+        r = compare_state(tag, g, ref)
+        assert r["bitdiff_x"] == 0 and r["bitdiff_v"] == 0, tag
+    assert np.array_equal(wide["neighbor_count"], ref["neighbor_count"][:nl])
+    assert np.array_equal(wide["neighbors"], ref["neighbors"][:nl])
+    sw, sn = wide["stats"], narrow["stats"]
+    assert 0 < sw["units"] < sn["units"], (sw["units"], sn["units"])
+    assert sw["slow_units"] == 0 and sw["tile_overflow"] == 0
 
 
 def test_spheres_and_planes():

@@ -432,6 +432,8 @@ template <bool kStaged>
 __device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, uint32_t kc, F3 self, float h2,
                                               uint32_t ncap, const float4* __restrict__ src,
                                               uint32_t* __restrict__ nl, uint32_t cap, uint32_t g, bool& trunc) {
+    uint32_t* __restrict__ dst = nl + list_at(cap, g, 0u);  // entry cnt at dst[off]
+    size_t off = 0;
     uint32_t cnt = 0;
     for (uint32_t c = 0; c < 27u; ++c) {
         const uint32_t e = (c / 3u) * (W + 2u) + kc - 1u + (c % 3u);
@@ -457,8 +459,9 @@ __device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, u
                     trunc = true;
                     return cnt;
                 }
-                nl[list_at(cap, g, cnt)] = gb + t + i;
+                dst[off] = gb + t + i;
                 ++cnt;
+                off += ((cnt & 3u) == 0u) ? (size_t)4u * cap - 3u : 1u;
             }
         }
     }
@@ -555,8 +558,9 @@ __global__ __launch_bounds__(256) void k_lambda(Bufs b, Scenes<kB> SC) {
 // calcPositionUpdateCUDA (memo form, pbf_position_update.cu:131-188),
 // responseCollisionCUDA (pbf_response_collision.cu:12-66) and
 // updateInterimPositionCUDA (pbf_constraint.cu:335-345).  Reads (x*, s),
-// writes the new x*.  One thread per slot.
-template <bool kB>
+// writes the new x*.  One thread per slot.  kPow4: every scene's s_corr
+// exponent is 4, so powi unrolls to its two squarings at compile time.
+template <bool kB, bool kPow4>
 __global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const bool active = i < b.offset[b.CT];
@@ -597,7 +601,7 @@ __global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
                         const float cl = self_cl + sup;
                         corr = cl * tc;
                     } else {
-                        corr = P.sk * powi(W * P.inv_wdq, P.sn);
+                        corr = P.sk * powi(W * P.inv_wdq, kPow4 ? 4 : P.sn);
                     }
                     const float a = -(si + sj + corr);
                     acc(sum, mulsv(a, kg));
@@ -866,9 +870,12 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
     auto build = [&](uint32_t q, uint32_t g, bool tiled, const Grid& G, float cut2) {
         const F3 self = f3of(R[2 * g]);
         uint32_t n = 0;
+        uint32_t* __restrict__ dst = fl + list_at(cap, q, 0u);  // entry n at dst[off]
+        size_t off = 0;
         auto emit = [&](uint32_t j, bool first) {
-            if (n < kFinalK) fl[list_at(cap, q, n)] = j | (first ? 0x80000000u : 0u);
+            if (n < kFinalK) dst[off] = j | (first ? 0x80000000u : 0u);
             ++n;
+            off += ((n & 3u) == 0u) ? (size_t)4u * cap - 3u : 1u;
         };
         if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, emit);
         else list_global(G, b.offset, self, cut2, [&](uint32_t j) { return f3of(R[2 * j]); }, emit);
@@ -1296,6 +1303,7 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
         Mark m(tm, s, K_NS);
         hipLaunchKernelGGL(k_ns<kB>, grid(K_NS), dim3(kBlock), smem, s, b, SC, T.tile_cap);
     }
+    const bool pow4 = !kB && SC.s0.P.sn == 4;  // a batch's scenes may differ: general powi
     for (int it = 0; it < iters; ++it) {
         Bufs bi = b;
         if (qtape) bi.Q = qtape + (size_t)it * b.cap;
@@ -1304,7 +1312,10 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
             hipLaunchKernelGGL(k_lambda<kB>, dim3(nb), dim3(256), 0, s, bi, SC);
         }
         Mark m(tm, s, K_DELTAP);
-        hipLaunchKernelGGL(k_deltap<kB>, dim3(nb), dim3(256), 0, s, bi, SC);
+        if (pow4)
+            hipLaunchKernelGGL((k_deltap<kB, true>), dim3(nb), dim3(256), 0, s, bi, SC);
+        else
+            hipLaunchKernelGGL((k_deltap<kB, false>), dim3(nb), dim3(256), 0, s, bi, SC);
     }
     if (!any_vort && !any_xsph) {
         Mark m(tm, s, K_OUT);

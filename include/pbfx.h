@@ -258,13 +258,13 @@ pbfx_status pbfx_get_stats(pbfx_ctx* ctx, void* stream, pbfx_stats* out);
 
 /* Tuning knobs.  tile_cap: particles held on chip per work unit (0 forces
  * the global-memory path everywhere, for testing; default 2048, max 4096);
- * unit_width: max cells per work unit along x (power of two 1..16, default 16). */
+ * unit_width: max cells per work unit along x (power of two 1..32, default 16). */
 pbfx_status pbfx_set_tuning(pbfx_ctx* ctx, uint32_t tile_cap, uint32_t unit_width);
 
 /* Autotuner: times one step of the given state for every candidate
- * (tile_cap in {1024, 2048, 3072, 4096} x unit_width in {4, 8, 16}, within
- * the grids' limits) on the scratch copies x_scratch / v_scratch (x and v are
- * only read), applies the fastest and returns it.  A single-scene context
+ * (tile_cap in {1024, 2048, 3072, 4096} x unit_width in {4, 8, 16, 32}, within
+ * the grids' limits), best of four runs in two sweeps, on the scratch copies x_scratch /
+ * v_scratch (x and v are only read), applies the fastest and returns it.  A single-scene context
  * steps n particles with area [lo, hi]; a batch steps the scenes described
  * by the device offsets in_offsets (lo, hi ignored).  No counterpart in the
  * reference. */

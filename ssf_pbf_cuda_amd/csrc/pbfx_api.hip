@@ -894,33 +894,44 @@ pbfx_status pbfx_autotune(pbfx_ctx* c, const float* x, const float* v, uint32_t 
     float best = 3.4e38f;
     const uint32_t tiles[] = {1024, 2048, 3072, 4096};
     const uint32_t widths[] = {4, 8, 16, 32};
+    constexpr int kNT = 4, kNW = 4;
+    float tbest[kNT][kNW];
+    for (auto& r : tbest)
+        for (float& t : r) t = 3.4e38f;
     pbfx_status st = PBFX_OK;
-    for (uint32_t tc : tiles)
-        for (uint32_t uw : widths) {
-            if (e != hipSuccess || st != PBFX_OK) break;
-            if (uw > c->uw_max && uw != widths[0]) continue;
-            c->T.tile_cap = tc;
-            c->T.unit_width = uw;
-            set_grid_units(c);
-            float t = 3.4e38f;
-            for (int rep = 0; rep < 2 && e == hipSuccess && st == PBFX_OK; ++rep) {
-                e = hipMemcpyAsync(xs, x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
-                if (e == hipSuccess) e = hipMemcpyAsync(vs, v, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
-                if (e == hipSuccess) e = hipEventRecord(e0, s);
-                if (e == hipSuccess)
-                    st = do_step(c, xs, vs, n, c->batch ? in_offsets : c->d_in_off, iters, false, s);
-                if (e == hipSuccess && st == PBFX_OK) e = hipEventRecord(e1, s);
-                if (e == hipSuccess && st == PBFX_OK) e = hipEventSynchronize(e1);
-                float ms = 0.f;
-                if (e == hipSuccess && st == PBFX_OK) e = hipEventElapsedTime(&ms, e0, e1);
-                if (ms < t) t = ms;
+    // two sweeps over the candidates, best time of each kept: clock ramps and
+    // the order of the candidates then bias no choice
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        for (int ti = 0; ti < kNT; ++ti)
+            for (int wi = 0; wi < kNW; ++wi) {
+                const uint32_t tc = tiles[ti], uw = widths[wi];
+                if (e != hipSuccess || st != PBFX_OK) break;
+                if (uw > c->uw_max && uw != widths[0]) continue;
+                c->T.tile_cap = tc;
+                c->T.unit_width = uw;
+                set_grid_units(c);
+                float& t = tbest[ti][wi];
+                for (int rep = 0; rep < 2 && e == hipSuccess && st == PBFX_OK; ++rep) {
+                    e = hipMemcpyAsync(xs, x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+                    if (e == hipSuccess) e = hipMemcpyAsync(vs, v, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s);
+                    if (e == hipSuccess) e = hipEventRecord(e0, s);
+                    if (e == hipSuccess)
+                        st = do_step(c, xs, vs, n, c->batch ? in_offsets : c->d_in_off, iters, false, s);
+                    if (e == hipSuccess && st == PBFX_OK) e = hipEventRecord(e1, s);
+                    if (e == hipSuccess && st == PBFX_OK) e = hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    if (e == hipSuccess && st == PBFX_OK) e = hipEventElapsedTime(&ms, e0, e1);
+                    if (ms < t) t = ms;
+                }
             }
-            if (t < best) {
-                best = t;
-                best_tile = tc;
-                best_uw = uw;
+    }
+    for (int ti = 0; ti < kNT; ++ti)
+        for (int wi = 0; wi < kNW; ++wi)
+            if (tbest[ti][wi] < best) {
+                best = tbest[ti][wi];
+                best_tile = tiles[ti];
+                best_uw = widths[wi];
             }
-        }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     c->timer = keep;

@@ -193,22 +193,42 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ke
 // density) exceeds `budget` particles is split into narrower units, and a
 // unit with more than kChunk own particles into several items that share its
 // neighbourhood (item = (first cell, W | chunk << 6)).
+// Called by all 64 lanes of the wave (k = kNoKey for lanes with nothing to
+// emit): the lanes' items are counted, scanned across the wave and placed
+// with one atomic per wave on the item counter, which every run of every
+// wave would otherwise hit in turn.
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 __device__ __forceinline__ void emit_units(const uint32_t* __restrict__ offset, uint32_t k, uint32_t f, uint32_t uw,
-                                           uint32_t budget, uint32_t min_own, uint2* __restrict__ units,
+                                           uint32_t budget, uint2* __restrict__ units,
                                            uint32_t* __restrict__ nunits) {
-    const uint32_t kb = k & ~(uw - 1u);
-    if (f != offset[kb]) return;
-    const uint32_t nseg = offset[kb + uw] - offset[kb];
-    if (nseg < min_own) return;  // left to the per-particle path
-    uint32_t W = uw;
-    while (W > 1u && 9ull * (W + 2u) * nseg > (unsigned long long)budget * uw) W >>= 1;
-    for (uint32_t q0 = 0; q0 < uw; q0 += W) {
-        const uint32_t kq = kb + q0;
-        const uint32_t own = offset[kq + W] - offset[kq];
-        if (own) {
-            const uint32_t nch = (own + kChunk - 1u) / kChunk;
-            const uint32_t u = atomicAdd(nunits, nch);
-            for (uint32_t c = 0; c < nch; ++c) units[u + c] = make_uint2(kq, W | (c << 6));
+    const uint32_t kb = (k != kNoKey ? k : 0u) & ~(uw - 1u);
+    const bool head = k != kNoKey && f == offset[kb];
+    uint32_t W = uw, total = 0;
+    if (head) {
+        const uint32_t nseg = offset[kb + uw] - offset[kb];
+        while (W > 1u && 9ull * (W + 2u) * nseg > (unsigned long long)budget * uw) W >>= 1;
+        for (uint32_t q0 = 0; q0 < uw; q0 += W) {
+            const uint32_t own = offset[kb + q0 + W] - offset[kb + q0];
+            total += (own + kChunk - 1u) / kChunk;
+        }
+    }
+    const uint32_t lane = lane_id();
+    uint32_t incl = total;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+        if (lane >= (uint32_t)d) incl += y;
+    }
+    const uint32_t wave_total = (uint32_t)__shfl((int)incl, 63, 64);
+    uint32_t base = 0;
+    if (lane == 0 && wave_total) base = atomicAdd(nunits, wave_total);
+    base = (uint32_t)__shfl((int)base, 0, 64);
+    if (head && total) {
+        uint32_t u = base + incl - total;
+        for (uint32_t q0 = 0; q0 < uw; q0 += W) {
+            const uint32_t kq = kb + q0;
+            const uint32_t nch = (offset[kq + W] - offset[kq] + kChunk - 1u) / kChunk;
+            for (uint32_t c = 0; c < nch; ++c) units[u++] = make_uint2(kq, W | (c << 6));
         }
     }
 }
@@ -228,6 +248,7 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
     const bool active = s < n;
     const uint32_t i = active ? tmp[s] : 0u;
     const uint32_t si = (kB && active) ? b.isid[i] : 0u;
+    uint32_t ek = kNoKey, ef = 0;
     by_scene<kB>(si, active, [&](uint32_t sc) {
         const uint32_t k = key[i];
         const uint32_t o0 = b.offset[k], o1 = b.offset[k + 1];
@@ -247,8 +268,12 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
         }
         b.Pf[f] = make_float4(q[0], q[1], q[2], 0.f);
         st3(b.X, b.cap, f, mk3(xo[0], xo[1], xo[2]));
-        if (k < b.CT) emit_units(b.offset, k, f, uw, budget, 0u, b.units, &b.ctr[CTR_UNITS]);
+        if (k < b.CT) {
+            ek = k;
+            ef = f;
+        }
     });
+    emit_units(b.offset, ek, ef, uw, budget, b.units, &b.ctr[CTR_UNITS]);  // all lanes
 }
 
 
@@ -626,10 +651,11 @@ __global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
 // (pbf/interaction/cuda/detail/pbf_contribution.inl:7-57).  Particles are
 // regrouped by run of uw final cells along x (the work unit of the final
 // passes; the order inside a run is irrelevant, every sum is per particle):
-// a counting sort over C/uw runs.  Each run has nc counters, one per
-// group of workgroups (b mod nc, nc <= 8, i.e. per XCD), so that a run that
-// collects particles from all over the domain -- a wall or corner pile --
-// is not counted through one contended word; key = run * nc + copy.  The same
+// a counting sort over C/uw runs.  Each run has nc = min(8, uw) counters,
+// one per group of workgroups (b mod nc, i.e. per XCD), each on its own cache
+// line (count_slot), so that a run that collects particles from all over the
+// domain -- a wall or corner pile of ~20k particles on the 1M dam -- is not
+// counted through one contended line; key = run * nc + copy.  The same
 // pass packs each slot's record R = (x*, 0), (v, 0) with
 // v = inv_dt*(x* - x_sorted) (updateVelocityCUDA, pbf_update.cu:25-39), so
 // that a particle visited in run order costs one cache line.
@@ -697,7 +723,8 @@ __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict_
 // the same sums: the skipped cells add +0.
 
 // Emits the in-range tile elements of the staged neighbourhood around own
-// column kc, in order: emit(slot, first pair of its cell).
+// column kc, in order: emit(slot, first pair of its cell), until emit returns
+// false.
 template <class EmitF>
 __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S, const float4* __restrict__ tp,
                                           uint32_t kc, F3 self, float cut2, EmitF&& emit) {
@@ -720,7 +747,7 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
                 const uint32_t i = (uint32_t)__builtin_ctz(m);
                 m &= m - 1u;
                 const float w = i == 0u ? el[0].w : (i == 1u ? el[1].w : (i == 2u ? el[2].w : el[3].w));
-                emit(__float_as_uint(w), first);
+                if (!emit(__float_as_uint(w), first)) return;
                 first = false;
             }
         }
@@ -729,7 +756,7 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
 
 // The same from global memory, row by row (dz outer, dy inner), the next
 // row's three cell ranges loaded while the current row is filtered; pos(j)
-// gives x*_j.
+// gives x*_j.  Stops when emit returns false.
 template <class PosF, class EmitF>
 __device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self, float cut2,
                                             PosF&& pos, EmitF&& emit) {
@@ -765,7 +792,7 @@ __device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __res
                 while (m) {
                     const uint32_t i = (uint32_t)__builtin_ctz(m);
                     m &= m - 1u;
-                    emit(j + i, first);
+                    if (!emit(j + i, first)) return;
                     first = false;
                 }
             }
@@ -856,7 +883,9 @@ __device__ __forceinline__ float4 pos_slot(const float4* __restrict__ R, uint32_
 // ------------------------------------------------- K9: final pair lists
 // For each particle (by position q of the final-cell order) the slots of its
 // in-range pairs in groups of four (list_at(cap, q, k), k < kFinalK); fcnt[q] =
-// the full count (above kFinalK: the passes fall back to the direct sum).
+// their count, or kFinalK + 1 when there are more (the passes then fall back
+// to the direct sum, so the filter stops at the (kFinalK+1)-th pair: a
+// particle in a pile of thousands does not scan its whole neighbourhood).
 template <bool kB>
 __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, uint32_t tile_cap, uint32_t lguw, uint32_t lgnc) {
     __shared__ UnitShared S;
@@ -873,9 +902,14 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         uint32_t* __restrict__ dst = fl + list_at(cap, q, 0u);  // entry n at dst[off]
         size_t off = 0;
         auto emit = [&](uint32_t j, bool first) {
-            if (n < kFinalK) dst[off] = j | (first ? 0x80000000u : 0u);
+            if (n == kFinalK) {
+                ++n;
+                return false;
+            }
+            dst[off] = j | (first ? 0x80000000u : 0u);
             ++n;
             off += ((n & 3u) == 0u) ? (size_t)4u * cap - 3u : 1u;
+            return true;
         };
         if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, emit);
         else list_global(G, b.offset, self, cut2, [&](uint32_t j) { return f3of(R[2 * j]); }, emit);

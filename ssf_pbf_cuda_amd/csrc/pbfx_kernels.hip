@@ -457,8 +457,8 @@ template <bool kStaged>
 __device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, uint32_t kc, F3 self, float h2,
                                               uint32_t ncap, const float4* __restrict__ src,
                                               uint32_t* __restrict__ nl, uint32_t cap, uint32_t g, bool& trunc) {
-    uint32_t* __restrict__ dst = nl + list_at(cap, g, 0u);  // entry cnt at dst[off]
-    size_t off = 0;
+    uint32_t* __restrict__ dst = nl + list_at(cap, g, 0u);  // where entry cnt goes
+    const size_t jump = (size_t)4u * cap - 3u;              // from entry 4q+3 to 4q+4
     uint32_t cnt = 0;
     for (uint32_t c = 0; c < 27u; ++c) {
         const uint32_t e = (c / 3u) * (W + 2u) + kc - 1u + (c % 3u);
@@ -468,7 +468,8 @@ __device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, u
         for (uint32_t t = lo; t < hi; t += 4u) {
             F3 pj[4];
 #pragma unroll
-            for (uint32_t i = 0; i < 4u; ++i) pj[i] = f3of(src[sb + min(t + i, hi - 1u)]);
+            for (uint32_t i = 0; i < 4u; ++i)  // the tile is padded: no clamp
+                pj[i] = f3of(kStaged ? src[t + i] : src[sb + min(t + i, hi - 1u)]);
             uint32_t m = 0;
 #pragma unroll
             for (uint32_t i = 0; i < 4u; ++i) {
@@ -477,16 +478,24 @@ __device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, u
                 m |= (r2 < h2) ? (1u << i) : 0u;
             }
             m &= (hi - t >= 4u) ? 0xFu : ((1u << (hi - t)) - 1u);
+            // keep the first ncap - cnt hits of the group; any beyond them
+            // truncate the list
+            const uint32_t room = ncap - cnt;
+            bool full = false;
+            while ((uint32_t)__builtin_popcount(m) > room) {
+                m &= ~(0x80000000u >> __builtin_clz(m));
+                full = true;
+            }
             while (m) {
                 const uint32_t i = (uint32_t)__builtin_ctz(m);
                 m &= m - 1u;
-                if (cnt == ncap) {
-                    trunc = true;
-                    return cnt;
-                }
-                dst[off] = gb + t + i;
+                *dst = gb + t + i;
                 ++cnt;
-                off += ((cnt & 3u) == 0u) ? (size_t)4u * cap - 3u : 1u;
+                dst += ((cnt & 3u) == 0u) ? jump : 1u;
+            }
+            if (full) {
+                trunc = true;
+                return cnt;
             }
         }
     }
@@ -735,7 +744,7 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
         for (uint32_t t = lo; t < hi; t += 4u) {
             float4 el[4];
 #pragma unroll
-            for (uint32_t i = 0; i < 4u; ++i) el[i] = tp[min(t + i, hi - 1u)];
+            for (uint32_t i = 0; i < 4u; ++i) el[i] = tp[t + i];  // the tile is padded: no clamp
             uint32_t m = 0;
 #pragma unroll
             for (uint32_t i = 0; i < 4u; ++i) {
@@ -899,16 +908,16 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
     auto build = [&](uint32_t q, uint32_t g, bool tiled, const Grid& G, float cut2) {
         const F3 self = f3of(R[2 * g]);
         uint32_t n = 0;
-        uint32_t* __restrict__ dst = fl + list_at(cap, q, 0u);  // entry n at dst[off]
-        size_t off = 0;
+        uint32_t* __restrict__ dst = fl + list_at(cap, q, 0u);  // where entry n goes
+        const size_t jump = (size_t)4u * cap - 3u;              // from entry 4q+3 to 4q+4
         auto emit = [&](uint32_t j, bool first) {
             if (n == kFinalK) {
                 ++n;
                 return false;
             }
-            dst[off] = j | (first ? 0x80000000u : 0u);
+            *dst = j | (first ? 0x80000000u : 0u);
             ++n;
-            off += ((n & 3u) == 0u) ? (size_t)4u * cap - 3u : 1u;
+            dst += ((n & 3u) == 0u) ? jump : 1u;
             return true;
         };
         if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, emit);
@@ -1245,7 +1254,11 @@ __global__ __launch_bounds__(256) void k_selftest_inv_sqrt(uint32_t lo, uint32_t
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // Dynamic LDS of each unit kernel: the tile.
-size_t smem_of(uint32_t tile_cap) { return (size_t)tile_cap * sizeof(float4); }
+// Dynamic LDS of each unit kernel: the tile, plus kTilePad entries so that
+// the filters read candidates four at a time past a cell's last one without
+// clamping (those reads are masked out).
+constexpr uint32_t kTilePad = 4;
+size_t smem_of(uint32_t tile_cap) { return (size_t)(tile_cap + kTilePad) * sizeof(float4); }
 
 }  // namespace
 

@@ -731,12 +731,24 @@ __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict_
 // lists with every lane busy.  Adding only the cells that have pairs gives
 // the same sums: the skipped cells add +0.
 
+// Keeps the first `room` set bits of a group's hit mask; returns whether any
+// were dropped.
+__device__ __forceinline__ bool clip_hits(uint32_t& m, uint32_t room) {
+    bool full = false;
+    while ((uint32_t)__builtin_popcount(m) > room) {
+        m &= ~(0x80000000u >> __builtin_clz(m));
+        full = true;
+    }
+    return full;
+}
+
 // Emits the in-range tile elements of the staged neighbourhood around own
-// column kc, in order: emit(slot, first pair of its cell), until emit returns
-// false.
+// column kc, in order: emit(slot, first pair of its cell), which counts them
+// in n; stops once n reaches lim.
 template <class EmitF>
 __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S, const float4* __restrict__ tp,
-                                          uint32_t kc, F3 self, float cut2, EmitF&& emit) {
+                                          uint32_t kc, F3 self, float cut2, const uint32_t& n, uint32_t lim,
+                                          EmitF&& emit) {
     for (uint32_t q = 0; q < 27u; ++q) {
         const uint32_t e = (q / 3u) * (I.W + 2u) + kc - 1u + (q % 3u);
         const uint32_t lo = S.tstart[e], hi = S.tstart[e + 1];
@@ -752,23 +764,25 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
                 m |= !(glm_dot(d, d) > cut2) ? (1u << i) : 0u;
             }
             m &= (hi - t >= 4u) ? 0xFu : ((1u << (hi - t)) - 1u);
+            const bool full = clip_hits(m, lim - n);
             while (m) {
                 const uint32_t i = (uint32_t)__builtin_ctz(m);
                 m &= m - 1u;
                 const float w = i == 0u ? el[0].w : (i == 1u ? el[1].w : (i == 2u ? el[2].w : el[3].w));
-                if (!emit(__float_as_uint(w), first)) return;
+                emit(__float_as_uint(w), first);
                 first = false;
             }
+            if (full || n == lim) return;
         }
     }
 }
 
 // The same from global memory, row by row (dz outer, dy inner), the next
 // row's three cell ranges loaded while the current row is filtered; pos(j)
-// gives x*_j.  Stops when emit returns false.
+// gives x*_j.  Stops once n reaches lim.
 template <class PosF, class EmitF>
 __device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self, float cut2,
-                                            PosF&& pos, EmitF&& emit) {
+                                            const uint32_t& n, uint32_t lim, PosF&& pos, EmitF&& emit) {
     int gp[3];
     grid_pos(self, G.cw, gp);
     auto row = [&](int r, uint32_t* st, uint32_t* en) {
@@ -798,12 +812,14 @@ __device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __res
                     m |= !(glm_dot(d, d) > cut2) ? (1u << i) : 0u;
                 }
                 m &= (en[dx] - j >= 4u) ? 0xFu : ((1u << (en[dx] - j)) - 1u);
+                const bool full = clip_hits(m, lim - n);
                 while (m) {
                     const uint32_t i = (uint32_t)__builtin_ctz(m);
                     m &= m - 1u;
-                    if (!emit(j + i, first)) return;
+                    emit(j + i, first);
                     first = false;
                 }
+                if (full || n == lim) return;
             }
         }
 #pragma unroll
@@ -910,18 +926,16 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         uint32_t n = 0;
         uint32_t* __restrict__ dst = fl + list_at(cap, q, 0u);  // where entry n goes
         const size_t jump = (size_t)4u * cap - 3u;              // from entry 4q+3 to 4q+4
+        // the (kFinalK+1)-th pair is counted, not stored: it marks the overflow
         auto emit = [&](uint32_t j, bool first) {
-            if (n == kFinalK) {
-                ++n;
-                return false;
+            if (n < kFinalK) {
+                *dst = j | (first ? 0x80000000u : 0u);
+                dst += (((n + 1u) & 3u) == 0u) ? jump : 1u;
             }
-            *dst = j | (first ? 0x80000000u : 0u);
             ++n;
-            dst += ((n & 3u) == 0u) ? jump : 1u;
-            return true;
         };
-        if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, emit);
-        else list_global(G, b.offset, self, cut2, [&](uint32_t j) { return f3of(R[2 * j]); }, emit);
+        if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, n, kFinalK + 1u, emit);
+        else list_global(G, b.offset, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(R[2 * j]); }, emit);
         b.fcnt[q] = n;
         tl.listovf += n > kFinalK ? 1u : 0u;
     };

@@ -184,6 +184,33 @@ __device__ __forceinline__ void list_group(uint4 e, uint32_t k, uint32_t n, uint
     j[3] = (k + 3u < n) ? e.w : e.x;
 }
 
+// Visits entries 0..n-1 of slot i's list in order, eight at a time: the
+// two list words of a group and its eight gathers (load(j) -> float4) are in
+// flight together, and the next group's words while this one is summed.
+// Entries at or past n are replaced by the group's first one (a written
+// index) and not passed to body(k, element).
+template <class LoadF, class BodyF>
+__device__ __forceinline__ void for_list8(const uint32_t* __restrict__ nl, uint32_t cap, uint32_t i, uint32_t n,
+                                          LoadF&& load, BodyF&& body) {
+    if (n == 0u) return;
+    const uint32_t last = (n - 1u) & ~3u;  // the last word holding a written entry
+    uint4 e0 = list_word(nl, cap, i, 0u), e1 = list_word(nl, cap, i, min(4u, last));
+    for (uint32_t k = 0; k < n; k += 8u) {
+        uint32_t j[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+#pragma unroll
+        for (uint32_t m = 1; m < 8u; ++m)
+            if (k + m >= n) j[m] = j[0];
+        float4 el[8];
+#pragma unroll
+        for (uint32_t m = 0; m < 8u; ++m) el[m] = load(j[m]);
+        e0 = list_word(nl, cap, i, min(k + 8u, last));
+        e1 = list_word(nl, cap, i, min(k + 12u, last));
+#pragma unroll
+        for (uint32_t m = 0; m < 8u; ++m)
+            if (k + m < n) body(k + m, el[m]);
+    }
+}
+
 // 1.f / sqrtf(r2) with both operations correctly rounded -- the IEEE
 // result -- in about a third of the instructions of the IEEE sequences.  On
 // [2^-96, FLT_MAX]: v_sqrt_f32, then the two-sided fma test of its

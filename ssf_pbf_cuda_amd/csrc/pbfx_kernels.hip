@@ -561,26 +561,15 @@ __global__ __launch_bounds__(256) void k_lambda(Bufs b, Scenes<kB> SC) {
         const uint32_t n = b.ncnt[i];
         float ks = 0.f, kg2s = 0.f;
         F3 kgs = mk3(0.f, 0.f, 0.f);
-        uint4 e = n ? list_word(nl, cap, i, 0u) : make_uint4(0u, 0u, 0u, 0u);
-        for (uint32_t k = 0; k < n; k += 4u) {
-            uint32_t j[4];
-            list_group(e, k, n, j);
-            F3 pj[4];
-#pragma unroll
-            for (uint32_t m = 0; m < 4u; ++m) pj[m] = f3of(Pf[j[m]]);
-            e = list_word(nl, cap, i, min(k + 4u, (n - 1u) & ~3u));  // next group, in flight during this one
-#pragma unroll
-            for (uint32_t m = 0; m < 4u; ++m)
-                if (k + m < n) {
-                    float W;
-                    F3 kg;
-                    pair_terms(self, pj[m], P, W, kg);
-                    const float kg2 = fdot(kg, kg);
-                    ks += W;
-                    acc(kgs, kg);
-                    kg2s += kg2;
-                }
-        }
+        for_list8(nl, cap, i, n, [&](uint32_t j) { return Pf[j]; }, [&](uint32_t, float4 pj) {
+            float W;
+            F3 kg;
+            pair_terms(self, f3of(pj), P, W, kg);
+            const float kg2 = fdot(kg, kg);
+            ks += W;
+            acc(kgs, kg);
+            kg2s += kg2;
+        });
         const float Cc = __builtin_fmaf(ks * P.m, P.inv_rho0, -1.f);
         const float Gs =
             __builtin_fmaf(kgs.x, kgs.x, __builtin_fmaf(kgs.z, kgs.z, __builtin_fmaf(kgs.y, kgs.y, kg2s)));
@@ -611,36 +600,25 @@ __global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
         const float self_cl = (si < 0.f) ? -si : 0.f;
         const uint32_t n = b.ncnt[i];
         F3 sum = mk3(0.f, 0.f, 0.f);
-        uint4 e = n ? list_word(nl, cap, i, 0u) : make_uint4(0u, 0u, 0u, 0u);
-        for (uint32_t k = 0; k < n; k += 4u) {
-            uint32_t j[4];
-            list_group(e, k, n, j);
-            float4 pe[4];
-#pragma unroll
-            for (uint32_t m = 0; m < 4u; ++m) pe[m] = Q[j[m]];
-            e = list_word(nl, cap, i, min(k + 4u, (n - 1u) & ~3u));  // next group, in flight during this one
-#pragma unroll
-            for (uint32_t m = 0; m < 4u; ++m)
-                if (k + m < n) {
-                    float W;
-                    F3 kg;
-                    pair_terms(self, f3of(pe[m]), P, W, kg);
-                    const float sj = pe[m].w;
-                    float corr;
-                    if (P.mode == 0) {
-                        const float kr2 = W * W;
-                        const float kr4 = kr2 * kr2;
-                        const float tc = 10.f * kr4 * P.inv_k;
-                        const float sup = (sj < 0.f) ? -sj : 0.f;
-                        const float cl = self_cl + sup;
-                        corr = cl * tc;
-                    } else {
-                        corr = P.sk * powi(W * P.inv_wdq, kPow4 ? 4 : P.sn);
-                    }
-                    const float a = -(si + sj + corr);
-                    acc(sum, mulsv(a, kg));
-                }
-        }
+        for_list8(nl, cap, i, n, [&](uint32_t j) { return Q[j]; }, [&](uint32_t, float4 pe) {
+            float W;
+            F3 kg;
+            pair_terms(self, f3of(pe), P, W, kg);
+            const float sj = pe.w;
+            float corr;
+            if (P.mode == 0) {
+                const float kr2 = W * W;
+                const float kr4 = kr2 * kr2;
+                const float tc = 10.f * kr4 * P.inv_k;
+                const float sup = (sj < 0.f) ? -sj : 0.f;
+                const float cl = self_cl + sup;
+                corr = cl * tc;
+            } else {
+                corr = P.sk * powi(W * P.inv_wdq, kPow4 ? 4 : P.sn);
+            }
+            const float a = -(si + sj + corr);
+            acc(sum, mulsv(a, kg));
+        });
         F3 dp = mk3(sum.x * P.m * P.inv_rho0, sum.y * P.m * P.inv_rho0, sum.z * P.m * P.inv_rho0);
         bool capped = false;
         const int sw = (B.n_planes == 6 && B.n_inner == 0 && B.n_outer == 0)

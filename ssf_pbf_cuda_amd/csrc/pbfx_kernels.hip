@@ -894,13 +894,15 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
     float4* tp = smem;
-    const float4* __restrict__ R = b.R;
+    // the final x*, packed: the same values as the passes' records R (built
+    // from Pf), at half the bytes per candidate
+    const float4* __restrict__ Pf = b.Pf;
     const uint32_t cap = b.cap;
     uint32_t* __restrict__ fl = b.flist;
     Tally tl;
     UnitInfo I;
     auto build = [&](uint32_t q, uint32_t g, bool tiled, const Grid& G, float cut2) {
-        const F3 self = f3of(R[2 * g]);
+        const F3 self = f3of(Pf[g]);
         uint32_t n = 0;
         uint32_t* __restrict__ dst = fl + list_at(cap, q, 0u);  // where entry n goes
         const size_t jump = (size_t)4u * cap - 3u;              // from entry 4q+3 to 4q+4
@@ -913,7 +915,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
             ++n;
         };
         if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, n, kFinalK + 1u, emit);
-        else list_global(G, b.offset, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(R[2 * j]); }, emit);
+        else list_global(G, b.offset, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(Pf[j]); }, emit);
         b.fcnt[q] = n;
         tl.listovf += n > kFinalK ? 1u : 0u;
     };
@@ -928,7 +930,11 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         const float cut2 = cut2_of(sc);
         I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         if (I.ok) {
-            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t j) { tp[t] = pos_slot(R, j); });
+            if (I.staged)
+                load_runs(S, I, [&](uint32_t t, uint32_t j) {
+                    const float4 p = Pf[j];
+                    tp[t] = make_float4(p.x, p.y, p.z, __uint_as_float(j));
+                });
             __syncthreads();
             const OwnCur o = own_cur(b, unit, lguw, lgnc);
             for (uint32_t i = threadIdx.x; i < o.n; i += kBlock) {

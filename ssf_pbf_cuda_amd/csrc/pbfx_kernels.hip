@@ -808,6 +808,13 @@ __device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __res
     }
 }
 
+// A candidate's records for a pass's term: p = (x*_j, pass-specific w),
+// q = the second record the pass needs (the velocity, or unused).  Loaded
+// together by the sums, so no load waits on another.
+struct El2 {
+    float4 p, q;
+};
+
 // Direct sum over the 27 cells from global memory, for particles whose list
 // overflowed: elem(j) builds the pass's element of slot j.
 template <class ElemF, class TermF>
@@ -827,11 +834,16 @@ __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __rest
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             F3 cs = mk3(0.f, 0.f, 0.f);
-            for (uint32_t j = st[q]; j < en[q]; ++j) {
-                const float4 el = elem(j);
-                const F3 d = f3of(el) - self;
-                if (glm_dot(d, d) > cut2) continue;
-                acc(cs, term(el));
+            // four candidates' records loaded together, then added in order
+            for (uint32_t j = st[q]; j < en[q]; j += 4u) {
+                El2 el[4];
+#pragma unroll
+                for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(min(j + i, en[q] - 1u));
+#pragma unroll
+                for (uint32_t i = 0; i < 4u; ++i) {
+                    const F3 d = f3of(el[i].p) - self;
+                    if (j + i < en[q] && !(glm_dot(d, d) > cut2)) acc(cs, term(el[i]));
+                }
             }
             acc(sum, cs);
         }
@@ -878,10 +890,6 @@ __device__ __forceinline__ uint32_t final_column(const Grid& G, const UnitInfo& 
 }
 
 // Tile element of the list build: x*_j and slot j (as float bits).
-__device__ __forceinline__ float4 pos_slot(const float4* __restrict__ R, uint32_t j) {
-    const float4 r = R[2 * j];
-    return make_float4(r.x, r.y, r.z, __uint_as_float(j));
-}
 
 // ------------------------------------------------- K9: final pair lists
 // For each particle (by position q of the final-cell order) the slots of its
@@ -964,7 +972,7 @@ __device__ __forceinline__ F3 sum_list(const uint32_t* __restrict__ fl, uint32_t
     uint4 w = n ? list_word(fl, cap, q, 0u) : make_uint4(0u, 0u, 0u, 0u);
     for (uint32_t k = 0; k < n; k += 4u) {
         uint32_t e[4];
-        float4 el[4];
+        El2 el[4];
         list_group(w, k, n, e);
 #pragma unroll
         for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(e[i] & 0x7FFFFFFFu);
@@ -1037,10 +1045,8 @@ __global__ __launch_bounds__(kBlock) void k_vort(Bufs b, Scenes<kB> SC) {
         const float mr = P.m * P.inv_rho0;
         const F3 self = f3of(R[2 * g]);
         const F3 selfv = f3of(R[2 * g + 1]);
-        auto elem = [&](uint32_t j) { return pos_slot(R, j); };
-        auto term = [&](float4 el) {
-            return vort_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.warg);
-        };
+        auto elem = [&](uint32_t j) { return El2{R[2 * j], R[2 * j + 1]}; };
+        auto term = [&](const El2& e) { return vort_pair(self, selfv, f3of(e.p), f3of(e.q), P.warg); };
         const F3 w = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
                                   : sum27_global(G, b.offset, self, P.cut2_wd, elem, term);
         const F3 om = mulsv(mr, w);
@@ -1062,9 +1068,9 @@ __global__ __launch_bounds__(kBlock) void k_confine(Bufs b, Scenes<kB> SC) {
         const F3 self = f3of(R[2 * g]);
         auto elem = [&](uint32_t j) {
             const float4 r = R[2 * j];
-            return make_float4(r.x, r.y, r.z, OM4[j].w);
+            return El2{make_float4(r.x, r.y, r.z, OM4[j].w), make_float4(0.f, 0.f, 0.f, 0.f)};
         };
-        auto term = [&](float4 el) { return loc_pair(self, f3of(el), el.w, P.warg); };
+        auto term = [&](const El2& e) { return loc_pair(self, f3of(e.p), e.p.w, P.warg); };
         F3 N = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
                             : sum27_global(G, b.offset, self, P.cut2_wd, elem, term);
         if (glm_length(N) > 0.f) {
@@ -1099,10 +1105,8 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Bufs b, Scenes<kB> SC, floa
             return;
         }
         const Grid G = SC[sc].G;
-        auto elem = [&](uint32_t j) { return pos_slot(R, j); };
-        auto term = [&](float4 el) {
-            return xsph_pair(self, selfv, f3of(el), f3of(R[2 * __float_as_uint(el.w) + 1]), P.inv_h);
-        };
+        auto elem = [&](uint32_t j) { return El2{R[2 * j], R[2 * j + 1]}; };
+        auto term = [&](const El2& e) { return xsph_pair(self, selfv, f3of(e.p), f3of(e.q), P.inv_h); };
         const F3 a = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
                                   : sum27_global(G, b.offset, self, P.cut2_w, elem, term);
         F3 out;

@@ -343,17 +343,30 @@ __global__ __launch_bounds__(256) void k_adj_ckey(StepTape t, AdjBufs a, Grid G,
 __global__ __launch_bounds__(256) void k_adj_cscatter(StepTape t, AdjBufs a) {
     const uint32_t s = gtid();
     if (s >= t.n_live) return;
-    a.ctmp[a.coffset[a.ckey[s]] + a.crank[s]] = s;
+    const uint32_t k = a.ckey[s], r = a.crank[s], o0 = a.coffset[k];
+    a.ctmp[o0 + r] = s;
+    if (r == 0u) list_big_cell(a.coffset, k, o0, a.big, a.nbig);
 }
 
-// slot order inside each cell: position = number of smaller slots in the cell
+// slot order inside each cell: position = number of smaller slots in the
+// cell -- in a cell sorted by k_sort_cells, found by binary search
 __global__ __launch_bounds__(256) void k_adj_corder(StepTape t, AdjBufs a) {
     const uint32_t s = gtid();
     if (s >= t.n_live) return;
     const uint32_t k = a.ckey[s];
     const uint32_t o0 = a.coffset[k], o1 = a.coffset[k + 1];
     uint32_t r = 0;
-    for (uint32_t q = o0; q < o1; ++q) r += a.ctmp[q] < s ? 1u : 0u;
+    if (sorted_cell(o1 - o0)) {
+        uint32_t lo = o0, hi = o1;  // the first entry >= s (it is s)
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.ctmp[mid] < s) lo = mid + 1u;
+            else hi = mid;
+        }
+        r = lo - o0;
+    } else {
+        for (uint32_t q = o0; q < o1; ++q) r += a.ctmp[q] < s ? 1u : 0u;
+    }
     a.cperm[o0 + r] = s;
 }
 
@@ -983,7 +996,11 @@ hipError_t launch_adj_step(const Bufs& b, const Scene& s0, const StepTape& t, co
         hipLaunchKernelGGL(k_adj_ckey, gl, blk, 0, s, t, a, G, b.lgT);
         e = launch_count_scan(a.ccount, a.coffset, a.ctsum, a.ctflag, a.ctprefix, b.nT, b.lgT, b.CT, s);
         if (e != hipSuccess) return e;
+        e = hipMemsetAsync(a.nbig, 0, sizeof(uint32_t), s);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_adj_cscatter, gl, blk, 0, s, t, a);
+        e = launch_sort_cells(a.coffset, a.big, a.nbig, a.ctmp, s);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_adj_corder, gl, blk, 0, s, t, a);
         // heavy lists of the final passes and the search cells: their
         // lengths size the heavy launches

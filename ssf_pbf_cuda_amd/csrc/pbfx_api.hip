@@ -219,7 +219,8 @@ void free_adj(pbfx_ctx* c) {
     void* ap[] = {A->id[0], A->id[1], A->xw, A->vw, a.ax, a.av, a.axin, a.avin, a.aP, a.aQ, a.aV, a.aVc, a.aOm,
                   a.etab, a.abar, a.coef, a.g2, a.aX, a.xnan, a.ckey, a.crank, a.ctmp, a.cperm, a.ccount,
                   a.coffset, a.ctsum, a.ctflag, a.ctprefix, a.gP, a.gV, a.gD, a.hflag, a.hlist, a.nh, a.last,
-                  a.rcount, a.roff, a.rtsum, a.rtflag, a.rtprefix, a.rlist, a.pc, a.pe, a.psum, a.amax};
+                  a.rcount, a.roff, a.rtsum, a.rtflag, a.rtprefix, a.rlist, a.pc, a.pe, a.psum, a.amax,
+                  a.big, a.nbig};
     for (void* p : ap) ptrs.push_back(p);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -237,7 +238,7 @@ void free_bufs(pbfx_ctx* c) {
     void* ptrs[] = {b.key,   b.rank,    b.tmp,     b.sidx,  b.Pf,     b.Q,     b.X,       b.R,
                     b.OM4,   b.ncnt,    b.nlist,   b.count, b.offset, b.tsum,  b.tprefix, b.units,
                     b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
-                    b.tflag, b.ctflag,  b.flist,   b.fcnt,  b.isid,   b.sid,   b.ctr,
+                    b.tflag, b.ctflag,  b.flist,   b.fcnt,  b.isid,   b.sid,   b.ctr,     b.big,
                     c->d_scenes, c->d_kbase, c->d_in_off, c->d_scene_off, c->d_default_sphere};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -358,7 +359,7 @@ pbfx_status read_counters(pbfx_ctx* c, hipStream_t s) {
     // fold the statistics copies into h[0..CTR_NUM)
     uint32_t* h = c->h_pinned;
     for (int k = 0; k < CTR_NUM; ++k) {
-        if (k == CTR_UNITS || k == CTR_CUNITS) continue;
+        if (k == CTR_UNITS || k == CTR_CUNITS || k == CTR_BIGCELLS) continue;
         const bool is_max = (k == CTR_MAXTILE || k == CTR_MAXSWEEP);
         uint32_t v = 0;
         for (uint32_t r = 1; r <= kStatRep; ++r) {
@@ -457,6 +458,7 @@ pbfx_status create_common(pbfx_ctx* c) {
     PBFX_ALLOC(b.key, bc);
     PBFX_ALLOC(b.rank, bc);
     PBFX_ALLOC(b.tmp, bc);
+    PBFX_ALLOC(b.big, bc / kSmallCell + 2);
     PBFX_ALLOC(b.sidx, bc);
     PBFX_ALLOC(b.isid, bc);
     PBFX_ALLOC(b.sid, bc);
@@ -1107,6 +1109,8 @@ pbfx_status pbfx_adj_enable(pbfx_ctx* c, uint32_t max_steps, uint32_t every) {
     PBFX_ALLOC(a.ckey, cap);
     PBFX_ALLOC(a.crank, cap);
     PBFX_ALLOC(a.ctmp, cap);
+    PBFX_ALLOC(a.big, cap / kSmallCell + 2);
+    PBFX_ALLOC(a.nbig, 1);
     PBFX_ALLOC(a.cperm, cap);
     PBFX_ALLOC(a.ccount, CT + 1);
     PBFX_ALLOC(a.coffset, CT + 2);

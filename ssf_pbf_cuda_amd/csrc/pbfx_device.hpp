@@ -442,6 +442,15 @@ __device__ __forceinline__ void mark_tile(uint32_t* __restrict__ tflag, uint32_t
 
 __device__ __forceinline__ F3 f3of(float4 f) { return mk3(f.x, f.y, f.z); }
 
+// Whether k_sort_cells sorts a cell of n particles.
+__device__ __forceinline__ bool sorted_cell(uint32_t n) { return n > kSmallCell && n <= kSortMax; }
+
+// Lists cell k (its segment starting at o0) for k_sort_cells when sorted_cell.
+__device__ __forceinline__ void list_big_cell(const uint32_t* __restrict__ offset, uint32_t k, uint32_t o0,
+                                              uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
+    if (sorted_cell(offset[k + 1] - o0)) big[atomicAdd(nbig, 1u)] = k;
+}
+
 // Block -> index range, contiguous per XCD (workgroup b runs on XCD b % 8),
 // so that neighbouring slots share an L2.
 __device__ __forceinline__ uint32_t xcd_block() {

@@ -36,6 +36,11 @@ constexpr uint32_t kTileIndexLimit = 65535;  // tile indices are stored as u16
 constexpr uint32_t kScanTile = 4096;      // cells per scan tile
 constexpr uint32_t kChunk = kBlock;       // own particles per work item
 constexpr uint32_t kSparse = 64;          // final-cell runs below this go one thread per particle
+// Cells above kSmallCell particles get their index segment sorted (up to
+// kSortMax), so that a particle's in-cell rank is its position rather than a
+// count over the whole cell.
+constexpr uint32_t kSmallCell = 64;
+constexpr uint32_t kSortMax = 16384;
 constexpr uint32_t kFinalK = 64;          // stored in-range pairs per particle for the final passes
 
 enum Ctr : int {
@@ -50,6 +55,7 @@ enum Ctr : int {
     CTR_MAXTILE,
     CTR_CUNITS,
     CTR_LISTOVF,
+    CTR_BIGCELLS,  // cells with more than kSmallCell particles (sorted by k_sort_cells)
     CTR_NUM = 16
 };
 constexpr uint32_t kStatRep = 64;                           // copies of the statistics counters
@@ -116,6 +122,7 @@ struct Bufs {
     uint32_t* key;
     uint32_t* rank;
     uint32_t* tmp;
+    uint32_t* big;            // [cap/kSmallCell+2] cells whose segment of tmp k_sort_cells sorts
     uint32_t* sidx;
     float4* Pf;
     float4* Q;
@@ -219,6 +226,7 @@ struct AdjBufs {
     uint32_t* xnan;     // XSPH NaN guard per slot
     // particles grouped by final cell: the transposed 27-cell sets
     uint32_t *ckey, *crank, *ctmp, *cperm, *ccount, *coffset, *ctsum, *ctflag, *ctprefix;
+    uint32_t *big, *nbig;  // final cells whose segment of ctmp k_sort_cells sorts, and their count
     // per-kernel copies in final-cell order (position q holds slot cperm[q]):
     // (final position, slot bits), a float4 and a double4 of the pass
     float4* gP;
@@ -266,6 +274,10 @@ hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t
 hipError_t launch_selftest_inv_sqrt(uint32_t lo, uint32_t hi, unsigned long long* mismatches, uint32_t* first,
                                     hipStream_t s);
 hipError_t launch_density(const Bufs& b, const Scene& s0, const float* x, float* rho, hipStream_t s);
+// Sorts, ascending, the segment vals[offset[c] .. offset[c+1]) of each cell c
+// listed in cells[0 .. *ncells) (segments of at most kSortMax).
+hipError_t launch_sort_cells(const uint32_t* offset, const uint32_t* cells, const uint32_t* ncells, uint32_t* vals,
+                             hipStream_t s);
 hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float* xo, float* vo, hipStream_t s);
 // Fills T.grid[] (and T.grid_units) for num_cu compute units.
 void size_unit_grids(Tuning& T, int num_cu);

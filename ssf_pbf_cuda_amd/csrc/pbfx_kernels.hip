@@ -179,12 +179,51 @@ __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count
 
 // ------------------------------------------------- K4: scatter
 // sortByCountingCUDA (pbf_counting_sort.cu:120-134): slot = off[key] + rank.
+// The rank-0 particle of a cell above kSmallCell particles lists the cell
+// for k_sort_cells.
 __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
                                                  const uint32_t* __restrict__ offset, uint32_t* __restrict__ tmp,
-                                                 uint32_t n) {
+                                                 uint32_t n, uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    tmp[offset[key[i]] + rank[i]] = i;
+    const uint32_t k = key[i], r = rank[i], o0 = offset[k];
+    tmp[o0 + r] = i;
+    if (r == 0u) list_big_cell(offset, k, o0, big, nbig);
+}
+
+// ------------------------------------------------- K4b: large cells in stable order
+// The segment of each listed cell (at most kSortMax indices, all distinct),
+// bitonic-sorted in LDS by one workgroup: the stable in-cell order is then
+// the segment's own order.
+__global__ __launch_bounds__(1024) void k_sort_cells(const uint32_t* __restrict__ offset,
+                                                     const uint32_t* __restrict__ cells,
+                                                     const uint32_t* __restrict__ ncells, uint32_t* __restrict__ vals) {
+    __shared__ uint32_t buf[kSortMax];
+    const uint32_t nc = *ncells;
+    for (uint32_t w = blockIdx.x; w < nc; w += gridDim.x) {
+        const uint32_t c = cells[w];
+        const uint32_t o0 = offset[c], len = offset[c + 1] - o0;
+        uint32_t P = 1u;
+        while (P < len) P <<= 1;
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < len ? vals[o0 + i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2u; k <= P; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint32_t a = buf[i], bv = buf[l];
+                        if ((a > bv) == ((i & k) == 0u)) {
+                            buf[i] = bv;
+                            buf[l] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) vals[o0 + i] = buf[i];
+        __syncthreads();
+    }
 }
 
 // Work items of one sort: the particle at the first slot f of a run of `uw`
@@ -252,9 +291,12 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
     by_scene<kB>(si, active, [&](uint32_t sc) {
         const uint32_t k = key[i];
         const uint32_t o0 = b.offset[k], o1 = b.offset[k + 1];
-        uint32_t r = 0;
-        for (uint32_t t = o0; t < o1; ++t) r += (tmp[t] < i) ? 1u : 0u;
-        const uint32_t f = o0 + r;
+        uint32_t f = s;  // a sorted cell (k_sort_cells): the rank is the position
+        if (!sorted_cell(o1 - o0)) {
+            uint32_t r = 0;
+            for (uint32_t t = o0; t < o1; ++t) r += (tmp[t] < i) ? 1u : 0u;
+            f = o0 + r;
+        }
         b.sidx[f] = i;
         if (kB) b.sid[f] = sc;
         const Phys& P = SC[sc].P;
@@ -1342,7 +1384,9 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     if (n == 0) return hipGetLastError();
     {
         Mark m(tm, s, K_SCATTER);
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, b.key, b.rank, b.offset, b.tmp, n);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, b.key, b.rank, b.offset, b.tmp, n, b.big,
+                           &b.ctr[CTR_BIGCELLS]);
+        hipLaunchKernelGGL(k_sort_cells, dim3(256), dim3(1024), 0, s, b.offset, b.big, &b.ctr[CTR_BIGCELLS], b.tmp);
     }
     {
         Mark m(tm, s, K_REORDER);
@@ -1453,6 +1497,12 @@ hipError_t launch_cell_ranges(const Bufs& b, uint32_t C, uint32_t* start, uint32
 hipError_t launch_copy_lambda(const Bufs& b, float* out, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_lambda, dim3(ceil_div(b.cap > 0 ? b.cap : 1, 256)), dim3(256), 0, s, b.offset, b.CT,
                        (const float4*)b.Q, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_cells(const uint32_t* offset, const uint32_t* cells, const uint32_t* ncells, uint32_t* vals,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_cells, dim3(256), dim3(1024), 0, s, offset, cells, ncells, vals);
     return hipGetLastError();
 }
 

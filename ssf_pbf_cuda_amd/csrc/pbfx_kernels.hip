@@ -866,25 +866,19 @@ __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __rest
     grid_pos(self, G.cw, gp);
     F3 sum = mk3(0.f, 0.f, 0.f);
     for (int dz = -1; dz <= 1; ++dz) {
-        uint32_t st[9], en[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
+        for (int q = 0; q < 9; ++q) {  // cells in z,y,x order; each cell's bounds read when it comes
             const uint32_t c = scene_key(gp[0] + (q % 3) - 1, gp[1] + (q / 3) - 1, gp[2] + dz, G);
-            st[q] = offset[c];
-            en[q] = offset[c + 1];
-        }
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
+            const uint32_t st = offset[c], en = offset[c + 1];
             F3 cs = mk3(0.f, 0.f, 0.f);
             // four candidates' records loaded together, then added in order
-            for (uint32_t j = st[q]; j < en[q]; j += 4u) {
+            for (uint32_t j = st; j < en; j += 4u) {
                 El2 el[4];
 #pragma unroll
-                for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(min(j + i, en[q] - 1u));
+                for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(min(j + i, en - 1u));
 #pragma unroll
                 for (uint32_t i = 0; i < 4u; ++i) {
                     const F3 d = f3of(el[i].p) - self;
-                    if (j + i < en[q] && !(glm_dot(d, d) > cut2)) acc(cs, term(el[i]));
+                    if (j + i < en && !(glm_dot(d, d) > cut2)) acc(cs, term(el[i]));
                 }
             }
             acc(sum, cs);

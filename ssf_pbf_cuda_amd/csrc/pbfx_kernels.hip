@@ -78,6 +78,11 @@ __device__ __forceinline__ uint32_t count_slot(uint32_t c, uint32_t C, uint32_t 
 // ------------------------------------------------- K2a: per-tile totals
 // Totals of the cell counts of each occupied scan tile (unoccupied tiles are
 // all zero); tile nT is the deleted bucket (cell C).
+// Scan workgroups have kScanBlock threads; a tile of at most kScanTile cells
+// gives each thread at most kScanPer of them.
+constexpr uint32_t kScanBlock = 256;
+constexpr uint32_t kScanPer = kScanTile / kScanBlock;
+
 __global__ __launch_bounds__(256) void k_tile_reduce(const uint32_t* __restrict__ count,
                                                      const uint32_t* __restrict__ tflag, uint32_t* __restrict__ tsum,
                                                      uint32_t nT, uint32_t lgT, uint32_t C, uint32_t lgnc) {
@@ -92,8 +97,16 @@ __global__ __launch_bounds__(256) void k_tile_reduce(const uint32_t* __restrict_
         return;
     }
     const uint32_t T = 1u << lgT, c0 = t << lgT;
+    // all of a thread's loads are issued before the first add
+    uint32_t v[kScanPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        const uint32_t j = threadIdx.x + k * kScanBlock;
+        v[k] = j < T ? count[count_slot(c0 + j, C, lgnc)] : 0u;
+    }
     uint32_t s = 0;
-    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) s += count[count_slot(c0 + j, C, lgnc)];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k) s += v[k];
     uint32_t total;
     block_excl_scan(s, wsum, total);
     if (threadIdx.x == 0) tsum[t] = total;
@@ -156,23 +169,45 @@ __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count
     const uint32_t c0 = t << lgT;
     const uint32_t tot = tsum[t];
     const uint32_t base = tprefix[t];
-    const uint32_t per = (T + blockDim.x - 1) / blockDim.x;
-    const uint32_t my0 = c0 + threadIdx.x * per;
-    const uint32_t myn = (threadIdx.x * per < T) ? min(per, T - threadIdx.x * per) : 0u;
     if (tot == 0) {
-        for (uint32_t j = 0; j < myn; ++j) offset[my0 + j] = base;
+        for (uint32_t i = threadIdx.x; i < T; i += blockDim.x) offset[c0 + i] = base;
         return;
     }
+    // The tile passes through LDS so that global reads and writes are
+    // coalesced; thread t scans cells t*per .. t*per+per-1 there.  Word w is
+    // kept at w + w/16, so the threads' 16-word strides hit distinct banks.
+    __shared__ uint32_t cell[kScanTile + kScanTile / 16];
+    auto at = [](uint32_t w) { return w + (w >> 4); };
+    {
+        uint32_t v[kScanPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t j = threadIdx.x + k * kScanBlock;
+            v[k] = j < T ? count[count_slot(c0 + j, C, lgnc)] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t j = threadIdx.x + k * kScanBlock;
+            if (j < T) cell[at(j)] = v[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t per = (T + blockDim.x - 1) / blockDim.x;
+    const uint32_t my0 = threadIdx.x * per;
+    const uint32_t myn = (my0 < T) ? min(per, T - my0) : 0u;
     uint32_t s = 0;
-    for (uint32_t j = 0; j < myn; ++j) s += count[count_slot(my0 + j, C, lgnc)];
+    for (uint32_t j = 0; j < myn; ++j) s += cell[at(my0 + j)];
     uint32_t total;
     uint32_t run = base + block_excl_scan(s, wsum, total);
     for (uint32_t j = 0; j < myn; ++j) {
-        const uint32_t cs = count_slot(my0 + j, C, lgnc);
-        uint32_t c = count[cs];
-        offset[my0 + j] = run;
+        const uint32_t c = cell[at(my0 + j)];
+        cell[at(my0 + j)] = run;
         run += c;
-        count[cs] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < T; i += blockDim.x) {
+        offset[c0 + i] = cell[at(i)];
+        count[count_slot(c0 + i, C, lgnc)] = 0;
     }
     if (threadIdx.x == 0) tflag[t] = 0;
 }
@@ -1367,12 +1402,12 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     }
     {
         Mark m(tm, s, K_SCAN_TILES);
-        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.tflag, b.tsum, b.nT, b.lgT, CT, 0u);
+        hipLaunchKernelGGL(k_tile_reduce, dim3(b.nT + 1), dim3(kScanBlock), 0, s, b.count, b.tflag, b.tsum, b.nT, b.lgT, CT, 0u);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.tsum, b.tprefix, b.nT + 1, b.ctr, 1u);
     }
     {
         Mark m(tm, s, K_SCAN_CELLS);
-        hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(256), 0, s, b.count, b.offset, b.tsum, b.tflag,
+        hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(kScanBlock), 0, s, b.count, b.offset, b.tsum, b.tflag,
                            b.tprefix, b.nT, b.lgT, CT, 0u);
     }
     if (n == 0) return hipGetLastError();
@@ -1421,10 +1456,10 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     {
         Mark m(tm, s, K_CUR_SORT);
         hipLaunchKernelGGL(k_cur_hash<kB>, dim3(nb), dim3(256), 0, s, b, SC, lguw, lgnc, lgTr, CR);
-        hipLaunchKernelGGL(k_tile_reduce, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.ctflag, b.ctsum, nTr, lgTr, CR,
+        hipLaunchKernelGGL(k_tile_reduce, dim3(nTr + 1), dim3(kScanBlock), 0, s, b.ccount, b.ctflag, b.ctsum, nTr, lgTr, CR,
                            lgnc);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.ctsum, b.ctprefix, nTr + 1, b.ctr, 0u);
-        hipLaunchKernelGGL(k_scan_cells, dim3(nTr + 1), dim3(256), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
+        hipLaunchKernelGGL(k_scan_cells, dim3(nTr + 1), dim3(kScanBlock), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
                            b.ctprefix, nTr, lgTr, CR, lgnc);
         hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, CT, b.ckey, b.crank, b.coffset,
                            b.cperm, uw, lguw, lgnc, b.cunits, b.ctr);
@@ -1466,9 +1501,9 @@ hipError_t launch_step(const Bufs& b, const Scene& s0, const Tuning& T, float* x
 
 hipError_t launch_count_scan(uint32_t* count, uint32_t* offset, uint32_t* tsum, uint32_t* tflag, uint32_t* tprefix,
                              uint32_t nT, uint32_t lgT, uint32_t C, hipStream_t s) {
-    hipLaunchKernelGGL(k_tile_reduce, dim3(nT + 1), dim3(256), 0, s, count, tflag, tsum, nT, lgT, C, 0u);
+    hipLaunchKernelGGL(k_tile_reduce, dim3(nT + 1), dim3(kScanBlock), 0, s, count, tflag, tsum, nT, lgT, C, 0u);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tsum, tprefix, nT + 1, (uint32_t*)nullptr, 0u);
-    hipLaunchKernelGGL(k_scan_cells, dim3(nT + 1), dim3(256), 0, s, count, offset, tsum, tflag, tprefix, nT, lgT, C, 0u);
+    hipLaunchKernelGGL(k_scan_cells, dim3(nT + 1), dim3(kScanBlock), 0, s, count, offset, tsum, tflag, tprefix, nT, lgT, C, 0u);
     return hipGetLastError();
 }
 

@@ -434,6 +434,34 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t* ctr, uint32_t k, bool ac
     return base + (lane - (uint32_t)(hl < 0 ? 0 : hl));
 }
 
+// wave_rank for keys in any lane order: all lanes holding one key share one
+// atomic, issued by the key's lowest lane, which adds the key's lane count;
+// a lane's rank is the number of lower lanes with its key.  The loop visits
+// each distinct key of the wave once and touches no memory, so that the
+// wave's atomics are all in flight together afterwards.
+__device__ __forceinline__ uint32_t wave_rank_any(uint32_t* ctr, uint32_t k, bool active, uint32_t slot) {
+    const uint32_t lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t todo = __ballot(active);
+    uint32_t leader = lane, rank = 0, cnt = 0;
+    while (todo) {
+        const int l = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t kl = (uint32_t)__shfl((int)k, l, 64);
+        const bool mine = active && k == kl;
+        const uint64_t m = __ballot(mine);
+        if (mine) {
+            leader = (uint32_t)l;
+            rank = (uint32_t)__popcll(m & below);
+        }
+        if (lane == (uint32_t)l) cnt = (uint32_t)__popcll(m);
+        todo &= ~m;
+    }
+    uint32_t base = 0;
+    if (active && leader == lane) base = atomicAdd(&ctr[slot], cnt);
+    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    return base + rank;
+}
+
 // Marks a scan tile as occupied: a read first, so that the common case (the
 // tile is already marked) issues no store.
 __device__ __forceinline__ void mark_tile(uint32_t* __restrict__ tflag, uint32_t t) {

@@ -722,7 +722,9 @@ __global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
 // counted through one contended line; key = run * nc + copy.  The same
 // pass packs each slot's record R = (x*, 0), (v, 0) with
 // v = inv_dt*(x* - x_sorted) (updateVelocityCUDA, pbf_update.cu:25-39), so
-// that a particle visited in run order costs one cache line.
+// that a particle visited in run order costs one cache line.  Final cells
+// need not follow slot order (a violent step scatters them), so equal keys
+// are merged over the whole wave, not only over neighbouring lanes.
 template <bool kB>
 __global__ __launch_bounds__(256) void k_cur_hash(Bufs b, Scenes<kB> SC, uint32_t lguw, uint32_t lgnc, uint32_t lgT,
                                                   uint32_t CR) {
@@ -740,7 +742,7 @@ __global__ __launch_bounds__(256) void k_cur_hash(Bufs b, Scenes<kB> SC, uint32_
         grid_pos(p, G.cw, gp);
         k = ((scene_key(gp[0], gp[1], gp[2], G) >> lguw) << lgnc) | (blockIdx.x & ((1u << lgnc) - 1u));
     });
-    const uint32_t r = wave_rank(b.ccount, k, active, count_slot(k, CR, lgnc));
+    const uint32_t r = wave_rank_any(b.ccount, k, active, count_slot(k, CR, lgnc));
     if (active) {
         mark_tile(b.ctflag, k >> lgT);
         b.ckey[s] = k;

@@ -203,4 +203,13 @@ def test_multi_step_config0_bit_exact():
 def test_config1_one_step():
     """One step of the 1,048,576-particle dam break against the oracle."""
     sc = scenes.config1()
-    compare_state("config1", gpu_run(sc), oracle_run(sc))
+    rep = compare_state("config1", gpu_run(sc), oracle_run(sc))
+    assert rep["bitdiff_x"] == 0 and rep["bitdiff_v"] == 0
+
+
+def test_config2_one_step():
+    """One step of the 8,388,608-particle double dam break (3 iterations, a
+    1024x256x256-cell grid) against the oracle, bit for bit."""
+    sc = scenes.config2()
+    rep = compare_state("config2", gpu_run(sc), oracle_run(sc))
+    assert rep["bitdiff_x"] == 0 and rep["bitdiff_v"] == 0

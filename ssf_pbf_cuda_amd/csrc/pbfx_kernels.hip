@@ -972,7 +972,10 @@ __device__ __forceinline__ uint32_t final_column(const Grid& G, const UnitInfo& 
 // particle in a pile of thousands does not scan its whole neighbourhood).
 template <bool kB>
 __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, uint32_t tile_cap, uint32_t lguw, uint32_t lgnc) {
+    static_assert(kChunk <= kBlock, "one own particle per thread");
     __shared__ UnitShared S;
+    __shared__ uint32_t colpos[kMaxW + 2];  // own particles per final column, then each column's first position
+    __shared__ uint32_t corder[kChunk];     // the item's own particles in final-column order
     extern __shared__ float4 smem[];
     float4* tp = smem;
     // the final x*, packed: the same values as the passes' records R (built
@@ -1009,16 +1012,38 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         const uint32_t sc = scene_index<kB>(b.kbase, b.nscenes, unit.x);
         const Grid G = SC[sc].G;
         const float cut2 = cut2_of(sc);
-        I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
+        for (uint32_t c = threadIdx.x; c < kMaxW + 2u; c += kBlock) colpos[c] = 0u;
+        I = unit_setup(G, b.offset, unit, tile_cap, S, tl);  // barriers order the clearing above
         if (I.ok) {
+            // The own particles are taken in final-column order (a counting
+            // sort in LDS), so that the lanes of a wave share their
+            // candidate cells; the order of the builds changes no list.
+            const OwnCur o = own_cur(b, unit, lguw, lgnc);
+            const bool mine = threadIdx.x < o.n;
+            uint32_t kc = 0, rk = 0;
+            if (mine) {
+                kc = min(final_column(G, I, f3of(Pf[b.cperm[o.c0 + threadIdx.x]])), I.W + 1u);
+                rk = atomicAdd(&colpos[kc], 1u);
+            }
             if (I.staged)
                 load_runs(S, I, [&](uint32_t t, uint32_t j) {
                     const float4 p = Pf[j];
                     tp[t] = make_float4(p.x, p.y, p.z, __uint_as_float(j));
                 });
             __syncthreads();
-            const OwnCur o = own_cur(b, unit, lguw, lgnc);
-            for (uint32_t i = threadIdx.x; i < o.n; i += kBlock) {
+            if (threadIdx.x == 0) {
+                uint32_t run = 0;
+                for (uint32_t c = 0; c <= I.W + 1u; ++c) {
+                    const uint32_t k = colpos[c];
+                    colpos[c] = run;
+                    run += k;
+                }
+            }
+            __syncthreads();
+            if (mine) corder[colpos[kc] + rk] = threadIdx.x;
+            __syncthreads();
+            if (mine) {
+                const uint32_t i = corder[threadIdx.x];
                 if (!I.staged) tl.slowpart++;
                 build(o.c0 + i, b.cperm[o.c0 + i], I.staged, G, cut2);
             }

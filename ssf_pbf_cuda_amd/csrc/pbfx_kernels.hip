@@ -975,7 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
     static_assert(kChunk <= kBlock, "one own particle per thread");
     __shared__ UnitShared S;
     __shared__ uint32_t colpos[kMaxW + 2];  // own particles per final column, then each column's first position
-    __shared__ uint32_t corder[kChunk];     // the item's own particles in final-column order
+    __shared__ uint32_t corder[kChunk];     // the item's own slots in final-column order
     extern __shared__ float4 smem[];
     float4* tp = smem;
     // the final x*, packed: the same values as the passes' records R (built
@@ -1015,14 +1015,16 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         for (uint32_t c = threadIdx.x; c < kMaxW + 2u; c += kBlock) colpos[c] = 0u;
         I = unit_setup(G, b.offset, unit, tile_cap, S, tl);  // barriers order the clearing above
         if (I.ok) {
-            // The own particles are taken in final-column order (a counting
-            // sort in LDS), so that the lanes of a wave share their
-            // candidate cells; the order of the builds changes no list.
+            // The item's own particles are put in final-column order (a
+            // counting sort in LDS), written back to cperm, so that the lanes
+            // of a wave share their candidate cells here and their pairs in
+            // the passes; the order of the particles changes no sum.
             const OwnCur o = own_cur(b, unit, lguw, lgnc);
             const bool mine = threadIdx.x < o.n;
-            uint32_t kc = 0, rk = 0;
+            uint32_t kc = 0, rk = 0, gm = 0;
             if (mine) {
-                kc = min(final_column(G, I, f3of(Pf[b.cperm[o.c0 + threadIdx.x]])), I.W + 1u);
+                gm = b.cperm[o.c0 + threadIdx.x];
+                kc = min(final_column(G, I, f3of(Pf[gm])), I.W + 1u);
                 rk = atomicAdd(&colpos[kc], 1u);
             }
             if (I.staged)
@@ -1040,12 +1042,13 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
                 }
             }
             __syncthreads();
-            if (mine) corder[colpos[kc] + rk] = threadIdx.x;
+            if (mine) corder[colpos[kc] + rk] = gm;
             __syncthreads();
             if (mine) {
-                const uint32_t i = corder[threadIdx.x];
+                const uint32_t g = corder[threadIdx.x];
+                b.cperm[o.c0 + threadIdx.x] = g;
                 if (!I.staged) tl.slowpart++;
-                build(o.c0 + i, b.cperm[o.c0 + i], I.staged, G, cut2);
+                build(o.c0 + threadIdx.x, g, I.staged, G, cut2);
             }
         } else {
             // no tile index range: the passes sum these particles directly

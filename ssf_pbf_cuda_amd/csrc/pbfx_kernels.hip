@@ -336,11 +336,13 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
         if (kB) b.sid[f] = sc;
         const Phys& P = SC[sc].P;
         const float dt = P.dt;
-        float q[3], xo[3];
+        // one 12-byte load each for the gathered x and v
+        const F3 xi = reinterpret_cast<const F3*>(x)[i], vi = reinterpret_cast<const F3*>(v)[i];
+        const float xo[3] = {xi.x, xi.y, xi.z}, vv[3] = {vi.x, vi.y, vi.z};
+        float q[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            xo[c] = x[3 * i + c];
-            float vs = __builtin_fmaf(P.g[c], dt, v[3 * i + c]);
+            float vs = __builtin_fmaf(P.g[c], dt, vv[c]);
             q[c] = __builtin_fmaf(vs, dt, xo[c]);
         }
         b.Pf[f] = make_float4(q[0], q[1], q[2], 0.f);
@@ -1197,12 +1199,8 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Bufs b, Scenes<kB> SC, floa
         const F3 self = f3of(R[2 * g]);
         const F3 selfv = f3of(R[2 * g + 1]);
         if (!P.xsph_on) {
-            xo[3 * g + 0] = self.x;
-            xo[3 * g + 1] = self.y;
-            xo[3 * g + 2] = self.z;
-            vo[3 * g + 0] = selfv.x;
-            vo[3 * g + 1] = selfv.y;
-            vo[3 * g + 2] = selfv.z;
+            reinterpret_cast<F3*>(xo)[g] = self;
+            reinterpret_cast<F3*>(vo)[g] = selfv;
             return;
         }
         const Grid G = SC[sc].G;
@@ -1217,12 +1215,8 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Bufs b, Scenes<kB> SC, floa
         } else {
             out = selfv + mulsv(P.xsph_c * P.m * P.inv_rho0, a);
         }
-        xo[3 * g + 0] = self.x;
-        xo[3 * g + 1] = self.y;
-        xo[3 * g + 2] = self.z;
-        vo[3 * g + 0] = out.x;
-        vo[3 * g + 1] = out.y;
-        vo[3 * g + 2] = out.z;
+        reinterpret_cast<F3*>(xo)[g] = self;  // 12-byte stores
+        reinterpret_cast<F3*>(vo)[g] = out;
     });
     flush_tally(b.ctr, tl);
 }
@@ -1245,12 +1239,8 @@ __global__ __launch_bounds__(256) void k_out(Bufs b, Scenes<kB> SC, const float4
         p = f3of(b.Pf[s]);
         vv = mulsv(SC[kB ? b.sid[s] : 0u].P.inv_dt, p - ld3(b.X, cap, s));
     }
-    xo[3 * s + 0] = p.x;
-    xo[3 * s + 1] = p.y;
-    xo[3 * s + 2] = p.z;
-    vo[3 * s + 0] = vv.x;
-    vo[3 * s + 1] = vv.y;
-    vo[3 * s + 2] = vv.z;
+    reinterpret_cast<F3*>(xo)[s] = p;  // 12-byte stores
+    reinterpret_cast<F3*>(vo)[s] = vv;
 }
 
 // ------------------------------------------------- views

@@ -1353,6 +1353,12 @@ inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 constexpr uint32_t kTilePad = 4;
 size_t smem_of(uint32_t tile_cap) { return (size_t)(tile_cap + kTilePad) * sizeof(float4); }
 
+// The final-list units' tile: at most kFinalTileMax entries whatever the
+// search's tile.  Their own particles are few per unit and taken in
+// final-column order, so the global-memory filter is about as fast as the
+// staged one, and the smaller tile lets more workgroups share a CU.
+uint32_t final_tile(const Tuning& T) { return T.tile_cap < kFinalTileMax ? T.tile_cap : kFinalTileMax; }
+
 }  // namespace
 
 void size_unit_grids(Tuning& T, int num_cu) {
@@ -1371,7 +1377,8 @@ void size_unit_grids(Tuning& T, int num_cu) {
         // current one fit a CU
         (void)hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_of(kTileCapMax));
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.f, kBlock, smem_of(T.tile_cap)) != hipSuccess ||
+        const uint32_t tc = k.id == K_FINAL_LIST ? final_tile(T) : T.tile_cap;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.f, kBlock, smem_of(tc)) != hipSuccess ||
             nb < 1)
             nb = 1;
         T.grid[k.id] = ncu * (uint32_t)nb < 8u ? 8u : ncu * (uint32_t)nb;
@@ -1486,7 +1493,8 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     }
     {
         Mark m(tm, s, K_FINAL_LIST);
-        hipLaunchKernelGGL(k_final_list<kB>, grid(K_FINAL_LIST), dim3(kBlock), smem, s, b, SC, T.tile_cap, lguw, lgnc);
+        const uint32_t ft = final_tile(T);
+        hipLaunchKernelGGL(k_final_list<kB>, grid(K_FINAL_LIST), dim3(kBlock), smem_of(ft), s, b, SC, ft, lguw, lgnc);
     }
     if (any_vort) {
         {

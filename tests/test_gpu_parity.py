@@ -146,7 +146,9 @@ def test_wide_units_bit_exact():
     assert np.array_equal(wide["neighbors"], ref["neighbors"][:nl])
     sw, sn = wide["stats"], narrow["stats"]
     assert 0 < sw["units"] < sn["units"], (sw["units"], sn["units"])
-    assert sw["slow_units"] == 0 and sw["tile_overflow"] == 0
+    # the search's 32-cell units fit the 4096 tile; the final-list units keep
+    # their tile at 1536 particles, so only they may be filtered from global memory
+    assert sw["tile_overflow"] == 0 and sw["slow_units"] <= sw["final_units"], sw
 
 
 def test_spheres_and_planes():

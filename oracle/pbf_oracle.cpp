@@ -130,6 +130,7 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
         store3(V, i, mulsv(k.inv_dt, load3(P, i) - load3(X, i)));
     }
 
+    const int plant = plant_code();
     const R warg = p->vort_h_as_inv_h ? k.h : k.inv_h;
     const R cut_w = R(support_bound2((float)k.inv_h));
     const R cut_wd = R(support_bound2(p->vort_h_as_inv_h ? p->smoothing_length : 1.f / p->smoothing_length));
@@ -146,11 +147,12 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
                 V3<R> vort = v3<R>(R(0), R(0), R(0));
                 if (r > R(0)) {
                     V3<R> dir = divvs(pd, r);
-                    vort = mulvs(glm_cross(vd, dir), wspiky_deriv<R>(r, warg));
+                    vort = mulvs(plant == ORC_PLANT_VORT_CROSS ? glm_cross(dir, vd) : glm_cross(vd, dir),
+                                 wspiky_deriv<R>(r, warg));
                 }
                 return vort;
             });
-            store3(Wv, i, mulsv(k.m * k.inv_rho0, w));
+            store3(Wv, i, plant == ORC_PLANT_VORT_NO_MR ? w : mulsv(k.m * k.inv_rho0, w));
         }
         // addVorticityConfinementCUDA pbf_vorticity_confinement.cu:121-222
         std::vector<R> Vn(V);
@@ -160,7 +162,7 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
             V3<R> self = load3(P, i), selfw = load3(Wv, i);
             V3<R> N = sum27<R>(ns, P, self, cut_wd, [&](uint32_t j, V3<R> pd, R r2) {
                 R r = std::sqrt(r2);
-                R w = glm_length(load3(Wv, j));
+                R w = glm_length(load3(Wv, plant == ORC_PLANT_LOC_SELF ? i : j));
                 V3<R> loc = v3<R>(R(0), R(0), R(0));
                 if (r > R(0)) {
                     V3<R> dir = divvs(pd, r);
@@ -171,7 +173,9 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
             if (glm_length(N) > R(0)) {
                 N = mulvs(mulvs(N, k.m), k.inv_rho0);
                 N = divvs(N, glm_length(N));
-                V3<R> add = mulvs(divvs(mulsv(k.vc, glm_cross(N, selfw)), k.inv_rho0), k.dtc);
+                const V3<R> nw = plant == ORC_PLANT_CONF_CROSS ? glm_cross(selfw, N) : glm_cross(N, selfw);
+                V3<R> add = plant == ORC_PLANT_CONF_RHO ? mulvs(mulvs(mulsv(k.vc, nw), k.inv_rho0), k.dtc)
+                                                        : mulvs(divvs(mulsv(k.vc, nw), k.inv_rho0), k.dtc);
                 store3(Vn, i, load3(V, i) + add);
             }
         }
@@ -187,10 +191,11 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
         for (int64_t ii = 0; ii < (int64_t)nl; ++ii) {
             uint32_t i = (uint32_t)ii;
             V3<R> self = load3(P, i), selfv = load3(V, i);
-            V3<R> acc = sum27<R>(ns, P, self, cut_w, [&](uint32_t j, V3<R>, R r2) {
+            const R cut_x = plant == ORC_PLANT_XSPH_H ? R(support_bound2(p->smoothing_length)) : cut_w;
+            V3<R> acc = sum27<R>(ns, P, self, cut_x, [&](uint32_t j, V3<R>, R r2) {
                 R r = std::sqrt(r2);
-                V3<R> vd = load3(V, j) - selfv;
-                R kr = wpoly6<R>(r, k.inv_h);
+                V3<R> vd = plant == ORC_PLANT_XSPH_SIGN ? selfv - load3(V, j) : load3(V, j) - selfv;
+                R kr = wpoly6<R>(r, plant == ORC_PLANT_XSPH_H ? k.h : k.inv_h);
                 return mulvs(vd, kr);
             });
             V3<R> out;
@@ -198,7 +203,7 @@ void solve_and_update(const orc_params* p, const orc_boundary* b, const NSData& 
                 out = selfv;
                 nan_count++;
             } else {
-                out = selfv + mulsv(k.xsph * k.m * k.inv_rho0, acc);
+                out = selfv + mulsv(plant == ORC_PLANT_XSPH_NO_MR ? k.xsph : k.xsph * k.m * k.inv_rho0, acc);
             }
             store3(v_out, i, out);
         }
@@ -222,6 +227,8 @@ void apply_threads() {
 extern "C" {
 
 void orc_set_threads(int n) { g_threads = n; }
+
+void orc_set_plant(int code) { plant_code() = code; }
 
 void orc_arith_f32(const float* a, const float* b, float* out, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) {

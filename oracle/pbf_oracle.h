@@ -185,6 +185,25 @@ int orc_adj_identity(const orc_tape* t, uint32_t* id);
 void orc_set_threads(int n);
 int orc_get_threads(void);
 
+/* Planted mistakes.  The closed-form tests of the oracle switch one of these
+ * on and assert that their check then FAILS, which shows the check would
+ * catch that mistake in the restatement.  ORC_PLANT_NONE (the default) is the
+ * faithful restatement; nothing but those tests may set another value. */
+enum {
+    ORC_PLANT_NONE = 0,
+    ORC_PLANT_XSPH_SIGN = 1,   /* (v_i - v_j) where pbf_xsph.cu:23 has v_j - v_i */
+    ORC_PLANT_XSPH_NO_MR = 2,  /* XSPH without the m * inv_rho0 of pbf_xsph.cu:110 */
+    ORC_PLANT_XSPH_H = 3,      /* poly6 evaluated with h where pbf_xsph.cu:25-26 passes 1/h */
+    ORC_PLANT_VORT_CROSS = 4,  /* cross(dir, v_j - v_i) where pbf_vorticity_confinement.cu:28 has cross(v_j - v_i, dir) */
+    ORC_PLANT_VORT_NO_MR = 5,  /* vorticity without its m * inv_rho0 factor */
+    ORC_PLANT_CONF_CROSS = 6,  /* cross(omega, N) where pbf_vorticity_confinement.cu:220 has cross(N, omega) */
+    ORC_PLANT_CONF_RHO = 7,    /* times inv_rho0 where pbf_vorticity_confinement.cu:220 divides by it */
+    ORC_PLANT_LOC_SELF = 8,    /* |omega_i| where pbf_vorticity_confinement.cu:129 takes |omega_j| */
+    ORC_PLANT_NUDGE = 9,       /* the sphere responses' nudges 1.0001 / 0.9999 swapped (pbf_sphere_boundary.cu:19,47) */
+    ORC_PLANT_CROSS_FRAC = 10  /* |pb_n| / (|pa_n| + |pb_n|) where pbf_plane_boundary.cu:23 has |pa_n| / (...) */
+};
+void orc_set_plant(int code);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,13 @@
 
 namespace orc_core {
 
+// The planted mistake in force (ORC_PLANT_*, pbf_oracle.h); ORC_PLANT_NONE
+// outside the tests that show their checks catch such mistakes.
+inline int& plant_code() {
+    static int code = ORC_PLANT_NONE;
+    return code;
+}
+
 template <class R>
 struct V3 {
     R x, y, z;
@@ -361,7 +368,8 @@ inline int response_plane(bool& hit, V3<R>& dp, V3<R> old_p, V3<R> pred, V3<R> P
     }
     if (kase == 1) {
         hit = true;
-        R frac = std::fabs(pa_n) / (std::fabs(pa_n) + std::fabs(pb_n));
+        R frac = (plant_code() == ORC_PLANT_CROSS_FRAC ? std::fabs(pb_n) : std::fabs(pa_n)) /
+                 (std::fabs(pa_n) + std::fabs(pb_n));
         I = old_p + mulvs(pred - old_p, frac);
     } else if (kase == 2) {
         hit = true;
@@ -387,7 +395,7 @@ inline int response_sphere(bool& hit, V3<R>& dp, V3<R> pred, V3<R> center, R rad
     bool coll = force || (inner ? (d2 > radius * radius) : (d2 < radius * radius));
     if (!coll) return 0;
     hit = true;
-    R nudge = inner ? KC<R>::nudge_in : KC<R>::nudge_out;
+    R nudge = (inner != (plant_code() == ORC_PLANT_NUDGE)) ? KC<R>::nudge_in : KC<R>::nudge_out;
     dp.x = std::fma(cd.x, nudge, dp.x);
     dp.y = std::fma(cd.y, nudge, dp.y);
     dp.z = std::fma(cd.z, nudge, dp.z);

@@ -146,6 +146,8 @@ def lib():
         L.orc_set_threads.restype = None
         L.orc_get_threads.argtypes = []
         L.orc_get_threads.restype = ctypes.c_int
+        L.orc_set_plant.argtypes = [ctypes.c_int]
+        L.orc_set_plant.restype = None
         _lib = L
     return _lib
 
@@ -208,6 +210,37 @@ def set_threads(n: int) -> None:
 
 def get_threads() -> int:
     return int(lib().orc_get_threads())
+
+
+# Planted mistakes (pbf_oracle.h, ORC_PLANT_*): the closed-form tests switch
+# one on to show that their check fails on it.
+PLANT_NONE = 0
+PLANT_XSPH_SIGN = 1
+PLANT_XSPH_NO_MR = 2
+PLANT_XSPH_H = 3
+PLANT_VORT_CROSS = 4
+PLANT_VORT_NO_MR = 5
+PLANT_CONF_CROSS = 6
+PLANT_CONF_RHO = 7
+PLANT_LOC_SELF = 8
+PLANT_NUDGE = 9
+PLANT_CROSS_FRAC = 10
+
+
+class planted:
+    """Context manager: the oracle runs with planted mistake `code` inside
+    the block and as the faithful restatement again after it."""
+
+    def __init__(self, code: int):
+        self.code = int(code)
+
+    def __enter__(self):
+        lib().orc_set_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(PLANT_NONE)
+        return False
 
 
 def calc_grid_size(cell_size: float, end) -> Optional[tuple]:

@@ -187,6 +187,74 @@ def run_gpu(scene, steps, warmup, kernel_steps, tile_cap, unit_width):
             "tuning": {"tile_cap": tile_cap, "unit_width": unit_width, "by": tuned_by}}
 
 
+def run_chained(scene, steps, tile_cap, unit_width, block=100):
+    """Times `steps` CHAINED steps from the workload's input (the running
+    simulation, as the reference's application times frames,
+    main.cpp:339-348): each step continues from the previous one's output.
+    The run is timed with HIP events at block boundaries only; being
+    deterministic, it is then repeated with per-launch kernel timing for the
+    per-kernel totals.  Returns ms per step over all steps and per block, the
+    per-kernel totals and the statistics of the last step of each block."""
+    import torch
+
+    from ssf_pbf_cuda_amd import pbfx
+
+    n = scene.n
+    solver = pbfx.Solver.for_scene(scene)
+    solver.set_tuning(tile_cap, unit_width)
+    stream = torch.cuda.current_stream()
+    x0 = torch.from_numpy(scene.x).cuda()
+    v0 = torch.from_numpy(scene.v).cuda()
+    lo, hi, iters = scene.area_lo, scene.area_hi, scene.iters
+    x, v = x0.clone(), v0.clone()
+    # warm-up on a scratch copy: the timed run starts from the input itself
+    xw, vw = x0.clone(), v0.clone()
+    for _ in range(3):
+        solver.step_async(xw, vw, n, lo, hi, iters, stream)
+    del xw, vw
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps // block + 2)]
+    log(f"{scene.name}: timing {steps} chained steps")
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    marks = []
+    for k in range(steps):
+        solver.step_async(x, v, n, lo, hi, iters, stream)
+        if (k + 1) % block == 0 or k + 1 == steps:
+            ev[len(marks) + 1].record(stream)
+            marks.append(k + 1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = ev[0].elapsed_time(ev[len(marks)])
+    blocks = []
+    prev = 0
+    for i, m in enumerate(marks):
+        blocks.append({"steps": f"{prev}-{m}", "ms_per_step": round(ev[i].elapsed_time(ev[i + 1]) / (m - prev), 4)})
+        prev = m
+    live = solver.num(stream)
+    assert live == n, f"particles were deleted ({n} -> {live})"
+    finite = bool(torch.isfinite(x).all() and torch.isfinite(v).all())
+    # the same run again, each launch timed, with the statistics at block ends
+    log(f"{scene.name}: chained per-kernel pass")
+    x.copy_(x0)
+    v.copy_(v0)
+    solver.set_kernel_timing(True)
+    totals = {}
+    for k in range(steps):
+        solver.step_async(x, v, n, lo, hi, iters, stream)
+        totals = solver.kernel_times(stream)  # running totals since timing was enabled
+        if (k + 1) % block == 0 or k + 1 == steps:
+            st = solver.stats(stream)
+            blk = next(b for b in blocks if b["steps"].endswith(f"-{k + 1}"))
+            blk.update({key: st[key] for key in ("max_tile", "list_overflow", "slow_units", "slow_particles",
+                                                 "truncated", "sweep_cap_hits", "tile_overflow", "scan_tiles")})
+    solver.set_kernel_timing(False)
+    solver.close()
+    torch.cuda.empty_cache()
+    return {"ms_total": ms_total, "wall_s": wall, "steps": steps, "blocks": blocks, "finite": finite,
+            "kernels": totals}
+
+
 def run_batch(scs, steps, warmup):
     """The batched form against a per-scene loop, on the same GPU and input.
 

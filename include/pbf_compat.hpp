@@ -24,12 +24,14 @@
 //    builds, do the same here.
 //  - Tensile correction, update and collision follow the reference as shipped:
 //    REFERENCE s_corr, no XSPH or vorticity (pbf/solver/pbf_solve.cpp:125-145).
-//    pbf_compat_options() switches to the full step (XSPH, vorticity, PAPER s_corr).
+//    A particle's `options` (defaults from pbf_compat_options()) switch its
+//    step to the full one (XSPH, vorticity, PAPER s_corr).
 #ifndef PBF_COMPAT_HPP
 #define PBF_COMPAT_HPP
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -73,6 +75,10 @@ inline void check_hip(const char* what, hipError_t e) {
 }  // namespace pbf_compat_detail
 
 // Step options beyond the reference's fields; the defaults are the shipped step.
+// Each pbf_particle carries its own copy (pbf_particle::options), taken from
+// pbf_compat_options() when the particle is constructed, so simulators in one
+// process can run different s_corr forms and passes (the reference keeps its
+// s_corr constant in a function-static, pbf_solve.cpp:92).
 struct pbf_compat_options_t {
     int32_t scorr_mode = PBFX_SCORR_REFERENCE;
     float scorr_k = 0.1f;
@@ -83,6 +89,7 @@ struct pbf_compat_options_t {
     int32_t max_sweeps = 32;
     hipStream_t stream = nullptr;
 };
+// The defaults new pbf_particle objects start from.
 inline pbf_compat_options_t& pbf_compat_options() {
     static pbf_compat_options_t o;
     return o;
@@ -136,6 +143,8 @@ struct pbf_particle {
     std::shared_ptr<pbfx_ctx> ns;  // the pbfx context (replaces pbf::neighbor_search)
     pbf_parameter parameter;
     pbf_external external;
+    pbf_compat_options_t options = pbf_compat_options();  // this simulatee's step options
+    uint64_t boundary_seen = ~0ull;                        // version of the boundary its context holds
     scalar_t current_time() const { return static_cast<scalar_t>(phase.current_step) * parameter.time_step; }
     void allocate(uint32_t n, dom_dim domain_end);
     void free() {
@@ -170,7 +179,7 @@ struct pbf_boundary {
         inner_spheres_num = (uint32_t)h_inner.size();
         outer_spheres_num = (uint32_t)h_outer.size();
         planes_num = (uint32_t)(h_points.size() < h_normals.size() ? h_points.size() : h_normals.size());
-        static uint64_t counter = 0;
+        static std::atomic<uint64_t> counter{0};
         version = ++counter;
     }
     void free() {
@@ -198,7 +207,7 @@ inline pbfx_params to_params(const pbf_particle& p, uint32_t max_pair) {
     q.body_force[0] = p.external.body_force.x;
     q.body_force[1] = p.external.body_force.y;
     q.body_force[2] = p.external.body_force.z;
-    const pbf_compat_options_t& o = pbf_compat_options();
+    const pbf_compat_options_t& o = p.options;
     q.scorr_mode = o.scorr_mode;
     q.scorr_k = o.scorr_k;
     q.scorr_n = o.scorr_n;
@@ -209,15 +218,6 @@ inline pbfx_params to_params(const pbf_particle& p, uint32_t max_pair) {
     q.max_pair = (int32_t)max_pair;
     return q;
 }
-
-// Which boundary the context last received, per context.
-inline uint64_t& boundary_version(pbfx_ctx* c) {
-    static std::vector<std::pair<pbfx_ctx*, uint64_t>> seen;
-    for (auto& s : seen)
-        if (s.first == c) return s.second;
-    seen.emplace_back(c, ~0ull);
-    return seen.back().second;
-}
 }  // namespace pbf_compat_detail
 
 inline void pbf_particle::allocate(uint32_t n, dom_dim domain_end) {
@@ -227,6 +227,7 @@ inline void pbf_particle::allocate(uint32_t n, dom_dim domain_end) {
         std::exit(-1);
     }
     phase.allocate(n);
+    boundary_seen = ~0ull;
     const float end[3] = {domain_end.x, domain_end.y, domain_end.z};
     const pbfx_params q = pbf_compat_detail::to_params(*this, 30);
     pbfx_ctx* c = nullptr;
@@ -245,7 +246,7 @@ inline void one_step(pbf_particle& simulatee, pbf_buffer& buffer, const pbf_boun
     if (!c) pbf_compat_detail::fail("one_step", PBFX_INVALID_ARG);
     const pbfx_params q = pbf_compat_detail::to_params(simulatee, buffer.max_pair_particle_num);
     pbf_compat_detail::check("one_step", pbfx_set_params(c, &q));
-    uint64_t& seen = pbf_compat_detail::boundary_version(c);
+    uint64_t& seen = simulatee.boundary_seen;
     if (seen != boundary.version) {
         pbf_compat_detail::check(
             "one_step",
@@ -262,7 +263,7 @@ inline void one_step(pbf_particle& simulatee, pbf_buffer& buffer, const pbf_boun
     pbf_compat_detail::check("one_step",
                              pbfx_step(c, reinterpret_cast<float*>(simulatee.phase.x),
                                        reinterpret_cast<float*>(simulatee.phase.v), &num, lo, hi,
-                                       num_solver_iteration, (void*)pbf_compat_options().stream));
+                                       num_solver_iteration, (void*)simulatee.options.stream));
     simulatee.phase.num = num;
 }
 
@@ -280,7 +281,7 @@ inline void one_step(pbf_particle& simulatee, pbf_buffer& buffer, const std::pai
     pbf_compat_detail::check("one_step",
                              pbfx_step_no_boundary(c, reinterpret_cast<float*>(simulatee.phase.x),
                                                    reinterpret_cast<float*>(simulatee.phase.v), &num, lo, hi,
-                                                   num_solver_iteration, (void*)pbf_compat_options().stream));
+                                                   num_solver_iteration, (void*)simulatee.options.stream));
     simulatee.phase.num = num;
 }
 
@@ -299,7 +300,7 @@ inline void addParticle(pbf_particle& particle, const dom_dim* adding_position, 
                                                 reinterpret_cast<float*>(particle.phase.v), &num,
                                                 reinterpret_cast<const float*>(adding_position),
                                                 reinterpret_cast<const float*>(adding_velocity), adding_num,
-                                                (void*)pbf_compat_options().stream));
+                                                (void*)particle.options.stream));
     particle.phase.num = num;
 }
 

@@ -40,8 +40,8 @@ typedef enum pbfx_status {
     PBFX_NEIGHBOR_TRUNCATED = 6,/* information: some particle had more in-range
                                   candidates than neighbour slots (the reference
                                   truncates silently, pbf_neighbor_search_util.cu:367) */
-    PBFX_TILE_OVERFLOW = 7,    /* a work unit's 27-cell neighbourhood held more than
-                                  65535 particles; the step result is invalid */
+    PBFX_TILE_OVERFLOW = 7,    /* reserved: never returned (neighbourhoods of any size
+                                  are searched) */
     PBFX_NOT_SUPPORTED = 8
 } pbfx_status;
 
@@ -188,6 +188,23 @@ pbfx_status pbfx_batch_step_async(pbfx_ctx* ctx, float* x, float* v, const uint3
                                   int iters, void* stream);
 pbfx_status pbfx_batch_offsets(pbfx_ctx* ctx, uint32_t* out_offsets, void* stream);
 
+/* Graph form of the batched step (the batched workload's "one recorded
+ * launch per step"): records `steps` consecutive batched steps on x, v (as
+ * pbfx_batch_step_async), each followed by the write of its output offsets
+ * into `offsets` (as pbfx_batch_offsets), into one hipGraph kept by the
+ * context; pbfx_batch_graph_launch replays it `launches` times on a stream.
+ * offsets: device [n_scenes+1], the input offsets of the next step (set by
+ * the caller before the first launch, updated in place by every step), so
+ * replays chain, per-scene deletion included: the kernels read the live
+ * input count from offsets[n_scenes] and size their grids for the capacity.
+ * The buffers, iteration count, parameters, boundaries and tuning are those
+ * at capture time (capture again after changing them).  `stream` must be a
+ * created stream (not the null stream); nothing runs while recording.
+ * No counterpart in the reference, which steps one scene per process. */
+pbfx_status pbfx_batch_graph_capture(pbfx_ctx* ctx, float* x, float* v, uint32_t* offsets, int iters, uint32_t steps,
+                                     void* stream);
+pbfx_status pbfx_batch_graph_launch(pbfx_ctx* ctx, uint32_t launches, void* stream);
+
 /* Replaces one scene's parameters (its grid must not change). */
 pbfx_status pbfx_batch_set_params(pbfx_ctx* ctx, uint32_t scene, const pbfx_params* params);
 
@@ -246,11 +263,12 @@ typedef struct pbfx_stats {
     uint32_t units;            /* work units (cell blocks) of the last step */
     uint32_t slow_units;       /* units whose neighbourhood exceeded the on-chip tile */
     uint32_t slow_particles;   /* final-pass particles filtered from global memory (sparse runs, large tiles) */
-    uint32_t tile_overflow;    /* units beyond 65535 neighbourhood particles (error) */
+    uint32_t tile_overflow;    /* reserved, always 0 (neighbourhoods of any size are searched) */
     uint32_t max_tile;         /* largest neighbourhood tile seen */
     uint32_t steps;            /* steps run by this context */
     uint32_t final_units;      /* work units of the final passes (final-cell runs) */
     uint32_t list_overflow;    /* particles with more than 64 in-range final pairs (summed directly) */
+    uint32_t scan_tiles;       /* occupied 4096-cell tiles of the cell scan (the others are not read) */
 } pbfx_stats;
 
 /* Statistics of the last step (synchronises `stream`). */

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_adj_cscatter(StepTape t, AdjBufs a) {
     if (s >= t.n_live) return;
     const uint32_t k = a.ckey[s], r = a.crank[s], o0 = a.coffset[k];
     a.ctmp[o0 + r] = s;
-    if (r == 0u) list_big_cell(a.coffset, k, o0, a.big, a.nbig);
+    if (r == 0u) list_big_cell(a.coffset, k, o0, a.big, a.nbig, a.cap);
 }
 
 // slot order inside each cell: position = number of smaller slots in the
@@ -996,10 +996,11 @@ hipError_t launch_adj_step(const Bufs& b, const Scene& s0, const StepTape& t, co
         hipLaunchKernelGGL(k_adj_ckey, gl, blk, 0, s, t, a, G, b.lgT);
         e = launch_count_scan(a.ccount, a.coffset, a.ctsum, a.ctflag, a.ctprefix, b.nT, b.lgT, b.CT, s);
         if (e != hipSuccess) return e;
-        e = hipMemsetAsync(a.nbig, 0, sizeof(uint32_t), s);
+        e = hipMemsetAsync(a.nbig, 0, 2 * sizeof(uint32_t), s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_adj_cscatter, gl, blk, 0, s, t, a);
-        e = launch_sort_cells(a.coffset, a.big, a.nbig, a.ctmp, s);
+        // the ranks are dead after the scatter: their array holds the huge cells' chunks
+        e = launch_sort_cells(a.coffset, a.big, a.nbig, a.ctmp, a.crank, a.cap, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_adj_corder, gl, blk, 0, s, t, a);
         // heavy lists of the final passes and the search cells: their

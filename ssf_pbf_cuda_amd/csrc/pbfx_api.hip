@@ -77,6 +77,8 @@ struct pbfx_ctx {
     double kt_ms[K_NUM] = {};
     uint32_t kt_n[K_NUM] = {};
     AdjState* adj = nullptr;             // pbfx_adj_enable
+    hipGraphExec_t graph = nullptr;      // pbfx_batch_graph_capture: one recorded step
+    uint32_t graph_steps = 0;            // steps each launch of it advances
 };
 
 namespace {
@@ -239,6 +241,7 @@ void free_bufs(pbfx_ctx* c) {
                     b.OM4,   b.ncnt,    b.nlist,   b.count, b.offset, b.tsum,  b.tprefix, b.units,
                     b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
                     b.tflag, b.ctflag,  b.flist,   b.fcnt,  b.isid,   b.sid,   b.ctr,     b.big,
+                    b.ovf,   b.part,
                     c->d_scenes, c->d_kbase, c->d_in_off, c->d_scene_off, c->d_default_sphere};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -359,7 +362,7 @@ pbfx_status read_counters(pbfx_ctx* c, hipStream_t s) {
     // fold the statistics copies into h[0..CTR_NUM)
     uint32_t* h = c->h_pinned;
     for (int k = 0; k < CTR_NUM; ++k) {
-        if (k == CTR_UNITS || k == CTR_CUNITS || k == CTR_BIGCELLS) continue;
+        if (k == CTR_UNITS || k == CTR_CUNITS || k == CTR_BIGCELLS || k == CTR_HUGECELLS || k == CTR_OVF) continue;
         const bool is_max = (k == CTR_MAXTILE || k == CTR_MAXSWEEP);
         uint32_t v = 0;
         for (uint32_t r = 1; r <= kStatRep; ++r) {
@@ -372,7 +375,6 @@ pbfx_status read_counters(pbfx_ctx* c, hipStream_t s) {
 }
 
 pbfx_status step_status(pbfx_ctx* c) {
-    if (c->h_pinned[CTR_OVERFLOW]) return PBFX_TILE_OVERFLOW;
     if (c->report_info) {
         if (c->h_pinned[CTR_TRUNC]) return PBFX_NEIGHBOR_TRUNCATED;
         if (c->h_pinned[CTR_SWEEPCAP]) return PBFX_NOT_CONVERGED;
@@ -458,7 +460,7 @@ pbfx_status create_common(pbfx_ctx* c) {
     PBFX_ALLOC(b.key, bc);
     PBFX_ALLOC(b.rank, bc);
     PBFX_ALLOC(b.tmp, bc);
-    PBFX_ALLOC(b.big, bc / kSmallCell + 2);
+    PBFX_ALLOC(b.big, big_list_len(b.cap));
     PBFX_ALLOC(b.sidx, bc);
     PBFX_ALLOC(b.isid, bc);
     PBFX_ALLOC(b.sid, bc);
@@ -484,6 +486,9 @@ pbfx_status create_common(pbfx_ctx* c) {
     PBFX_ALLOC(b.cunits, items);
     PBFX_ALLOC(b.flist, (size_t)kFinalK * bc);
     PBFX_ALLOC(b.fcnt, bc);
+    b.ovf_max = b.cap < kOvfMax ? b.cap : kOvfMax;
+    PBFX_ALLOC(b.ovf, b.ovf_max);
+    PBFX_ALLOC(b.part, 27 * (size_t)b.ovf_max);
     PBFX_ALLOC(b.tflag, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctflag, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctr, kCtrWords);
@@ -660,6 +665,7 @@ void pbfx_destroy(pbfx_ctx* c) {
     if (!c) return;
     DeviceGuard g(c->device);
     (void)hipDeviceSynchronize();
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     free_bufs(c);
     delete c;
 }
@@ -764,6 +770,64 @@ pbfx_status pbfx_batch_step(pbfx_ctx* c, float* x, float* v, uint32_t* counts, i
     return step_status(c);
 }
 
+pbfx_status pbfx_batch_graph_capture(pbfx_ctx* c, float* x, float* v, uint32_t* offsets, int iters, uint32_t steps,
+                                     void* stream) {
+    if (!c || !c->batch || !x || !v || !offsets || !stream || iters < 1 || steps < 1) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    const hipStream_t s = (hipStream_t)stream;
+    for (const auto& h : c->sc)
+        if ((uint32_t)make_phys(h.params).stride > c->list_stride) return PBFX_INVALID_ARG;
+    // the scene table goes to the device before the recording (a copy and a
+    // wait cannot be recorded)
+    pbfx_status st = sync_table(c, false, s);
+    if (st != PBFX_OK) return st;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e);
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    Bufs b = c->b;
+    b.in_off = offsets;
+    hipGraph_t graph = nullptr;
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(c, e);
+    hipError_t le = hipSuccess;
+    for (uint32_t k = 0; k < steps && le == hipSuccess; ++k) {
+        // grids sized for the capacity; the kernels read the live input count
+        // from offsets[n_scenes]
+        le = launch_step(b, c->last_table[0], c->T, x, v, c->capacity, iters, any_flag(c, 0), any_flag(c, 1),
+                         unit_width(c), s, nullptr);
+        // the output offsets become the next step's input offsets (every
+        // reader of the input offsets has run by now)
+        if (le == hipSuccess) le = launch_scene_offsets(b, offsets, s);
+    }
+    e = hipStreamEndCapture(s, &graph);
+    if (le != hipSuccess || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(c, le != hipSuccess ? le : e);
+    }
+    e = hipGraphInstantiate(&c->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        c->graph = nullptr;
+        return hip_fail(c, e);
+    }
+    c->graph_steps = steps;
+    return PBFX_OK;
+}
+
+pbfx_status pbfx_batch_graph_launch(pbfx_ctx* c, uint32_t launches, void* stream) {
+    if (!c || !c->batch || !c->graph) return PBFX_INVALID_ARG;
+    DeviceGuard g(c->device);
+    for (uint32_t k = 0; k < launches; ++k) {
+        hipError_t e = hipGraphLaunch(c->graph, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(c, e);
+        c->steps += c->graph_steps;
+    }
+    return PBFX_OK;
+}
+
 pbfx_status pbfx_add_particles(pbfx_ctx* c, float* x, float* v, uint32_t* num_inout, const float* add_x,
                                const float* add_v, uint32_t add_n, void* stream) {
     if (!c || c->batch || !x || !v || !num_inout || (add_n && (!add_x || !add_v))) return PBFX_INVALID_ARG;
@@ -803,11 +867,12 @@ pbfx_status pbfx_get_stats(pbfx_ctx* c, void* stream, pbfx_stats* out) {
     out->units = h[CTR_UNITS];
     out->slow_units = h[CTR_SLOWUNITS];
     out->slow_particles = h[CTR_SLOWPART];
-    out->tile_overflow = h[CTR_OVERFLOW];
+    out->tile_overflow = 0;
     out->max_tile = h[CTR_MAXTILE];
     out->steps = c->steps;
     out->final_units = h[CTR_CUNITS];
     out->list_overflow = h[CTR_LISTOVF];
+    out->scan_tiles = h[CTR_SCANTILES];
     return PBFX_OK;
 }
 
@@ -1109,8 +1174,8 @@ pbfx_status pbfx_adj_enable(pbfx_ctx* c, uint32_t max_steps, uint32_t every) {
     PBFX_ALLOC(a.ckey, cap);
     PBFX_ALLOC(a.crank, cap);
     PBFX_ALLOC(a.ctmp, cap);
-    PBFX_ALLOC(a.big, cap / kSmallCell + 2);
-    PBFX_ALLOC(a.nbig, 1);
+    PBFX_ALLOC(a.big, big_list_len(cap));
+    PBFX_ALLOC(a.nbig, 2);
     PBFX_ALLOC(a.cperm, cap);
     PBFX_ALLOC(a.ccount, CT + 1);
     PBFX_ALLOC(a.coffset, CT + 2);

@@ -470,13 +470,17 @@ __device__ __forceinline__ void mark_tile(uint32_t* __restrict__ tflag, uint32_t
 
 __device__ __forceinline__ F3 f3of(float4 f) { return mk3(f.x, f.y, f.z); }
 
-// Whether k_sort_cells sorts a cell of n particles.
-__device__ __forceinline__ bool sorted_cell(uint32_t n) { return n > kSmallCell && n <= kSortMax; }
+// Whether the step sorts the segment of a cell of n particles.
+__device__ __forceinline__ bool sorted_cell(uint32_t n) { return n > kSmallCell; }
 
-// Lists cell k (its segment starting at o0) for k_sort_cells when sorted_cell.
+// Lists cell k (its segment starting at o0) for the segment sort when
+// sorted_cell: cells up to kSortMax at big[nbig[0]++], larger ones at
+// big[huge_base(cap) + nbig[1]++].
 __device__ __forceinline__ void list_big_cell(const uint32_t* __restrict__ offset, uint32_t k, uint32_t o0,
-                                              uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
-    if (sorted_cell(offset[k + 1] - o0)) big[atomicAdd(nbig, 1u)] = k;
+                                              uint32_t* __restrict__ big, uint32_t* __restrict__ nbig, uint32_t cap) {
+    const uint32_t n = offset[k + 1] - o0;
+    if (n > kSortMax) big[huge_base(cap) + atomicAdd(&nbig[1], 1u)] = k;
+    else if (n > kSmallCell) big[atomicAdd(&nbig[0], 1u)] = k;
 }
 
 // Block -> index range, contiguous per XCD (workgroup b runs on XCD b % 8),

@@ -18,6 +18,8 @@
 //                                                 the same sort keyed by each particle's final cell
 //   flist, fcnt                    u32[64*cap], u32[cap]  in-range pairs of the final passes by
 //                                                 final-order position (groups of four, list_at), counts
+//   ovf, part                      u32[M], f32x4[27*M]  overflow positions and their per-cell
+//                                                 partial sums (M = min(cap, kOvfMax))
 //   ctr                            u32[16*65]    allocation counters, then 64 copies of the
 //                                                 statistics counters of the last step
 #pragma once
@@ -32,17 +34,26 @@ constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kMaxW = 32;            // widest work unit along x (cells)
 constexpr uint32_t kMaxCells = 9 * (kMaxW + 2);
 constexpr uint32_t kTileCapMax = 4096;
-constexpr uint32_t kTileIndexLimit = 65535;  // tile indices are stored as u16
 constexpr uint32_t kScanTile = 4096;      // cells per scan tile
 constexpr uint32_t kChunk = kBlock;       // own particles per work item
 constexpr uint32_t kFinalTileMax = 1536;  // largest LDS tile of the final-list units (particles)
 constexpr uint32_t kSparse = 64;          // final-cell runs below this go one thread per particle
-// Cells above kSmallCell particles get their index segment sorted (up to
-// kSortMax), so that a particle's in-cell rank is its position rather than a
-// count over the whole cell.
+// Cells above kSmallCell particles get their index segment sorted, so that a
+// particle's in-cell rank is its position rather than a count over the whole
+// cell: up to kSortMax in one LDS sort, above that ("huge" cells) in chunks of
+// kSortMax that are sorted in LDS and then merged by rank.
 constexpr uint32_t kSmallCell = 64;
 constexpr uint32_t kSortMax = 16384;
+// Entries of a big-cell list: the cells above kSmallCell, then (from
+// huge_base on) the cells above kSortMax.
+__host__ __device__ constexpr uint32_t huge_base(uint32_t cap) { return cap / kSmallCell + 2u; }
+__host__ __device__ constexpr uint32_t big_list_len(uint32_t cap) { return huge_base(cap) + cap / kSortMax + 2u; }
 constexpr uint32_t kFinalK = 64;          // stored in-range pairs per particle for the final passes
+// Final-pass particles with more than kFinalK pairs (piles): up to kOvfMax of
+// them per step get their 27 per-cell partial sums computed in parallel
+// (kOvfBlocks workgroups); the rest sum the cells one after another.
+constexpr uint32_t kOvfMax = 65536;
+constexpr uint32_t kOvfBlocks = 2048;
 
 enum Ctr : int {
     CTR_UNITS = 0,
@@ -52,11 +63,14 @@ enum Ctr : int {
     CTR_XSPHNAN,
     CTR_SLOWUNITS,
     CTR_SLOWPART,
-    CTR_OVERFLOW,
+    CTR_RESERVED7,
     CTR_MAXTILE,
     CTR_CUNITS,
     CTR_LISTOVF,
-    CTR_BIGCELLS,  // cells with more than kSmallCell particles (sorted by k_sort_cells)
+    CTR_BIGCELLS,   // cells with kSmallCell < n <= kSortMax particles (sorted by k_sort_cells)
+    CTR_HUGECELLS,  // cells with more than kSortMax particles (chunk-sorted, then merged)
+    CTR_SCANTILES,  // occupied tiles of the cell scan (statistics)
+    CTR_OVF,        // final-pass positions whose list overflowed (allocation counter of ovf)
     CTR_NUM = 16
 };
 constexpr uint32_t kStatRep = 64;                           // copies of the statistics counters
@@ -123,7 +137,8 @@ struct Bufs {
     uint32_t* key;
     uint32_t* rank;
     uint32_t* tmp;
-    uint32_t* big;            // [cap/kSmallCell+2] cells whose segment of tmp k_sort_cells sorts
+    uint32_t* big;            // [big_list_len(cap)] cells whose segment of tmp k_sort_cells sorts (huge
+                              // cells from huge_base(cap) on)
     uint32_t* sidx;
     float4* Pf;
     float4* Q;
@@ -149,7 +164,10 @@ struct Bufs {
     uint32_t* ctprefix;
     uint2* cunits;
     uint32_t* flist;   // final pair lists, kFinalK entries per position in groups of four (list_at)
-    uint32_t* fcnt;    // pairs per final-order position
+    uint32_t* fcnt;    // pairs per final-order position (kFinalK + 1: direct sum; kFinalK + 2 + k: overflow entry k)
+    uint32_t* ovf;     // [ovf_max] final-order positions of the overflow entries
+    float4* part;      // [27 * ovf_max] their per-cell partial sums, cell-major
+    uint32_t ovf_max;
     uint32_t* ctr;
     uint32_t lgT;
     uint32_t nT;
@@ -227,7 +245,8 @@ struct AdjBufs {
     uint32_t* xnan;     // XSPH NaN guard per slot
     // particles grouped by final cell: the transposed 27-cell sets
     uint32_t *ckey, *crank, *ctmp, *cperm, *ccount, *coffset, *ctsum, *ctflag, *ctprefix;
-    uint32_t *big, *nbig;  // final cells whose segment of ctmp k_sort_cells sorts, and their count
+    uint32_t *big, *nbig;  // final cells whose segment of ctmp is sorted (big_list_len(cap)), and the two
+                           // counts (big, huge)
     // per-kernel copies in final-cell order (position q holds slot cperm[q]):
     // (final position, slot bits), a float4 and a double4 of the pass
     float4* gP;
@@ -275,10 +294,12 @@ hipError_t launch_selftest(const float* a, const float* bb, float* out, uint32_t
 hipError_t launch_selftest_inv_sqrt(uint32_t lo, uint32_t hi, unsigned long long* mismatches, uint32_t* first,
                                     hipStream_t s);
 hipError_t launch_density(const Bufs& b, const Scene& s0, const float* x, float* rho, hipStream_t s);
-// Sorts, ascending, the segment vals[offset[c] .. offset[c+1]) of each cell c
-// listed in cells[0 .. *ncells) (segments of at most kSortMax).
+// Sorts, ascending, the segment vals[offset[c] .. offset[c+1]) (distinct
+// values) of each cell c listed in cells: cells[0 .. ncells[0]) hold at most
+// kSortMax each, cells[huge_base(cap) .. + ncells[1]) more.  scratch (as long
+// as vals) holds the huge cells' sorted chunks in between.
 hipError_t launch_sort_cells(const uint32_t* offset, const uint32_t* cells, const uint32_t* ncells, uint32_t* vals,
-                             hipStream_t s);
+                             uint32_t* scratch, uint32_t cap, hipStream_t s);
 hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float* xo, float* vo, hipStream_t s);
 // Fills T.grid[] (and T.grid_units) for num_cu compute units.
 void size_unit_grids(Tuning& T, int num_cu);

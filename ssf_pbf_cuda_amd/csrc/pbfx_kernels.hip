@@ -30,14 +30,17 @@ namespace {
 // (pbf_delete.cu:9-28, sentinel key CT) and the histogram of insertAndCountCUDA
 // (pbf_counting_sort.cu:21-32) with wave-aggregated atomics; each particle
 // marks its 4096-cell scan tile as occupied (a plain store, no contention).
-// Input i belongs to the scene whose input range holds it.
+// Input i belongs to the scene whose input range holds it.  A batch takes
+// its input count from the device (in_off[nscenes], at most n, the count the
+// grid was sized for), so that a recorded graph of steps stays valid while
+// deletion shrinks the scenes.
 template <bool kB>
 __global__ __launch_bounds__(256) void k_predict_hash(const float* __restrict__ x, const float* __restrict__ v,
                                                       uint32_t n, Bufs b, Scenes<kB> SC, uint32_t* __restrict__ key,
                                                       uint32_t* __restrict__ rank, uint32_t* __restrict__ count,
                                                       uint32_t* __restrict__ tflag, uint32_t lgT) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = i < n;
+    const bool active = i < (kB ? min(n, b.in_off[b.nscenes]) : n);
     uint32_t k = 0;
     const uint32_t si = active ? scene_index<kB>(b.in_off, b.nscenes, i) : 0u;
     by_scene<kB>(si, active, [&](uint32_t sc) {
@@ -149,11 +152,14 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict_
 // Workgroup t scans the counts of cell tile t (empty tiles only write their
 // prefix), clears the counts it read and the tile's occupied flag; the last
 // workgroup writes the deleted bucket: off[C] = live count, off[C+1] = n.
-// (CUB DeviceScan::ExclusiveSum in pbf_counting_sort.cu:62-71.)
+// (CUB DeviceScan::ExclusiveSum in pbf_counting_sort.cu:62-71.)  When
+// `stat` is given, the occupied tiles are counted into its statistics
+// copies (CTR_SCANTILES).
 __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
                                                     const uint32_t* __restrict__ tsum, uint32_t* __restrict__ tflag,
                                                     const uint32_t* __restrict__ tprefix, uint32_t nT,
-                                                    uint32_t lgT, uint32_t C, uint32_t lgnc) {
+                                                    uint32_t lgT, uint32_t C, uint32_t lgnc,
+                                                    uint32_t* __restrict__ stat) {
     __shared__ uint32_t wsum[kWaves];
     const uint32_t t = blockIdx.x;
     if (t == nT) {
@@ -173,6 +179,7 @@ __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count
         for (uint32_t i = threadIdx.x; i < T; i += blockDim.x) offset[c0 + i] = base;
         return;
     }
+    if (stat && threadIdx.x == 0) atomicAdd(&stat[CTR_NUM * (1u + (t & (kStatRep - 1u))) + CTR_SCANTILES], 1u);
     // The tile passes through LDS so that global reads and writes are
     // coalesced; thread t scans cells t*per .. t*per+per-1 there.  Word w is
     // kept at w + w/16, so the threads' 16-word strides hit distinct banks.
@@ -216,48 +223,108 @@ __global__ __launch_bounds__(256) void k_scan_cells(uint32_t* __restrict__ count
 // sortByCountingCUDA (pbf_counting_sort.cu:120-134): slot = off[key] + rank.
 // The rank-0 particle of a cell above kSmallCell particles lists the cell
 // for k_sort_cells.
+// n_dev (a batch's device-resident input count, or NULL) bounds the inputs.
 __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
                                                  const uint32_t* __restrict__ offset, uint32_t* __restrict__ tmp,
-                                                 uint32_t n, uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
+                                                 uint32_t n, const uint32_t* __restrict__ n_dev,
+                                                 uint32_t* __restrict__ big, uint32_t* __restrict__ nbig,
+                                                 uint32_t cap) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (n_dev ? min(n, *n_dev) : n)) return;
     const uint32_t k = key[i], r = rank[i], o0 = offset[k];
     tmp[o0 + r] = i;
-    if (r == 0u) list_big_cell(offset, k, o0, big, nbig);
+    if (r == 0u) list_big_cell(offset, k, o0, big, nbig, cap);
 }
 
 // ------------------------------------------------- K4b: large cells in stable order
-// The segment of each listed cell (at most kSortMax indices, all distinct),
-// bitonic-sorted in LDS by one workgroup: the stable in-cell order is then
-// the segment's own order.
-__global__ __launch_bounds__(1024) void k_sort_cells(const uint32_t* __restrict__ offset,
-                                                     const uint32_t* __restrict__ cells,
-                                                     const uint32_t* __restrict__ ncells, uint32_t* __restrict__ vals) {
-    __shared__ uint32_t buf[kSortMax];
-    const uint32_t nc = *ncells;
-    for (uint32_t w = blockIdx.x; w < nc; w += gridDim.x) {
-        const uint32_t c = cells[w];
-        const uint32_t o0 = offset[c], len = offset[c + 1] - o0;
-        uint32_t P = 1u;
-        while (P < len) P <<= 1;
-        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < len ? vals[o0 + i] : 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t k = 2u; k <= P; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-                    const uint32_t l = i ^ j;
-                    if (l > i) {
-                        const uint32_t a = buf[i], bv = buf[l];
-                        if ((a > bv) == ((i & k) == 0u)) {
-                            buf[i] = bv;
-                            buf[l] = a;
-                        }
+// Ascending bitonic sort of src[0..len) (len <= kSortMax, distinct values)
+// into dst[0..len) through LDS, by the whole workgroup (dst may be src: all
+// of src is read before the first write).  Contains barriers: call uniformly.
+__device__ void lds_sort_segment(const uint32_t* src, uint32_t* dst, uint32_t len, uint32_t* __restrict__ buf) {
+    uint32_t P = 1u;
+    while (P < len) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < len ? src[i] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t k = 2u; k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = buf[i], bv = buf[l];
+                    if ((a > bv) == ((i & k) == 0u)) {
+                        buf[i] = bv;
+                        buf[l] = a;
                     }
                 }
-                __syncthreads();
             }
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) vals[o0 + i] = buf[i];
-        __syncthreads();
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = buf[i];
+    __syncthreads();
+}
+
+// The segment of each listed cell (all indices distinct) in ascending order,
+// so that the stable in-cell order is the segment's own order.  Cells of at
+// most kSortMax (cells[0 .. ncells[0])) are sorted in place by one workgroup
+// each; the segment of a larger cell (cells[huge_base(cap) ..] + ncells[1])
+// is cut into chunks of kSortMax, each sorted by one workgroup into the same
+// range of `scratch`, for k_merge_huge.
+__global__ __launch_bounds__(1024) void k_sort_cells(const uint32_t* __restrict__ offset,
+                                                     const uint32_t* __restrict__ cells,
+                                                     const uint32_t* __restrict__ ncells, uint32_t* __restrict__ vals,
+                                                     uint32_t* __restrict__ scratch, uint32_t cap) {
+    __shared__ uint32_t buf[kSortMax];
+    const uint32_t nc = ncells[0];
+    for (uint32_t w = blockIdx.x; w < nc; w += gridDim.x) {
+        const uint32_t c = cells[w];
+        const uint32_t o0 = offset[c];
+        lds_sort_segment(vals + o0, vals + o0, offset[c + 1] - o0, buf);
+    }
+    const uint32_t nh = ncells[1];
+    const uint32_t* __restrict__ huge = cells + huge_base(cap);
+    uint32_t w = blockIdx.x;  // chunk index over all huge cells, in list order
+    for (uint32_t h = 0; h < nh; ++h) {
+        const uint32_t c = huge[h];
+        const uint32_t o0 = offset[c], len = offset[c + 1] - o0;
+        const uint32_t nch = (len + kSortMax - 1u) / kSortMax;
+        for (; w < nch; w += gridDim.x) {
+            const uint32_t q0 = w * kSortMax;
+            lds_sort_segment(vals + o0 + q0, scratch + o0 + q0, min(kSortMax, len - q0), buf);
+        }
+        w -= nch;
+    }
+}
+
+// Merges the sorted chunks of every huge cell: the element at position p of
+// chunk q lands at p + (the number of smaller elements in every other
+// chunk), found by binary search.  Reads scratch, writes vals.
+__global__ __launch_bounds__(256) void k_merge_huge(const uint32_t* __restrict__ offset,
+                                                    const uint32_t* __restrict__ cells,
+                                                    const uint32_t* __restrict__ ncells, uint32_t* __restrict__ vals,
+                                                    const uint32_t* __restrict__ scratch, uint32_t cap) {
+    const uint32_t nh = ncells[1];
+    const uint32_t* __restrict__ huge = cells + huge_base(cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t h = 0; h < nh; ++h) {
+        const uint32_t c = huge[h];
+        const uint32_t o0 = offset[c], len = offset[c + 1] - o0;
+        const uint32_t* __restrict__ seg = scratch + o0;
+        const uint32_t nch = (len + kSortMax - 1u) / kSortMax;
+        for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < len; p += stride) {
+            const uint32_t x = seg[p], q = p / kSortMax;
+            uint32_t r = p - q * kSortMax;
+            for (uint32_t o = 0; o < nch; ++o) {
+                if (o == q) continue;
+                uint32_t lo = o * kSortMax, hi = min(lo + kSortMax, len);  // first element >= x
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (seg[mid] < x) lo = mid + 1u;
+                    else hi = mid;
+                }
+                r += lo - o * kSortMax;
+            }
+            vals[o0 + r] = x;
+        }
     }
 }
 
@@ -319,14 +386,14 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
                                                  const uint32_t* __restrict__ key, const uint32_t* __restrict__ tmp,
                                                  uint32_t n, uint32_t uw, uint32_t budget) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = s < n;
+    const bool active = s < (kB ? min(n, b.in_off[b.nscenes]) : n);
     const uint32_t i = active ? tmp[s] : 0u;
     const uint32_t si = (kB && active) ? b.isid[i] : 0u;
     uint32_t ek = kNoKey, ef = 0;
     by_scene<kB>(si, active, [&](uint32_t sc) {
         const uint32_t k = key[i];
         const uint32_t o0 = b.offset[k], o1 = b.offset[k + 1];
-        uint32_t f = s;  // a sorted cell (k_sort_cells): the rank is the position
+        uint32_t f = s;  // a sorted cell (k_sort_cells, k_merge_huge): the rank is the position
         if (!sorted_cell(o1 - o0)) {
             uint32_t r = 0;
             for (uint32_t t = o0; t < o1; ++t) r += (tmp[t] < i) ? 1u : 0u;
@@ -367,12 +434,12 @@ struct UnitShared {
 
 struct UnitInfo {
     uint32_t W, ncell, xb, y, z, total, own_t0, own_g0, n_own, nruns;
-    bool staged, ok;
+    bool staged;  // the neighbourhood fits the LDS tile; otherwise it is read from global memory
 };
 
 // Per-thread event tallies, summed per wave and added once at kernel end.
 struct Tally {
-    uint32_t trunc = 0, slowpart = 0, slowunits = 0, overflow = 0, maxtile = 0, sweepcap = 0, maxsweep = 0,
+    uint32_t trunc = 0, slowpart = 0, slowunits = 0, maxtile = 0, sweepcap = 0, maxsweep = 0,
              nan = 0, listovf = 0;
 };
 
@@ -394,13 +461,12 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 __device__ void flush_tally(uint32_t* __restrict__ ctr_base, const Tally& t) {
     uint32_t* ctr = ctr_base + CTR_NUM * (1u + (blockIdx.x & (kStatRep - 1u)));
     const uint32_t tr = wave_sum(t.trunc), sp = wave_sum(t.slowpart), su = wave_sum(t.slowunits),
-                   ov = wave_sum(t.overflow), mt = wave_max(t.maxtile), sc = wave_sum(t.sweepcap),
+                   mt = wave_max(t.maxtile), sc = wave_sum(t.sweepcap),
                    ms = wave_max(t.maxsweep), nn = wave_sum(t.nan), lo = wave_sum(t.listovf);
     if (lane_id() == 0) {
         if (tr) atomicAdd(&ctr[CTR_TRUNC], tr);
         if (sp) atomicAdd(&ctr[CTR_SLOWPART], sp);
         if (su) atomicAdd(&ctr[CTR_SLOWUNITS], su);
-        if (ov) atomicAdd(&ctr[CTR_OVERFLOW], ov);
         if (mt && mt > ctr[CTR_MAXTILE]) atomicMax(&ctr[CTR_MAXTILE], mt);  // a stale read only costs an atomic
         if (sc) atomicAdd(&ctr[CTR_SWEEPCAP], sc);
         if (ms && ms > ctr[CTR_MAXSWEEP]) atomicMax(&ctr[CTR_MAXSWEEP], ms);
@@ -485,12 +551,10 @@ __device__ UnitInfo unit_setup(const Grid& G, const uint32_t* __restrict__ offse
     I.own_t0 = S.tstart[base + 1] + skip;
     I.own_g0 = S.gstart[base + 1] + skip;
     I.n_own = min(kChunk, S.tstart[base + I.W + 1] - I.own_t0);
-    I.ok = total <= kTileIndexLimit;
-    I.staged = I.ok && total <= tile_cap;
+    I.staged = total <= tile_cap;
     if (threadIdx.x == 0) {
         tl.maxtile = max(tl.maxtile, total);
-        if (!I.ok) tl.overflow++;
-        else if (!I.staged) tl.slowunits++;
+        if (!I.staged) tl.slowunits++;
     }
     return I;
 }
@@ -600,23 +664,18 @@ __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t t
         const float h2 = SC[sc].P.h2;
         const uint32_t ncap = (uint32_t)SC[sc].P.cap;
         const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
-        if (I.ok) {
-            if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tile[t] = Pf[g]; });
-            __syncthreads();
-            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
-                const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
-                const uint32_t kc = own_column(S, I.W, ts);
-                bool trunc = false;
-                const uint32_t cnt =
-                    I.staged ? ns_filter<true>(S, I.W, kc, f3of(tile[ts]), h2, ncap, tile, nl, cap, g, trunc)
-                             : ns_filter<false>(S, I.W, kc, f3of(Pf[g]), h2, ncap, Pf, nl, cap, g, trunc);
-                b.ncnt[g] = cnt;
-                tl.trunc += trunc ? 1u : 0u;
-            }
-        } else {
-            // neighbourhood beyond the u16 tile index range: reported as
-            // tile_overflow; its particles get empty lists
-            for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) b.ncnt[I.own_g0 + q] = 0;
+        // a neighbourhood larger than the tile (any size) is filtered from global memory
+        if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tile[t] = Pf[g]; });
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
+            const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
+            const uint32_t kc = own_column(S, I.W, ts);
+            bool trunc = false;
+            const uint32_t cnt =
+                I.staged ? ns_filter<true>(S, I.W, kc, f3of(tile[ts]), h2, ncap, tile, nl, cap, g, trunc)
+                         : ns_filter<false>(S, I.W, kc, f3of(Pf[g]), h2, ncap, Pf, nl, cap, g, trunc);
+            b.ncnt[g] = cnt;
+            tl.trunc += trunc ? 1u : 0u;
         }
         __syncthreads();
     }
@@ -896,33 +955,42 @@ struct El2 {
     float4 p, q;
 };
 
+// Key of cell c (0..26, in z,y,x order) of the 27 around grid position gp.
+__device__ __forceinline__ uint32_t cell27(const int gp[3], uint32_t c, const Grid& G) {
+    const int dz = (int)(c / 9u) - 1, q = (int)(c % 9u);
+    return scene_key(gp[0] + (q % 3) - 1, gp[1] + (q / 3) - 1, gp[2] + dz, G);
+}
+
+// Partial sum of one cell (key) in slot order: the per-cell sum of
+// calcContribution; elem(j) builds the pass's element of slot j.
+template <class ElemF, class TermF>
+__device__ __forceinline__ F3 sum_cell(const uint32_t* __restrict__ offset, uint32_t key, F3 self, float cut2,
+                                       ElemF&& elem, TermF&& term) {
+    const uint32_t st = offset[key], en = offset[key + 1];
+    F3 cs = mk3(0.f, 0.f, 0.f);
+    // four candidates' records loaded together, then added in order
+    for (uint32_t j = st; j < en; j += 4u) {
+        El2 el[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(min(j + i, en - 1u));
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) {
+            const F3 d = f3of(el[i].p) - self;
+            if (j + i < en && !(glm_dot(d, d) > cut2)) acc(cs, term(el[i]));
+        }
+    }
+    return cs;
+}
+
 // Direct sum over the 27 cells from global memory, for particles whose list
-// overflowed: elem(j) builds the pass's element of slot j.
+// overflowed: the cells' partial sums added in z,y,x order.
 template <class ElemF, class TermF>
 __device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self, float cut2,
                                            ElemF&& elem, TermF&& term) {
     int gp[3];
     grid_pos(self, G.cw, gp);
     F3 sum = mk3(0.f, 0.f, 0.f);
-    for (int dz = -1; dz <= 1; ++dz) {
-        for (int q = 0; q < 9; ++q) {  // cells in z,y,x order; each cell's bounds read when it comes
-            const uint32_t c = scene_key(gp[0] + (q % 3) - 1, gp[1] + (q / 3) - 1, gp[2] + dz, G);
-            const uint32_t st = offset[c], en = offset[c + 1];
-            F3 cs = mk3(0.f, 0.f, 0.f);
-            // four candidates' records loaded together, then added in order
-            for (uint32_t j = st; j < en; j += 4u) {
-                El2 el[4];
-#pragma unroll
-                for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(min(j + i, en - 1u));
-#pragma unroll
-                for (uint32_t i = 0; i < 4u; ++i) {
-                    const F3 d = f3of(el[i].p) - self;
-                    if (j + i < en && !(glm_dot(d, d) > cut2)) acc(cs, term(el[i]));
-                }
-            }
-            acc(sum, cs);
-        }
-    }
+    for (uint32_t c = 0; c < 27u; ++c) acc(sum, sum_cell(offset, cell27(gp, c, G), self, cut2, elem, term));
     return sum;
 }
 
@@ -1016,46 +1084,40 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         const float cut2 = cut2_of(sc);
         for (uint32_t c = threadIdx.x; c < kMaxW + 2u; c += kBlock) colpos[c] = 0u;
         I = unit_setup(G, b.offset, unit, tile_cap, S, tl);  // barriers order the clearing above
-        if (I.ok) {
-            // The item's own particles are put in final-column order (a
-            // counting sort in LDS), written back to cperm, so that the lanes
-            // of a wave share their candidate cells here and their pairs in
-            // the passes; the order of the particles changes no sum.
-            const OwnCur o = own_cur(b, unit, lguw, lgnc);
-            const bool mine = threadIdx.x < o.n;
-            uint32_t kc = 0, rk = 0, gm = 0;
-            if (mine) {
-                gm = b.cperm[o.c0 + threadIdx.x];
-                kc = min(final_column(G, I, f3of(Pf[gm])), I.W + 1u);
-                rk = atomicAdd(&colpos[kc], 1u);
+        // The item's own particles are put in final-column order (a
+        // counting sort in LDS), written back to cperm, so that the lanes
+        // of a wave share their candidate cells here and their pairs in
+        // the passes; the order of the particles changes no sum.
+        const OwnCur o = own_cur(b, unit, lguw, lgnc);
+        const bool mine = threadIdx.x < o.n;
+        uint32_t kc = 0, rk = 0, gm = 0;
+        if (mine) {
+            gm = b.cperm[o.c0 + threadIdx.x];
+            kc = min(final_column(G, I, f3of(Pf[gm])), I.W + 1u);
+            rk = atomicAdd(&colpos[kc], 1u);
+        }
+        if (I.staged)
+            load_runs(S, I, [&](uint32_t t, uint32_t j) {
+                const float4 p = Pf[j];
+                tp[t] = make_float4(p.x, p.y, p.z, __uint_as_float(j));
+            });
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (uint32_t c = 0; c <= I.W + 1u; ++c) {
+                const uint32_t k = colpos[c];
+                colpos[c] = run;
+                run += k;
             }
-            if (I.staged)
-                load_runs(S, I, [&](uint32_t t, uint32_t j) {
-                    const float4 p = Pf[j];
-                    tp[t] = make_float4(p.x, p.y, p.z, __uint_as_float(j));
-                });
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t run = 0;
-                for (uint32_t c = 0; c <= I.W + 1u; ++c) {
-                    const uint32_t k = colpos[c];
-                    colpos[c] = run;
-                    run += k;
-                }
-            }
-            __syncthreads();
-            if (mine) corder[colpos[kc] + rk] = gm;
-            __syncthreads();
-            if (mine) {
-                const uint32_t g = corder[threadIdx.x];
-                b.cperm[o.c0 + threadIdx.x] = g;
-                if (!I.staged) tl.slowpart++;
-                build(o.c0 + threadIdx.x, g, I.staged, G, cut2);
-            }
-        } else {
-            // no tile index range: the passes sum these particles directly
-            const OwnCur o = own_cur(b, unit, lguw, lgnc);
-            for (uint32_t i = threadIdx.x; i < o.n; i += kBlock) b.fcnt[o.c0 + i] = 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        if (mine) corder[colpos[kc] + rk] = gm;
+        __syncthreads();
+        if (mine) {
+            const uint32_t g = corder[threadIdx.x];
+            b.cperm[o.c0 + threadIdx.x] = g;
+            if (!I.staged) tl.slowpart++;
+            build(o.c0 + threadIdx.x, g, I.staged, G, cut2);
         }
         __syncthreads();
     }
@@ -1131,27 +1193,122 @@ __device__ __forceinline__ F3 xsph_pair(F3 self, F3 selfv, F3 pj, F3 vj, float i
     return mulvs(vd, wpoly6(r, inv_h));
 }
 
+// The three final passes as (elements, pair term, support) for the particle
+// at slot g: the list walk, the direct sum, the per-cell partials and their
+// combination all evaluate a pass through these.
+enum FinalPass : int { PASS_VORT = 0, PASS_CONF = 1, PASS_XSPH = 2 };
+
+template <int kPass>
+struct PassTerms {
+    const float4* __restrict__ R;
+    const float4* __restrict__ OM4;
+    F3 self, selfv;
+    float arg, cut2;
+    __device__ __forceinline__ PassTerms(const Bufs& b, const Phys& P, uint32_t g)
+        : R(b.R), OM4(b.OM4), self(f3of(b.R[2 * g])),
+          selfv(kPass == PASS_CONF ? mk3(0.f, 0.f, 0.f) : f3of(b.R[2 * g + 1])),
+          arg(kPass == PASS_XSPH ? P.inv_h : P.warg), cut2(kPass == PASS_XSPH ? P.cut2_w : P.cut2_wd) {}
+    __device__ __forceinline__ El2 elem(uint32_t j) const {
+        if (kPass == PASS_CONF) {
+            const float4 r = R[2 * j];
+            return El2{make_float4(r.x, r.y, r.z, OM4[j].w), make_float4(0.f, 0.f, 0.f, 0.f)};
+        }
+        return El2{R[2 * j], R[2 * j + 1]};
+    }
+    __device__ __forceinline__ F3 term(const El2& e) const {
+        if (kPass == PASS_VORT) return vort_pair(self, selfv, f3of(e.p), f3of(e.q), arg);
+        if (kPass == PASS_CONF) return loc_pair(self, f3of(e.p), e.p.w, arg);
+        return xsph_pair(self, selfv, f3of(e.p), f3of(e.q), arg);
+    }
+};
+
+// Partial sum of cell c (0..26) of the 27 around the pass particle's final cell.
+template <class Pass>
+__device__ __forceinline__ F3 pass_cell(const Bufs& b, const Grid& G, const Pass& ps, uint32_t c) {
+    int gp[3];
+    grid_pos(ps.self, G.cw, gp);
+    return sum_cell(b.offset, cell27(gp, c, G), ps.self, ps.cut2, [&](uint32_t j) { return ps.elem(j); },
+                    [&](const El2& e) { return ps.term(e); });
+}
+
+// The pass's sum for final-order position q with stored count n: the list
+// (n <= kFinalK), the direct 27-cell sum (n = kFinalK + 1), or the 27
+// per-cell partials that k_ovf_part left for overflow entry n - (kFinalK + 2),
+// added in the same z,y,x order as the direct sum adds its cells.
+template <class Pass>
+__device__ __forceinline__ F3 pass_sum(const Bufs& b, const Grid& G, const Pass& ps, uint32_t q, uint32_t n) {
+    auto elem = [&](uint32_t j) { return ps.elem(j); };
+    auto term = [&](const El2& e) { return ps.term(e); };
+    if (n <= kFinalK) return sum_list(b.flist, b.cap, q, n, elem, term);
+    if (n == kFinalK + 1u) return sum27_global(G, b.offset, ps.self, ps.cut2, elem, term);
+    const size_t k = n - (kFinalK + 2u);
+    F3 sum = mk3(0.f, 0.f, 0.f);
+    for (uint32_t c = 0; c < 27u; ++c) acc(sum, f3of(b.part[(size_t)c * b.ovf_max + k]));
+    return sum;
+}
+
+// ------------------------------------------------- overflow lists
+// The final-order positions whose list overflowed (fcnt = kFinalK + 1) are
+// listed in ovf, up to ovf_max of them (one atomic per wave); a listed
+// position's fcnt becomes kFinalK + 2 + its index, so that the passes add its
+// per-cell partials, computed for all 27 cells at once by k_ovf_part, instead
+// of summing the cells one after another.  The others keep the direct sum.
+__global__ __launch_bounds__(256) void k_ovf_list(Bufs b) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool mine = q < b.offset[b.CT] && b.fcnt[q] == kFinalK + 1u;
+    const uint64_t m = __ballot(mine);
+    if (!m) return;  // uniform over the wave
+    const uint32_t lane = lane_id();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&b.ctr[CTR_OVF], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, 0, 64);
+    if (mine) {
+        const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (k < b.ovf_max) {
+            b.ovf[k] = q;
+            b.fcnt[q] = kFinalK + 2u + k;
+        }
+    }
+}
+
+// Per-cell partials of pass kPass for the listed overflow positions.  A wave
+// task is (group of 64 list entries, cell c): each lane sums cell c of its
+// particle's 27 in slot order (sum_cell, as the direct sum does) into
+// part[c * ovf_max + k].  The lanes of a group are neighbours in final-cell
+// order, so they mostly scan the same cell and load the same records.
+template <bool kB, int kPass>
+__global__ __launch_bounds__(256) void k_ovf_part(Bufs b, Scenes<kB> SC) {
+    const uint32_t nov = min(b.ctr[CTR_OVF], b.ovf_max);
+    const uint32_t ntask = (nov + 63u) / 64u * 27u;
+    const uint32_t lane = lane_id();
+    const uint32_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t t = w0; t < ntask; t += nw) {
+        const uint32_t grp = t / 27u, c = t - grp * 27u;
+        const uint32_t k = grp * 64u + lane;
+        const bool active = k < nov;
+        const uint32_t g = active ? b.cperm[b.ovf[k]] : 0u;
+        by_scene<kB>((kB && active) ? b.sid[g] : 0u, active, [&](uint32_t sc) {
+            const PassTerms<kPass> ps(b, SC[sc].P, g);
+            const F3 cs = pass_cell(b, SC[sc].G, ps, c);
+            b.part[(size_t)c * b.ovf_max + k] = make_float4(cs.x, cs.y, cs.z, 0.f);
+        });
+    }
+}
+
 // ------------------------------------------------- K10: vorticity
 // calcVorticityCUDA (pbf_vorticity_confinement.cu:87-115) with inv_h in the
 // spiky derivative (or h, when reproducing the reference's argument);
 // writes (omega, |omega|).  Elements: (x*_j, slot j).
 template <bool kB>
 __global__ __launch_bounds__(kBlock) void k_vort(Bufs b, Scenes<kB> SC) {
-    const float4* __restrict__ R = b.R;
     for_final<kB>(b, [&](uint32_t q, uint32_t g, uint32_t n, uint32_t sc) {
         const Phys P = SC[sc].P;
         if (!P.vort_on) {
             b.OM4[g] = make_float4(0.f, 0.f, 0.f, 0.f);
             return;
         }
-        const Grid G = SC[sc].G;
         const float mr = P.m * P.inv_rho0;
-        const F3 self = f3of(R[2 * g]);
-        const F3 selfv = f3of(R[2 * g + 1]);
-        auto elem = [&](uint32_t j) { return El2{R[2 * j], R[2 * j + 1]}; };
-        auto term = [&](const El2& e) { return vort_pair(self, selfv, f3of(e.p), f3of(e.q), P.warg); };
-        const F3 w = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
-                                  : sum27_global(G, b.offset, self, P.cut2_wd, elem, term);
+        const F3 w = pass_sum(b, SC[sc].G, PassTerms<PASS_VORT>(b, P, g), q, n);
         const F3 om = mulsv(mr, w);
         b.OM4[g] = make_float4(om.x, om.y, om.z, glm_length(om));
     });
@@ -1167,15 +1324,7 @@ __global__ __launch_bounds__(kBlock) void k_confine(Bufs b, Scenes<kB> SC) {
     for_final<kB>(b, [&](uint32_t q, uint32_t g, uint32_t n, uint32_t sc) {
         const Phys P = SC[sc].P;
         if (!P.vort_on) return;
-        const Grid G = SC[sc].G;
-        const F3 self = f3of(R[2 * g]);
-        auto elem = [&](uint32_t j) {
-            const float4 r = R[2 * j];
-            return El2{make_float4(r.x, r.y, r.z, OM4[j].w), make_float4(0.f, 0.f, 0.f, 0.f)};
-        };
-        auto term = [&](const El2& e) { return loc_pair(self, f3of(e.p), e.p.w, P.warg); };
-        F3 N = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
-                            : sum27_global(G, b.offset, self, P.cut2_wd, elem, term);
+        F3 N = pass_sum(b, SC[sc].G, PassTerms<PASS_CONF>(b, P, g), q, n);
         if (glm_length(N) > 0.f) {
             N = mulvs(mulvs(N, P.m), P.inv_rho0);
             N = divvs(N, glm_length(N));
@@ -1203,11 +1352,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph_out(Bufs b, Scenes<kB> SC, floa
             reinterpret_cast<F3*>(vo)[g] = selfv;
             return;
         }
-        const Grid G = SC[sc].G;
-        auto elem = [&](uint32_t j) { return El2{R[2 * j], R[2 * j + 1]}; };
-        auto term = [&](const El2& e) { return xsph_pair(self, selfv, f3of(e.p), f3of(e.q), P.inv_h); };
-        const F3 a = n <= kFinalK ? sum_list(b.flist, b.cap, q, n, elem, term)
-                                  : sum27_global(G, b.offset, self, P.cut2_w, elem, term);
+        const F3 a = pass_sum(b, SC[sc].G, PassTerms<PASS_XSPH>(b, P, g), q, n);
         F3 out;
         if (isnan(a.x) || isnan(a.y) || isnan(a.z)) {
             out = selfv;
@@ -1435,14 +1580,19 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     {
         Mark m(tm, s, K_SCAN_CELLS);
         hipLaunchKernelGGL(k_scan_cells, dim3(b.nT + 1), dim3(kScanBlock), 0, s, b.count, b.offset, b.tsum, b.tflag,
-                           b.tprefix, b.nT, b.lgT, CT, 0u);
+                           b.tprefix, b.nT, b.lgT, CT, 0u, b.ctr);
     }
     if (n == 0) return hipGetLastError();
     {
         Mark m(tm, s, K_SCATTER);
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, b.key, b.rank, b.offset, b.tmp, n, b.big,
-                           &b.ctr[CTR_BIGCELLS]);
-        hipLaunchKernelGGL(k_sort_cells, dim3(256), dim3(1024), 0, s, b.offset, b.big, &b.ctr[CTR_BIGCELLS], b.tmp);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, b.key, b.rank, b.offset, b.tmp, n,
+                           kB ? b.in_off + b.nscenes : (const uint32_t*)nullptr, b.big, &b.ctr[CTR_BIGCELLS], b.cap);
+        // the histogram ranks are dead after the scatter: their array holds the huge cells' chunks
+        static_assert(CTR_HUGECELLS == CTR_BIGCELLS + 1, "the sort reads the two cell counts together");
+        hipLaunchKernelGGL(k_sort_cells, dim3(256), dim3(1024), 0, s, b.offset, b.big, &b.ctr[CTR_BIGCELLS], b.tmp,
+                           b.rank, b.cap);
+        hipLaunchKernelGGL(k_merge_huge, dim3(256), dim3(256), 0, s, b.offset, b.big, &b.ctr[CTR_BIGCELLS], b.tmp,
+                           (const uint32_t*)b.rank, b.cap);
     }
     {
         Mark m(tm, s, K_REORDER);
@@ -1487,7 +1637,7 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
                            lgnc);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.ctsum, b.ctprefix, nTr + 1, b.ctr, 0u);
         hipLaunchKernelGGL(k_scan_cells, dim3(nTr + 1), dim3(kScanBlock), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
-                           b.ctprefix, nTr, lgTr, CR, lgnc);
+                           b.ctprefix, nTr, lgTr, CR, lgnc, (uint32_t*)nullptr);
         hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, CT, b.ckey, b.crank, b.coffset,
                            b.cperm, uw, lguw, lgnc, b.cunits, b.ctr);
     }
@@ -1495,17 +1645,23 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
         Mark m(tm, s, K_FINAL_LIST);
         const uint32_t ft = final_tile(T);
         hipLaunchKernelGGL(k_final_list<kB>, grid(K_FINAL_LIST), dim3(kBlock), smem_of(ft), s, b, SC, ft, lguw, lgnc);
+        hipLaunchKernelGGL(k_ovf_list, dim3(nb), dim3(256), 0, s, b);
     }
+    // the overflow positions' per-cell partials, before each pass that adds them
+    const dim3 ovf_grid(kOvfBlocks);
     if (any_vort) {
         {
             Mark m(tm, s, K_VORT);
+            hipLaunchKernelGGL((k_ovf_part<kB, PASS_VORT>), ovf_grid, dim3(256), 0, s, b, SC);
             hipLaunchKernelGGL(k_vort<kB>, dim3(nb), dim3(kBlock), 0, s, b, SC);
         }
         Mark m(tm, s, K_CONFINE);
+        hipLaunchKernelGGL((k_ovf_part<kB, PASS_CONF>), ovf_grid, dim3(256), 0, s, b, SC);
         hipLaunchKernelGGL(k_confine<kB>, dim3(nb), dim3(kBlock), 0, s, b, SC);
     }
     if (any_xsph) {
         Mark m(tm, s, K_XSPH_OUT);
+        hipLaunchKernelGGL((k_ovf_part<kB, PASS_XSPH>), ovf_grid, dim3(256), 0, s, b, SC);
         hipLaunchKernelGGL(k_xsph_out<kB>, dim3(nb), dim3(kBlock), 0, s, b, SC, x, v);
     } else {
         Mark m(tm, s, K_OUT);
@@ -1531,7 +1687,8 @@ hipError_t launch_count_scan(uint32_t* count, uint32_t* offset, uint32_t* tsum, 
                              uint32_t nT, uint32_t lgT, uint32_t C, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_reduce, dim3(nT + 1), dim3(kScanBlock), 0, s, count, tflag, tsum, nT, lgT, C, 0u);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tsum, tprefix, nT + 1, (uint32_t*)nullptr, 0u);
-    hipLaunchKernelGGL(k_scan_cells, dim3(nT + 1), dim3(kScanBlock), 0, s, count, offset, tsum, tflag, tprefix, nT, lgT, C, 0u);
+    hipLaunchKernelGGL(k_scan_cells, dim3(nT + 1), dim3(kScanBlock), 0, s, count, offset, tsum, tflag, tprefix, nT, lgT, C, 0u,
+                       (uint32_t*)nullptr);
     return hipGetLastError();
 }
 
@@ -1558,8 +1715,10 @@ hipError_t launch_copy_lambda(const Bufs& b, float* out, hipStream_t s) {
 }
 
 hipError_t launch_sort_cells(const uint32_t* offset, const uint32_t* cells, const uint32_t* ncells, uint32_t* vals,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_sort_cells, dim3(256), dim3(1024), 0, s, offset, cells, ncells, vals);
+                             uint32_t* scratch, uint32_t cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_cells, dim3(256), dim3(1024), 0, s, offset, cells, ncells, vals, scratch, cap);
+    hipLaunchKernelGGL(k_merge_huge, dim3(256), dim3(256), 0, s, offset, cells, ncells, vals,
+                       (const uint32_t*)scratch, cap);
     return hipGetLastError();
 }
 

@@ -99,7 +99,7 @@ class KernelTime(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "num_in", "num_live", "truncated", "sweep_cap_hits", "max_sweeps_used", "xsph_nan", "units", "slow_units",
-        "slow_particles", "tile_overflow", "max_tile", "steps", "final_units", "list_overflow")]
+        "slow_particles", "tile_overflow", "max_tile", "steps", "final_units", "list_overflow", "scan_tiles")]
 
 
 _lib = None
@@ -130,6 +130,8 @@ def lib():
             "pbfx_batch_step_async": ([vp, vp, vp, vp, u32, ctypes.c_int, vp], ctypes.c_int),
             "pbfx_batch_offsets": ([vp, vp, vp], ctypes.c_int),
             "pbfx_batch_set_params": ([vp, u32, ctypes.POINTER(Params)], ctypes.c_int),
+            "pbfx_batch_graph_capture": ([vp, vp, vp, vp, ctypes.c_int, u32, vp], ctypes.c_int),
+            "pbfx_batch_graph_launch": ([vp, u32, vp], ctypes.c_int),
             "pbfx_get_stats": ([vp, vp, ctypes.POINTER(Stats)], ctypes.c_int),
             "pbfx_ns_view_get": ([vp, ctypes.POINTER(NsView)], ctypes.c_int),
             "pbfx_ns_neighbors": ([vp, vp, vp, vp], ctypes.c_int),
@@ -472,3 +474,13 @@ class BatchSolver(Solver):
     def batch_set_params(self, scene: int, params: StepParams) -> None:
         _check(self._lib.pbfx_batch_set_params(self._ctx, int(scene), ctypes.byref(make_params(params))),
                "pbfx_batch_set_params")
+
+    def graph_capture(self, x, v, offsets, iters: int, steps: int, stream) -> None:
+        """pbfx_batch_graph_capture: records `steps` chained batched steps
+        (offsets: device int32 [n_scenes+1], updated in place by each step)."""
+        _check(self._lib.pbfx_batch_graph_capture(self._ctx, _ptr(x), _ptr(v), _ptr(offsets), int(iters),
+                                                  int(steps), _stream(stream)), "pbfx_batch_graph_capture")
+
+    def graph_launch(self, launches: int, stream) -> None:
+        _check(self._lib.pbfx_batch_graph_launch(self._ctx, int(launches), _stream(stream)),
+               "pbfx_batch_graph_launch")

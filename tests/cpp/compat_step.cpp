@@ -7,7 +7,11 @@
 //         xsph_parameter, vc_parameter, body_force[3], domain_end[3],
 //         area_lo[3], area_hi[3], plane_points[3*n_planes],
 //         plane_normals[3*n_planes], x[3n], v[3n]
-// output: u32 num, f32 x[3*num], v[3*num] (sorted order, as the reference leaves them)
+// output: twice (u32 num, f32 x[3*num], v[3*num]) (sorted order, as the
+//         reference leaves them): first the step as shipped, then the full
+//         step (PAPER s_corr k = 0.1, n = 4, dq = 0.2h, XSPH, vorticity).
+// The two simulatees live in one process and are stepped in turn, each with
+// its own options.
 #include <cstdio>
 #include <vector>
 
@@ -38,41 +42,55 @@ int main(int argc, char** argv) {
         return 2;
     std::fclose(f);
 
-    pbf_particle p;
-    p.parameter.stable_density = par[0];
-    p.parameter.stable_distance = par[1];
-    p.parameter.particle_mass = par[2];
-    p.parameter.smoothing_length = par[3];
-    p.parameter.time_step = par[4];
-    p.parameter.relaxation = par[5];
-    p.parameter.xsph_parameter = par[6];
-    p.parameter.vc_parameter = par[7];
-    p.external.body_force = dom_dim{g[0], g[1], g[2]};
-    p.allocate(n, dom_dim{end[0], end[1], end[2]});
+    auto make = [&](pbf_particle& p, bool full) {
+        p.parameter.stable_density = par[0];
+        p.parameter.stable_distance = par[1];
+        p.parameter.particle_mass = par[2];
+        p.parameter.smoothing_length = par[3];
+        p.parameter.time_step = par[4];
+        p.parameter.relaxation = par[5];
+        p.parameter.xsph_parameter = par[6];
+        p.parameter.vc_parameter = par[7];
+        p.external.body_force = dom_dim{g[0], g[1], g[2]};
+        if (full) {
+            p.options.scorr_mode = PBFX_SCORR_PAPER;
+            p.options.scorr_k = 0.1f;
+            p.options.scorr_n = 4;
+            p.options.scorr_dq_over_h = 0.2f;
+            p.options.xsph_on = 1;
+            p.options.vort_on = 1;
+        }
+        p.allocate(n, dom_dim{end[0], end[1], end[2]});
+        return hipMemcpy(p.phase.x, x.data(), n * sizeof(dom_dim), hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(p.phase.v, v.data(), n * sizeof(dom_dim), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    pbf_particle shipped, full;
+    if (!make(shipped, false) || !make(full, true)) return 3;
+    shipped.phase.num = full.phase.num = n;
     pbf_buffer buf;
     buf.allocate(n, 30);
     pbf_boundary bnd;
     bnd.allocate({}, {}, pts, nrm);
-    if (hipMemcpy(p.phase.x, x.data(), n * sizeof(dom_dim), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p.phase.v, v.data(), n * sizeof(dom_dim), hipMemcpyHostToDevice) != hipSuccess)
-        return 3;
-    p.phase.num = n;
     const std::pair<dom_dim, dom_dim> area{dom_dim{lo[0], lo[1], lo[2]}, dom_dim{hi[0], hi[1], hi[2]}};
-    for (uint32_t s = 0; s < steps; ++s) pbf::one_step(p, buf, bnd, area, (int)iters);
+    for (uint32_t s = 0; s < steps; ++s) {
+        pbf::one_step(shipped, buf, bnd, area, (int)iters);
+        pbf::one_step(full, buf, bnd, area, (int)iters);
+    }
 
-    const uint32_t num = p.phase.num;
-    x.resize(num);
-    v.resize(num);
-    if (hipMemcpy(x.data(), p.phase.x, num * sizeof(dom_dim), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(v.data(), p.phase.v, num * sizeof(dom_dim), hipMemcpyDeviceToHost) != hipSuccess)
-        return 3;
     std::FILE* o = std::fopen(argv[2], "wb");
     if (!o) return 2;
-    std::fwrite(&num, sizeof(num), 1, o);
-    std::fwrite(x.data(), sizeof(dom_dim), num, o);
-    std::fwrite(v.data(), sizeof(dom_dim), num, o);
+    for (pbf_particle* p : {&shipped, &full}) {
+        const uint32_t num = p->phase.num;
+        std::vector<dom_dim> xo(num), vo(num);
+        if (hipMemcpy(xo.data(), p->phase.x, num * sizeof(dom_dim), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(vo.data(), p->phase.v, num * sizeof(dom_dim), hipMemcpyDeviceToHost) != hipSuccess)
+            return 3;
+        std::fwrite(&num, sizeof(num), 1, o);
+        std::fwrite(xo.data(), sizeof(dom_dim), num, o);
+        std::fwrite(vo.data(), sizeof(dom_dim), num, o);
+        p->free();
+    }
     std::fclose(o);
-    p.free();
     buf.free();
     bnd.free();
     return 0;

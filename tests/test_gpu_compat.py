@@ -1,8 +1,9 @@
 """The reference-shaped C++ interface (include/pbf_compat.hpp): a C++ program
 that uses pbf_particle / pbf_buffer / pbf_boundary / pbf::one_step exactly as
 the reference's simulators do (simulator/sphere/pbf_sphere_sim.cpp:384-389)
-runs through libpbfx.so, and its results match the oracle's reference-mode
-step (as shipped: REFERENCE s_corr, no XSPH, no vorticity) bit for bit."""
+runs through libpbfx.so, and its results match the oracle bit for bit: one
+simulatee stepped as shipped (REFERENCE s_corr, no XSPH, no vorticity) and,
+in the same process, one with its own options for the full step."""
 from __future__ import annotations
 
 import os
@@ -33,10 +34,15 @@ def write_input(path, sc, steps):
 
 
 def read_output(path):
+    """The two simulatees' results: [(num, x, v) as shipped, (num, x, v) full]."""
     raw = np.fromfile(path, np.uint8)
-    num = int(raw[:4].view(np.uint32)[0])
-    f = raw[4:].view(np.float32)
-    return num, f[: 3 * num].reshape(num, 3), f[3 * num: 6 * num].reshape(num, 3)
+    out, o = [], 0
+    for _ in range(2):
+        num = int(raw[o:o + 4].view(np.uint32)[0])
+        f = raw[o + 4:o + 4 + 24 * num].view(np.float32)
+        out.append((num, f[: 3 * num].reshape(num, 3), f[3 * num: 6 * num].reshape(num, 3)))
+        o += 4 + 24 * num
+    return out
 
 
 @pytest.mark.gpu
@@ -51,8 +57,13 @@ def test_compat_one_step_matches_reference_mode_oracle(steps):
         print(f"compat_step: {sc.name} n={sc.n} steps={steps}", flush=True)
         r = subprocess.run([EXE, inp, out], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
-        num, x, v = read_output(out)
-    ref = oracle_run(sc, steps=steps)
-    assert num == ref["num"] == sc.n
-    assert np.array_equal(x.view(np.uint32), ref["x"].view(np.uint32))
-    assert np.array_equal(v.view(np.uint32), ref["v"].view(np.uint32))
+        shipped, full = read_output(out)
+    # two simulatees in one process, each with its own options: the step as
+    # shipped and the full step (PAPER s_corr, XSPH, vorticity)
+    full_sc = replace_params(sc, scorr_mode=scenes.SCORR_PAPER, scorr_k=0.1, scorr_n=4, scorr_dq_over_h=0.2,
+                             xsph_on=1, vort_on=1)
+    for (num, x, v), s in ((shipped, sc), (full, full_sc)):
+        ref = oracle_run(s, steps=steps)
+        assert num == ref["num"] == sc.n
+        assert np.array_equal(x.view(np.uint32), ref["x"].view(np.uint32))
+        assert np.array_equal(v.view(np.uint32), ref["v"].view(np.uint32))

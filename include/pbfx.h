@@ -358,6 +358,35 @@ pbfx_status pbfx_adj_identity(pbfx_ctx* ctx, uint32_t* id, void* stream);
 pbfx_status pbfx_adj_backward(pbfx_ctx* ctx, const double* gx, const double* gv, double* gx0, double* gv0,
                               double gparam[2], int32_t* exponent, void* stream);
 
+/* ---- Allocation-free step.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  One plain step of a
+ * single scene, as pbfx_create + pbfx_set_boundary + pbfx_step_async would run
+ * it (same kernels, same results bit for bit), with no context: every scratch
+ * array lives in a static device array of PBFX_STATIC_SCRATCH_BYTES that the
+ * library holds on each GPU it runs on until the process ends.  Nothing is
+ * allocated, by the library or any routine it calls.
+ *  params, domain_end  as for pbfx_create (grid = calcGridSize(2h, domain_end))
+ *  inner_spheres4 .. plane_normals3  the boundary, as for pbfx_set_boundary but
+ *               DEVICE pointers (counts may be zero)
+ *  x, v, num, lo, hi, iters, stream  as for pbfx_step_async (the count after
+ *               deletion is not reported: pass an area that deletes nothing,
+ *               or use a context)
+ * Scratch needed: about (400 + 4 * stride) bytes per particle, stride =
+ * max_pair - 1 rounded up to a multiple of 4 (528 B at 30 slots), plus 16 bytes
+ * per grid cell (the grid padded to a power of two or to 4096 cells) and
+ * ~8 KB; PBFX_CAPACITY when that exceeds PBFX_STATIC_SCRATCH_BYTES (e.g. the
+ * 32,768-particle dam on its 32 x 32 x 16 grid needs ~17.6 MB).  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU (concurrent
+ * streams or host threads) race and must be serialised by the caller.  The
+ * array keeps the last call's contents; each call clears what must start at
+ * zero with kernels (no memset), so a call can be captured in a graph. */
+#define PBFX_STATIC_SCRATCH_BYTES (32u << 20)
+pbfx_status pbfx_step_static(const pbfx_params* params, const float domain_end[3], const float* inner_spheres4,
+                             uint32_t n_inner, const float* outer_spheres4, uint32_t n_outer,
+                             const float* plane_points3, const float* plane_normals3, uint32_t n_planes, float* x,
+                             float* v, uint32_t num, const float lo[3], const float hi[3], int iters, void* stream);
+
 /* Arithmetic self-test on the device: for n inputs computes
  * out[4*i+0] = a/b, out[4*i+1] = sqrtf(a), out[4*i+2] = fmaf(a,b,a),
  * out[4*i+3] = floorf(a/b), bitwise comparable with IEEE host results.

@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 using namespace pbfx;
@@ -1371,6 +1373,173 @@ pbfx_status pbfx_adj_backward(pbfx_ctx* c, const double* gx, const double* gv, d
     gparam[1] = gp[1];
     *exponent = E;
     return PBFX_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ allocation-free step
+// pbfx_step_static keeps every scratch array of a single-scene step in this
+// static device array (one copy per GPU, held until the process ends).  Host
+// code never uses its name as an address: each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[PBFX_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+// overflow entries of the allocation-free step (their partials take 432 B each)
+constexpr uint32_t kStaticOvfMax = 4096;
+
+struct StaticDev {
+    bool ready = false;
+    unsigned char* ptr = nullptr;  // device address of `scratch` on this device
+    Tuning T{};
+};
+StaticDev g_static[kMaxDevices];
+std::mutex g_static_mu;
+
+// The scratch address and launch sizing of the current device, set up on the
+// first call on it.
+pbfx_status static_device(int dev, StaticDev** out, int* hip_err) {
+    if (dev < 0 || dev >= kMaxDevices) return PBFX_NOT_SUPPORTED;
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    StaticDev& d = g_static[dev];
+    if (!d.ready) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        int ncu = 0;
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) {
+            *hip_err = (int)e;
+            return PBFX_HIP_ERROR;
+        }
+        d.ptr = static_cast<unsigned char*>(p);
+        d.T.tile_cap = 2048;
+        d.T.unit_width = 16;
+        size_unit_grids(d.T, ncu);
+        d.ready = true;
+    }
+    *out = &d;
+    return PBFX_OK;
+}
+
+// Places every array of a single-scene step for `cap` particles and CT cell
+// keys (nT scan tiles) at 256-byte-aligned offsets from `base`; returns the
+// bytes used.  base == 0 only measures.
+size_t carve_static(Bufs& b, uintptr_t base, uint32_t cap, uint32_t CT, uint32_t nT, uint32_t stride) {
+    size_t off = 0;
+    auto take = [&](auto*& ptr, size_t count) {
+        using T = std::remove_reference_t<decltype(*ptr)>;
+        off = (off + 255) & ~(size_t)255;
+        ptr = reinterpret_cast<T*>(base + off);
+        off += (count ? count : 1) * sizeof(T);
+    };
+    const size_t bc = cap, C = CT, items = bc + bc / kChunk + 64;
+    take(b.key, bc);
+    take(b.rank, bc);
+    take(b.tmp, bc);
+    take(b.big, big_list_len(cap));
+    take(b.sidx, bc);
+    take(b.Pf, bc);
+    take(b.Q, bc);
+    take(b.X, 3 * bc);
+    take(b.R, 2 * bc);
+    take(b.OM4, bc);
+    take(b.nlist, (size_t)stride * bc);
+    take(b.ncnt, bc);
+    take(b.count, C + 1);
+    take(b.offset, C + 2);
+    take(b.tsum, (size_t)nT + 1);
+    take(b.tflag, (size_t)nT + 1);
+    take(b.tprefix, (size_t)nT + 1);
+    take(b.units, items);
+    take(b.ckey, bc);
+    take(b.crank, bc);
+    take(b.cperm, bc);
+    take(b.ccount, C + 1);
+    take(b.coffset, C + 2);
+    take(b.ctsum, (size_t)nT + 1);
+    take(b.ctflag, (size_t)nT + 1);
+    take(b.ctprefix, (size_t)nT + 1);
+    take(b.cunits, items);
+    take(b.flist, (size_t)kFinalK * bc);
+    take(b.fcnt, bc);
+    b.ovf_max = cap < kStaticOvfMax ? cap : kStaticOvfMax;
+    take(b.ovf, b.ovf_max);
+    take(b.part, 27 * (size_t)b.ovf_max);
+    take(b.ctr, kCtrWords);
+    return off;
+}
+
+}  // namespace
+
+extern "C" {
+
+pbfx_status pbfx_step_static(const pbfx_params* params, const float domain_end[3], const float* inner_spheres4,
+                             uint32_t n_inner, const float* outer_spheres4, uint32_t n_outer,
+                             const float* plane_points3, const float* plane_normals3, uint32_t n_planes, float* x,
+                             float* v, uint32_t num, const float lo[3], const float hi[3], int iters, void* stream) {
+    pbfx_status st = check_params(params);
+    if (st != PBFX_OK) return st;
+    if (!domain_end || !lo || !hi || iters < 1 || (num > 0 && (!x || !v))) return PBFX_INVALID_ARG;
+    if ((n_inner && !inner_spheres4) || (n_outer && !outer_spheres4) || (n_planes && (!plane_points3 || !plane_normals3)))
+        return PBFX_INVALID_ARG;
+    Grid G{};
+    st = make_grid(*params, domain_end, G);
+    if (st != PBFX_OK) return st;
+    // the key space padded as a context pads it (create_common)
+    uint32_t CT = G.C;
+    if (CT <= kScanTile) {
+        uint32_t p2 = 1;
+        while (p2 < CT) p2 <<= 1;
+        CT = p2;
+    } else {
+        CT = (CT + kScanTile - 1) / kScanTile * kScanTile;
+    }
+    const uint32_t lgT = ilog2(CT < kScanTile ? CT : kScanTile), nT = CT >> lgT;
+    const Phys P = make_phys(*params);
+    const uint32_t cap = num > 0 ? num : 1u;
+    Bufs b{};
+    if (carve_static(b, 0, cap, CT, nT, (uint32_t)P.stride) > sizeof(scratch)) return PBFX_CAPACITY;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return PBFX_HIP_ERROR;
+    StaticDev* d = nullptr;
+    int herr = 0;
+    st = static_device(dev, &d, &herr);
+    if (st != PBFX_OK) return st;
+    carve_static(b, reinterpret_cast<uintptr_t>(d->ptr), cap, CT, nT, (uint32_t)P.stride);
+    b.cap = cap;
+    b.CT = CT;
+    b.lgT = lgT;
+    b.nT = nT;
+    b.nscenes = 1;
+    Scene S;
+    std::memset(&S, 0, sizeof(S));
+    S.G = G;
+    S.P = P;
+    S.B.inner = reinterpret_cast<const float4*>(inner_spheres4);
+    S.B.n_inner = n_inner;
+    S.B.outer = reinterpret_cast<const float4*>(outer_spheres4);
+    S.B.n_outer = n_outer;
+    S.B.pts = plane_points3;
+    S.B.nrm = plane_normals3;
+    S.B.n_planes = n_planes;
+    for (int a = 0; a < 3; ++a) {
+        S.lo[a] = lo[a];
+        S.hi[a] = hi[a];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // the counters and flags the step accumulates into start at zero; the
+    // array keeps the last call's contents, possibly of another layout
+    hipError_t e = launch_fill_u32(b.count, (size_t)CT + 1, 0u, s);
+    if (e == hipSuccess) e = launch_fill_u32(b.tflag, (size_t)nT + 1, 0u, s);
+    if (e == hipSuccess) e = launch_fill_u32(b.ccount, (size_t)CT + 1, 0u, s);
+    if (e == hipSuccess) e = launch_fill_u32(b.ctflag, (size_t)nT + 1, 0u, s);
+    const uint32_t uw = d->T.unit_width < G.gx ? d->T.unit_width : G.gx;
+    if (e == hipSuccess)
+        e = launch_step(b, S, d->T, x, v, num, iters, params->xsph_on, params->vort_on, uw < kMaxW ? uw : kMaxW, s,
+                        nullptr);
+    return e == hipSuccess ? PBFX_OK : PBFX_HIP_ERROR;
 }
 
 pbfx_status pbfx_selftest_inv_sqrt(uint32_t lo_bits, uint32_t hi_bits, unsigned long long* mismatches,

@@ -301,6 +301,8 @@ hipError_t launch_density(const Bufs& b, const Scene& s0, const float* x, float*
 hipError_t launch_sort_cells(const uint32_t* offset, const uint32_t* cells, const uint32_t* ncells, uint32_t* vals,
                              uint32_t* scratch, uint32_t cap, hipStream_t s);
 hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float* xo, float* vo, hipStream_t s);
+// p[0..n) = value by a kernel (no memset node when the caller captures a graph).
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t s);
 // Fills T.grid[] (and T.grid_units) for num_cu compute units.
 void size_unit_grids(Tuning& T, int num_cu);
 

@@ -1456,6 +1456,13 @@ __global__ void k_scene_offsets(Bufs b, uint32_t* __restrict__ out) {
     if (s <= b.nscenes) out[s] = b.offset[b.kbase[s]];
 }
 
+// p[0..n) = value (grid-stride): clears counters and flags inside code that a
+// caller may capture in a graph, where memset nodes are avoided.
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t value) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
 __global__ void k_selftest(const float* __restrict__ a, const float* __restrict__ bb, float* __restrict__ out,
                            uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1742,6 +1749,13 @@ hipError_t launch_restore(const Bufs& b, const float* xs, const float* vs, float
 hipError_t launch_selftest_inv_sqrt(uint32_t lo, uint32_t hi, unsigned long long* mismatches, uint32_t* first,
                                     hipStream_t s) {
     hipLaunchKernelGGL(k_selftest_inv_sqrt, dim3(16384), dim3(256), 0, s, lo, hi, mismatches, first);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_fill_u32, dim3(nb < 4096 ? (uint32_t)nb : 4096u), dim3(256), 0, s, p, n, value);
     return hipGetLastError();
 }
 

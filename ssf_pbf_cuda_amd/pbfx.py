@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libpbfx.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "pbfx.h")
 
 PBFX_OK = 0
+PBFX_CAPACITY = 2
 STATUS = {
     0: "ok",
     1: "invalid argument",
@@ -148,6 +149,8 @@ def lib():
             "pbfx_status_string": ([ctypes.c_int], ctypes.c_char_p),
             "pbfx_selftest_arith": ([vp, vp, vp, u32, vp], ctypes.c_int),
             "pbfx_selftest_inv_sqrt": ([u32, u32, vp, vp, vp], ctypes.c_int),
+            "pbfx_step_static": ([ctypes.POINTER(Params), f32p, vp, u32, vp, u32, vp, vp, u32, vp, vp, u32, f32p, f32p,
+                                  ctypes.c_int, vp], ctypes.c_int),
             "pbfx_set_kernel_timing": ([vp, ctypes.c_int], ctypes.c_int),
             "pbfx_kernel_timing_read": ([vp, vp, ctypes.POINTER(KernelTime), u32, u32p], ctypes.c_int),
             "pbfx_add_particles": ([vp, vp, vp, u32p, vp, vp, u32, vp], ctypes.c_int),
@@ -194,6 +197,22 @@ def make_params(p: StepParams) -> Params:
         setattr(o, name, int(getattr(p, name)))
     o.body_force[:] = [float(c) for c in p.body_force]
     return o
+
+
+def step_static(params: StepParams, domain_end, x, v, num: int, lo, hi, iters: int, inner=None, outer=None,
+                plane_points=None, plane_normals=None, stream=None) -> None:
+    """pbfx_step_static: one step with no context and no allocation (scratch
+    in the library's static device array).  x, v: device (N, 3) float32;
+    the boundary arrays are device tensors ((k, 4) spheres, (k, 3) planes)
+    or None."""
+    def n_rows(t):
+        return 0 if t is None else int(t.shape[0])
+
+    st = lib().pbfx_step_static(ctypes.byref(make_params(params)), _f3(domain_end), _ptr(inner), n_rows(inner),
+                                _ptr(outer), n_rows(outer), _ptr(plane_points), _ptr(plane_normals),
+                                n_rows(plane_points), _ptr(x), _ptr(v), int(num), _f3(lo), _f3(hi), int(iters),
+                                _stream(stream))
+    _check(st, "pbfx_step_static")
 
 
 def calc_grid_size(cell_size: float, end: Sequence[float]):

@@ -5,10 +5,12 @@ A step is one full pbf::one_step over the whole particle set: predict,
 hashed-grid neighbour search, I Jacobi iterations, velocity update,
 vorticity confinement and XSPH.  Default workload: the 1,048,576-particle
 dam break (128x128x64 lattice, spacing 0.5h, h = 0.05, 256x128x128 cells, 4
-Jacobi iterations); the 8,388,608-particle double dam break (3 iterations),
-the 256-scene batch and the differentiable 65,536-particle workload (40
-steps forward + backward) are reported beside it.  Inputs are resident in HBM
-before the timed region.
+Jacobi iterations), each timed step from the workload's input; the
+8,388,608-particle double dam break (3 iterations), chained runs (the 32k
+dam's 500 steps, the 1M and 8M dams' first steps), the 256-scene batch over
+chained steps against a per-scene loop, and the differentiable
+65,536-particle workload (40 steps forward + backward) are reported beside
+it.  Inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line.
 """
@@ -28,33 +30,48 @@ sys.path.insert(0, ROOT)
 METRIC = "M particle-steps/s at 1M/8M (4 solver iters); HBM GB/s; batched vs plain"
 HBM_PEAK_GBS = 8000.0
 
-# Compulsory FP32 state traffic per particle for each kernel: each logical
-# array read once and written once per pass (neighbour gathers counted once).
+# Algorithmic bytes per particle of each kernel: the compulsory FP32 state
+# traffic of the step's passes (each logical array read once and written once
+# per pass, neighbour gathers counted once), assigned to the kernel that
+# carries the pass.  Their sum over a step is 252 + 44*I bytes per particle.
 ALG_BYTES = {
-    "k_predict_hash": 48,   # x 12, v 12 -> hash 4 + (x* 12 of the reference's predict) ; rank 4, histogram read 4
+    "k_predict_hash": 48,   # x 12, v 12 -> x* 12, hash 4 (predict + hash); histogram: hash 4 -> rank 4
     "k_scatter": 16,        # hash 4, rank 4 -> sorted index 4, sorted hash 4
-    "k_reorder": 56,        # index 4, x* 12, x 12 -> x*s 12, xs 12 ; cell ranges 4
-    "k_ns": 12,             # x* 12 (the lists themselves are not compulsory traffic)
+    "k_reorder": 56,        # index 4, x* 12, x 12 -> x*s 12, xs 12 ; cell ranges: sorted hash 4
     "k_lambda": 16,         # x* 12 -> lambda 4, once per Jacobi iteration
-    "k_deltap": 28,         # x* 12, lambda 4 -> x*' 12
-    "k_vort": 48,           # x* 12, xs 12 -> v 12, omega 12
+    "k_deltap": 28,         # x* 12, lambda 4 -> x*' 12, once per Jacobi iteration
+    "k_cur_sort": 36,       # velocity update: x* 12, xs 12 -> v 12 (k_cur_hash packs it into the records)
+    "k_vort": 12,           # omega 12 written (its reads are the records above)
     "k_confine": 48,        # x* 12, omega 12, v 12 -> v 12
     "k_xsph_out": 36,       # x* 12, v 12 -> v' 12
-    "k_out": 36,
+    "k_out": 36,            # (no final passes) x* 12, xs 12 -> v 12
 }
-ALG_BYTES_PER_CELL = {"k_scan_cells": 12}  # count read + clear 8, offset write 4
-# Kernels that carry one of the step's compulsory state passes.  The list
-# builds (k_ns, k_final_list) and the final-cell regrouping (k_cur_sort) move
-# no compulsory state -- lists are an implementation choice -- so the
-# roofline's dominant kernel is taken among these passes; every kernel's time
-# is still reported under "kernels".
-STATE_PASSES = ("k_predict_hash", "k_scan_cells", "k_scatter", "k_reorder", "k_lambda", "k_deltap", "k_vort",
-                "k_confine", "k_xsph_out", "k_out")
+STEP_BYTES_PER_PARTICLE = lambda iters: 252 + 44 * iters  # noqa: E731
+# The list builds (k_ns, k_final_list) carry no pass of the step's compulsory
+# traffic (lists are an implementation choice); for their per-kernel rate
+# only the x* read they need is counted.
+LIST_BYTES = {"k_ns": 12, "k_final_list": 12}
+# Cells: the scan writes every cell's offset (4 B); the counts are read and
+# cleared (8 B) only in the occupied 4096-cell tiles -- the others are skipped.
+CELL_OFFSET_BYTES = 4
+CELL_COUNT_BYTES = 8
+SCAN_TILE = 4096
 
 
-def step_bytes(n: int, cells: int, iters: int) -> float:
-    """252 + 44*I bytes per particle + 12 bytes per cell."""
-    return (252 + 44 * iters) * n + 12 * cells
+def cell_bytes(cells: int, scan_tiles: int) -> int:
+    return CELL_OFFSET_BYTES * cells + CELL_COUNT_BYTES * SCAN_TILE * scan_tiles
+
+
+def step_bytes(n: int, cells: int, iters: int, scan_tiles: int) -> float:
+    """(252 + 44*I) bytes per particle + the cell bytes actually moved."""
+    return STEP_BYTES_PER_PARTICLE(iters) * n + cell_bytes(cells, scan_tiles)
+
+
+def kernel_alg_bytes(k: str, n: int, cells: int, scan_tiles: int) -> int:
+    b = (ALG_BYTES.get(k, 0) + LIST_BYTES.get(k, 0)) * n
+    if k == "k_scan_cells":
+        b += cell_bytes(cells, scan_tiles)
+    return b
 
 
 def parse():
@@ -65,7 +82,16 @@ def parse():
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
     ap.add_argument("--steps-8m", type=int, default=200, help="timed steps of the 8M workload (0: skip)")
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps of the per-kernel event pass")
-    ap.add_argument("--batch-steps", type=int, default=10, help="timed steps of the 256-scene batch (0: skip)")
+    ap.add_argument("--chain-steps-32k", type=int, default=500,
+                    help="chained steps of the 32,768-particle dam from its input (0: skip)")
+    ap.add_argument("--chain-steps-1m", type=int, default=100,
+                    help="chained steps of the headline workload from its input (0: skip)")
+    ap.add_argument("--chain-steps-8m", type=int, default=50,
+                    help="chained steps of the 8M double dam from its input (0: skip)")
+    ap.add_argument("--cpu-chain-32k", type=int, default=0,
+                    help="chained steps of the 32k dam on the oracle (host CPU), for the CPU baseline (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="chained steps of the 256-scene batch (0: skip)")
     ap.add_argument("--adjoint-runs", type=int, default=2,
                     help="timed forward+backward runs of the differentiable workload (0: skip)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -99,11 +125,14 @@ def log(msg: str) -> None:
 def run_gpu(scene, steps, warmup, kernel_steps, tile_cap, unit_width):
     """Times `steps` plain steps.  Every step processes its own copy of the
     workload's input (the seeded lattice at rest), resident in HBM before the
-    timed region, so each timed step is the configuration's step and the
-    state never drifts into the degenerate regime the workload's parameters
-    reach after a few steps (the paper s_corr with k = 0.1 at epsilon = 600
-    makes the Jacobi sweep diverge, and 30 neighbour slots cannot register
-    compression at 0.5h spacing)."""
+    timed region, so each timed step is the configuration's step.  Chained,
+    the workload's parameters wreck the state: the paper s_corr's k = 0.1
+    term is of the order of the scaling factors themselves (s ~ C/1190 at
+    epsilon = 600), so the Jacobi sweep throws the particles out in the first
+    steps (the neighbour-slot cap is not the cause: the oracle's first step
+    gives the same mean speed with 64 slots as with 30), the dam collapses
+    into piles and the 1M state holds NaN within ~300 steps; run_chained
+    reports those runs beside this one."""
     import torch
 
     from ssf_pbf_cuda_amd import pbfx
@@ -256,13 +285,14 @@ def run_chained(scene, steps, tile_cap, unit_width, block=100):
 
 
 def run_batch(scs, steps, warmup):
-    """The batched form against a per-scene loop, on the same GPU and input.
+    """The batched form against a per-scene loop, on the same GPU, both over
+    `steps` CHAINED steps from the workload's input.
 
-    batched: one pbfx_batch_step_async for all scenes, captured in a hipGraph
-    together with the restore of the input (each timed step starts from the
-    workload's input, as in run_gpu).  loop: one plain context per scene and
-    one pbfx_step_async per scene, the same restore first.  Returns ms per
-    step of each and whether one step of each gives identical bytes."""
+    batched: one step recorded by pbfx_batch_graph_capture (all scenes, the
+    output offsets written over the input offsets) and replayed `steps` times
+    by pbfx_batch_graph_launch.  loop: one plain context per scene, one
+    pbfx_step per scene per step.  Returns ms per step of each and whether
+    the two end states are identical bit for bit, scene by scene."""
     import torch
 
     from ssf_pbf_cuda_amd import pbfx
@@ -272,64 +302,79 @@ def run_batch(scs, steps, warmup):
     iters = scs[0].iters
     x0 = torch.from_numpy(np.concatenate([s.x for s in scs])).cuda()
     v0 = torch.from_numpy(np.concatenate([s.v for s in scs])).cuda()
-    off = torch.tensor(np.concatenate([[0], np.cumsum(counts)]).astype(np.int32), device="cuda")
-    x, v = x0.clone(), v0.clone()
+    off0 = torch.tensor(np.concatenate([[0], np.cumsum(counts)]).astype(np.int32), device="cuda")
+    x, v, off = x0.clone(), v0.clone(), off0.clone()
     stream = torch.cuda.Stream()
     batch = pbfx.BatchSolver(scs)
-    tune = batch.autotune(x0, v0, n, (0, 0, 0), (0, 0, 0), iters, x, v, in_offsets=off, stream=stream)
+    tune = batch.autotune(x0, v0, n, (0, 0, 0), (0, 0, 0), iters, x, v, in_offsets=off0, stream=stream)
     log(f"batch: autotuned tile_cap={tune[0]} unit_width={tune[1]}")
-    with torch.cuda.stream(stream):
-        for _ in range(max(warmup, 1)):
+    torch.cuda.synchronize()
+    batch.graph_capture(x, v, off, iters, 1, stream)
+
+    def restore():
+        with torch.cuda.stream(stream):
             x.copy_(x0)
             v.copy_(v0)
-            batch.batch_step_async(x, v, off, n, iters, stream=stream)
+            off.copy_(off0)
+
+    restore()
+    batch.graph_launch(max(warmup, 1), stream)
+    restore()
     stream.synchronize()
-    xb, vb = x.clone(), v.clone()  # one batched step from the input
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g, stream=stream):
-        x.copy_(x0)
-        v.copy_(v0)
-        batch.batch_step_async(x, v, off, n, iters, stream=stream)
-    torch.cuda.synchronize()
-    log(f"batch of {len(scs)} scenes ({n} particles): graph captured, timing {steps} steps")
+    log(f"batch of {len(scs)} scenes ({n} particles): graph recorded, timing {steps} chained steps")
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
-        g.replay()
-    e1.record()
-    torch.cuda.synchronize()
+    e0.record(stream)
+    batch.graph_launch(steps, stream)
+    e1.record(stream)
+    stream.synchronize()
     ms_batch = e0.elapsed_time(e1) / steps
+    ob = off.cpu().numpy().astype(np.int64)
+    xb, vb = x.cpu().numpy(), v.cpu().numpy()
     batch.close()
-    del g
-    log(f"batch: {ms_batch:.3f} ms/step; per-scene loop")
+    log(f"batch: {ms_batch:.3f} ms/step ({n - int(ob[-1])} particles deleted); per-scene loop")
+    # The loop: one context per scene, stepped with pbfx_step (one_step's
+    # semantics: the live count comes back each step, so a scene whose
+    # particle left its area -- a velocity overflowing to infinity -- goes on
+    # with the right count).  Each scene keeps its own slice.
     solvers = [pbfx.Solver.for_scene(sc) for sc in scs]
+    for sv in solvers:
+        sv.set_tuning(*tune)
     offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     cur = torch.cuda.current_stream()
+    live = list(counts)
 
-    def loop_step():
-        x.copy_(x0)
-        v.copy_(v0)
-        for sv, sc, o, c in zip(solvers, scs, offs[:-1], counts):
-            sv.step_async(x[o:o + c], v[o:o + c], c, sc.area_lo, sc.area_hi, iters, cur)
+    def loop_steps(k):
+        for _ in range(k):
+            for i, (sv, sc, o) in enumerate(zip(solvers, scs, offs[:-1])):
+                live[i] = sv.step(x[o:o + counts[i]], v[o:o + counts[i]], live[i], sc.area_lo, sc.area_hi, iters,
+                                  stream=cur)
 
-    for _ in range(max(warmup, 1)):
-        loop_step()
+    x.copy_(x0)
+    v.copy_(v0)
+    loop_steps(max(warmup, 1))
+    x.copy_(x0)
+    v.copy_(v0)
+    live[:] = counts
     torch.cuda.synchronize()
-    same = bool(torch.equal(x.view(torch.int32), xb.view(torch.int32)) and
-                torch.equal(v.view(torch.int32), vb.view(torch.int32)))
     e0.record()
-    for _ in range(steps):
-        loop_step()
+    loop_steps(steps)
     e1.record()
     torch.cuda.synchronize()
     ms_loop = e0.elapsed_time(e1) / steps
+    xl, vl = x.cpu().numpy(), v.cpu().numpy()
+    same = True
+    for i in range(len(scs)):
+        nb_ = int(ob[i + 1] - ob[i])
+        same = same and nb_ == live[i] and \
+            np.array_equal(xb[ob[i]:ob[i] + nb_].view(np.uint32), xl[offs[i]:offs[i] + nb_].view(np.uint32)) and \
+            np.array_equal(vb[ob[i]:ob[i] + nb_].view(np.uint32), vl[offs[i]:offs[i] + nb_].view(np.uint32))
     for sv in solvers:
         sv.close()
-    log(f"loop: {ms_loop:.3f} ms/step")
-    del x, v, x0, v0, xb, vb
+    log(f"loop: {ms_loop:.3f} ms/step; end states identical scene by scene: {same}")
+    del x, v, x0, v0
     torch.cuda.empty_cache()
     return {"ms_batch": ms_batch, "ms_loop": ms_loop, "bit_exact": same, "particles": n,
-            "tuning": {"tile_cap": tune[0], "unit_width": tune[1]}}
+            "deleted": n - int(ob[-1]), "tuning": {"tile_cap": tune[0], "unit_width": tune[1]}}
 
 
 def run_adjoint(runs, steps=40, every=10):
@@ -430,14 +475,51 @@ def cpu_adjoint(steps):
             "sample": f"{steps} steps of {sc.name} forward + FP64 adjoint, {el:.1f} s, OpenMP threads={threads}"}
 
 
-def kernel_report(kt, n, cells):
+def kernel_report(kt, n, cells, scan_tiles, steps=1):
+    """Per kernel: mean launch duration, launches per step and the algorithmic
+    rate (its bytes per launch over its mean duration)."""
     rows = {}
-    for k, (tot, cnt) in kt.items():
+    for k, (tot, cnt) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
         avg_ms = tot / cnt
-        b = ALG_BYTES.get(k, 0) * n + ALG_BYTES_PER_CELL.get(k, 0) * cells
-        rows[k] = {"avg_ms": round(avg_ms, 5), "launches": cnt,
+        b = kernel_alg_bytes(k, n, cells, scan_tiles)
+        rows[k] = {"avg_ms": round(avg_ms, 5), "ms_per_step": round(tot / steps, 5),
+                   "launches_per_step": round(cnt / steps, 2),
                    "alg_GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}
     return rows
+
+
+def roofline(kt, n, cells, scan_tiles, workload):
+    """The dominant kernel -- the largest total time over all kernels of the
+    per-kernel pass -- priced against the HBM peak with its algorithmic bytes
+    per launch, beside its measured HBM bytes per launch (rocprofv3 counters,
+    profiles/hbm_traffic_pbf_step_<workload>.txt)."""
+    dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
+    tot, cnt = kt[dom]
+    avg_ms = tot / cnt
+    b = kernel_alg_bytes(dom, n, cells, scan_tiles)
+    achieved = b / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom, workload),
+            "alg_bytes_per_launch": b, "avg_launch_ms": round(avg_ms, 5),
+            "share_of_step": round(tot / sum(t for t, _ in kt.values()), 4)}
+
+
+def cpu_chained(scene, steps):
+    """The oracle on `steps` CHAINED steps of `scene` from its input, on this
+    host's cores (the reference's CPU-runnable case run as a simulation)."""
+    from oracle import pyoracle as orc
+
+    threads = orc.get_threads()
+    x, v = scene.x, scene.v
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = orc.step(scene, x=x, v=v)
+        x, v = r["x"], r["v"]
+    el = time.perf_counter() - t0
+    return {"value": round(scene.n * steps / el / 1e6, 4), "unit": "M particle-steps/s", "cores": threads,
+            "kind": "port", "ms_per_step": round(1e3 * el / steps, 3),
+            "sample": f"{steps} chained steps of {scene.name} ({scene.n} particles), {el:.1f} s, "
+                      f"OpenMP threads={threads}, cpu={os.cpu_count()} logical"}
 
 
 def cpu_baseline(scene, budget_s):
@@ -459,6 +541,22 @@ def cpu_baseline(scene, budget_s):
             "sample": f"{done} full steps of {scene.name} ({scene.n} particles, {scene.iters} iterations), "
                       f"each from the workload's input, {el:.1f} s, OpenMP threads={threads}, "
                       f"cpu={os.cpu_count()} logical"}
+
+
+def chained_report(sc, rc):
+    n, I = sc.n, sc.iters
+    ms = rc["ms_total"] / rc["steps"]
+    last = rc["blocks"][-1]
+    return {
+        "workload": f"{sc.name}: {n} particles, {I} iterations, chained from the workload's input",
+        "steps": rc["steps"],
+        "value": round(n * rc["steps"] / (rc["ms_total"] * 1e-3) / 1e6, 3),
+        "unit": "M particle-steps/s",
+        "ms_per_step": round(ms, 4),
+        "finite_at_end": rc["finite"],
+        "blocks": rc["blocks"],
+        "kernels": kernel_report(rc["kernels"], n, sc.cells, last.get("scan_tiles", 0), rc["steps"]),
+    }
 
 
 def main():
@@ -485,14 +583,8 @@ def main():
     n, C, I = sc.n, sc.cells, sc.iters
     ms_step = r["ms_total"] / a.steps
     value = n * a.steps / (r["ms_total"] * 1e-3) / 1e6
-    kern = kernel_report(r["kernels"], n, C)
-    passes = {k: t for k, t in r["kernels"].items() if k in STATE_PASSES}
-    dom = max(passes.items(), key=lambda kv: kv[1][0])[0] if passes else None
-    dom_avg_ms = r["kernels"][dom][0] / r["kernels"][dom][1]
-    dom_bytes = ALG_BYTES.get(dom, 0) * n + ALG_BYTES_PER_CELL.get(dom, 0) * C
-    achieved = dom_bytes / (dom_avg_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles(dom, sc.name)
-    sb = step_bytes(n, C, I)
+    tiles = r["stats"]["scan_tiles"]
+    sb = step_bytes(n, C, I, tiles)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -505,7 +597,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded jittered lattice at rest; each timed step runs on its own resident copy)",
+        "data": "synthetic (seeded jittered lattice at rest; each timed step runs the workload's step on its own "
+                "resident copy of the input; chained runs reported under 'chained')",
         "config": {
             "workload": f"{sc.name}: {n} particles, grid {sc.grid[0]}x{sc.grid[1]}x{sc.grid[2]} cells, "
                         f"{I} Jacobi iterations, paper s_corr k=0.1 n=4 dq=0.2h, XSPH c=0.01, "
@@ -515,54 +608,65 @@ def main():
             "iterations": I,
             "tuning": r["tuning"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "alg_bytes_per_launch": dom_bytes,
-            "avg_launch_ms": round(dom_avg_ms, 5),
-            "share_of_step": round(r["kernels"][dom][0] / sum(t[0] for t in r["kernels"].values()), 4),
-        },
+        "roofline": roofline(r["kernels"], n, C, tiles, sc.name),
         "step_roofline": {
             "alg_bytes_per_step": sb,
+            "alg_bytes_per_particle": STEP_BYTES_PER_PARTICLE(I),
+            "cell_bytes": cell_bytes(C, tiles),
+            "occupied_scan_tiles": tiles,
             "achieved_GBps": round(sb / (ms_step * 1e-3) / 1e9, 1),
             "frac": round(sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
-        "kernels": kern,
+        "kernels": kernel_report(r["kernels"], n, C, tiles, a.kernel_steps),
         "state": {k: r["stats"][k] for k in ("units", "slow_units", "slow_particles", "max_tile", "truncated",
-                                             "sweep_cap_hits", "tile_overflow")},
+                                             "sweep_cap_hits", "list_overflow", "scan_tiles")},
     }
+    tuned = (r["tuning"]["tile_cap"], r["tuning"]["unit_width"])
     if a.steps_8m and a.config != 2:
         s8 = scenes.config2()
         r8 = run_gpu(s8, a.steps_8m, min(a.warmup, 5), min(a.kernel_steps, 5), a.tile_cap, a.unit_width)
         ms8 = r8["ms_total"] / a.steps_8m
-        sb8 = step_bytes(s8.n, s8.cells, s8.iters)
+        t8 = r8["stats"]["scan_tiles"]
+        sb8 = step_bytes(s8.n, s8.cells, s8.iters, t8)
         out["double_dam_8M"] = {
             "value": round(s8.n * a.steps_8m / (r8["ms_total"] * 1e-3) / 1e6, 3),
             "unit": "M particle-steps/s",
             "steps": a.steps_8m,
             "ms_per_step": round(ms8, 5),
             "iterations": s8.iters,
+            "tuning": r8["tuning"],
+            "roofline": roofline(r8["kernels"], s8.n, s8.cells, t8, s8.name),
             "step_roofline_frac": round(sb8 / (ms8 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "kernels": kernel_report(r8["kernels"], s8.n, s8.cells),
-            "state": {k: r8["stats"][k] for k in ("units", "slow_units", "max_tile", "truncated")},
+            "kernels": kernel_report(r8["kernels"], s8.n, s8.cells, t8, min(a.kernel_steps, 5)),
+            "state": {k: r8["stats"][k] for k in ("units", "slow_units", "max_tile", "truncated", "list_overflow",
+                                                  "scan_tiles")},
         }
+    chained = {}
+    if a.chain_steps_32k:
+        s0 = scenes.config0()
+        chained[s0.name] = chained_report(s0, run_chained(s0, a.chain_steps_32k, 4096, 32, block=100))
+    if a.chain_steps_1m:
+        chained[sc.name] = chained_report(sc, run_chained(sc, a.chain_steps_1m, *tuned, block=25))
+    if a.chain_steps_8m:
+        s8 = scenes.config2()
+        chained[s8.name] = chained_report(s8, run_chained(s8, a.chain_steps_8m, 4096, 32, block=25))
+    if chained:
+        out["chained"] = chained
     if a.batch_steps:
         scs = scenes.batch_config3()
         rb = run_batch(scs, a.batch_steps, min(a.warmup, 2))
         out["batched"] = {
             "workload": f"{len(scs)} dam-break scenes, {rb['particles']} particles (4,096-200,000 each, "
-                        f"log-uniform), per-scene rest density / XSPH / vorticity, 4 iterations",
+                        f"log-uniform), per-scene rest density / XSPH / vorticity, 4 iterations, "
+                        f"{a.batch_steps} chained steps",
             "value": round(rb["particles"] / (rb["ms_batch"] * 1e-3) / 1e6, 3),
             "unit": "M particle-steps/s",
             "ms_per_step": round(rb["ms_batch"], 4),
             "loop_ms_per_step": round(rb["ms_loop"], 4),
             "speedup_vs_loop": round(rb["ms_loop"] / rb["ms_batch"], 3),
             "bit_exact_vs_loop": rb["bit_exact"],
+            "deleted": rb["deleted"],
+            "loop": "one context per scene, pbfx_step per scene per step (one_step's semantics)",
             "tuning": rb["tuning"],
             "steps": a.batch_steps,
         }
@@ -573,6 +677,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sc, a.cpu_seconds)
         if a.adjoint_runs:
             out["differentiable"]["cpu_baseline"] = cpu_adjoint(10)
+    if a.cpu_chain_32k:
+        log(f"cpu chained 32k dam ({a.cpu_chain_32k} steps)")
+        out["cpu_chained_dam32k"] = cpu_chained(scenes.config0(), a.cpu_chain_32k)
     if a.calibrate_copy_mb:
         src = torch.ones(a.calibrate_copy_mb * (1 << 18), dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

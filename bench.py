@@ -118,6 +118,30 @@ def traffic_from_profiles(kernel: str, workload: str):
     return None
 
 
+def valu_from_profiles(kernel: str, workload: str):
+    """VALU instructions per launch of `kernel` (rocprofv3 SQ_INSTS_VALU,
+    wave instructions summed over the dispatch) and the engine clock of the
+    same launches (GRBM_GUI_ACTIVE per XCD over the kernel time), from
+    profiles/valu_pbf_step_<workload>.txt (lines '<kernel> valu_insts_per_launch: X'
+    and '<kernel> clock_mhz: Y')."""
+    path = os.path.join(ROOT, "profiles", f"valu_pbf_step_{workload}.txt")
+    if not os.path.exists(path):
+        return None, None
+    insts, mhz = None, None
+    for line in open(path):
+        if line.startswith(f"{kernel} valu_insts_per_launch:"):
+            insts = float(line.split(":", 1)[1])
+        elif line.startswith(f"{kernel} clock_mhz:"):
+            mhz = float(line.split(":", 1)[1])
+    return insts, mhz
+
+
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD
+# every 4 cycles (16 lanes per cycle).
+VALU_SIMDS = 256 * 4
+VALU_CYCLES_PER_INST = 4
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -498,10 +522,18 @@ def roofline(kt, n, cells, scan_tiles, workload):
     avg_ms = tot / cnt
     b = kernel_alg_bytes(dom, n, cells, scan_tiles)
     achieved = b / (avg_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom, workload),
-            "alg_bytes_per_launch": b, "avg_launch_ms": round(avg_ms, 5),
-            "share_of_step": round(tot / sum(t for t, _ in kt.values()), 4)}
+    out = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom, workload),
+           "alg_bytes_per_launch": b, "avg_launch_ms": round(avg_ms, 5),
+           "share_of_step": round(tot / sum(t for t, _ in kt.values()), 4)}
+    insts, mhz = valu_from_profiles(dom, workload)
+    if insts and mhz:
+        # what the kernel actually spends its time on: VALU issue against the
+        # issue rate of all SIMDs at the measured clock
+        peak = VALU_SIMDS * mhz * 1e6 / VALU_CYCLES_PER_INST
+        out["valu"] = {"insts_per_launch": insts, "clock_mhz": mhz,
+                       "issue_frac": round(insts / (avg_ms * 1e-3) / peak, 4)}
+    return out
 
 
 def cpu_chained(scene, steps):

@@ -469,6 +469,7 @@ __device__ __forceinline__ void mark_tile(uint32_t* __restrict__ tflag, uint32_t
 }
 
 __device__ __forceinline__ F3 f3of(float4 f) { return mk3(f.x, f.y, f.z); }
+__device__ __forceinline__ F3 f3of(F3 f) { return f; }
 
 // Whether the step sorts the segment of a cell of n particles.
 __device__ __forceinline__ bool sorted_cell(uint32_t n) { return n > kSmallCell; }

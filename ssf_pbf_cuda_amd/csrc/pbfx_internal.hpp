@@ -36,7 +36,7 @@ constexpr uint32_t kMaxCells = 9 * (kMaxW + 2);
 constexpr uint32_t kTileCapMax = 4096;
 constexpr uint32_t kScanTile = 4096;      // cells per scan tile
 constexpr uint32_t kChunk = kBlock;       // own particles per work item
-constexpr uint32_t kFinalTileMax = 1536;  // largest LDS tile of the final-list units (particles)
+constexpr uint32_t kFinalTileMax = 2560;  // LDS tile of the final-list units (particles)
 constexpr uint32_t kSparse = 64;          // final-cell runs below this go one thread per particle
 // Cells above kSmallCell particles get their index segment sorted, so that a
 // particle's in-cell rank is its position rather than a count over the whole

@@ -596,9 +596,9 @@ __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, 
 // sorted positions, with sb the cell's slot base.  Returns the number kept
 // (at most ncap); trunc is set when an (ncap+1)-th pair exists.  Each group
 // of four candidates gives a 4-bit hit mask whose pairs are stored in order.
-template <bool kStaged>
+template <bool kStaged, class E>
 __device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, uint32_t kc, F3 self, float h2,
-                                              uint32_t ncap, const float4* __restrict__ src,
+                                              uint32_t ncap, const E* __restrict__ src,
                                               uint32_t* __restrict__ nl, uint32_t cap, uint32_t g, bool& trunc) {
     uint32_t* __restrict__ dst = nl + list_at(cap, g, 0u);  // where entry cnt goes
     const size_t jump = (size_t)4u * cap - 3u;              // from entry 4q+3 to 4q+4
@@ -649,7 +649,7 @@ template <bool kB>
 __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t tile_cap) {
     __shared__ UnitShared S;
     extern __shared__ float4 smem[];
-    float4* tile = smem;
+    F3* tile = reinterpret_cast<F3*>(smem);  // x*, 12 bytes a candidate
     const float4* __restrict__ Pf = b.Pf;
     uint32_t* __restrict__ nl = b.nlist;
     const uint32_t cap = b.cap;
@@ -665,14 +665,14 @@ __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t t
         const uint32_t ncap = (uint32_t)SC[sc].P.cap;
         const UnitInfo I = unit_setup(G, b.offset, unit, tile_cap, S, tl);
         // a neighbourhood larger than the tile (any size) is filtered from global memory
-        if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tile[t] = Pf[g]; });
+        if (I.staged) load_runs(S, I, [&](uint32_t t, uint32_t g) { tile[t] = f3of(Pf[g]); });
         __syncthreads();
         for (uint32_t q = threadIdx.x; q < I.n_own; q += kBlock) {
             const uint32_t ts = I.own_t0 + q, g = I.own_g0 + q;
             const uint32_t kc = own_column(S, I.W, ts);
             bool trunc = false;
             const uint32_t cnt =
-                I.staged ? ns_filter<true>(S, I.W, kc, f3of(tile[ts]), h2, ncap, tile, nl, cap, g, trunc)
+                I.staged ? ns_filter<true>(S, I.W, kc, tile[ts], h2, ncap, tile, nl, cap, g, trunc)
                          : ns_filter<false>(S, I.W, kc, f3of(Pf[g]), h2, ncap, Pf, nl, cap, g, trunc);
             b.ncnt[g] = cnt;
             tl.trunc += trunc ? 1u : 0u;
@@ -864,21 +864,22 @@ __device__ __forceinline__ bool clip_hits(uint32_t& m, uint32_t room) {
 // column kc, in order: emit(slot, first pair of its cell), which counts them
 // in n; stops once n reaches lim.
 template <class EmitF>
-__device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S, const float4* __restrict__ tp,
+__device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S, const F3* __restrict__ tp,
                                           uint32_t kc, F3 self, float cut2, const uint32_t& n, uint32_t lim,
                                           EmitF&& emit) {
     for (uint32_t q = 0; q < 27u; ++q) {
         const uint32_t e = (q / 3u) * (I.W + 2u) + kc - 1u + (q % 3u);
         const uint32_t lo = S.tstart[e], hi = S.tstart[e + 1];
+        const uint32_t gb = S.gstart[e] - lo;  // slot = gb + tile index
         bool first = true;
         for (uint32_t t = lo; t < hi; t += 4u) {
-            float4 el[4];
+            F3 el[4];
 #pragma unroll
             for (uint32_t i = 0; i < 4u; ++i) el[i] = tp[t + i];  // the tile is padded: no clamp
             uint32_t m = 0;
 #pragma unroll
             for (uint32_t i = 0; i < 4u; ++i) {
-                const F3 d = f3of(el[i]) - self;
+                const F3 d = el[i] - self;
                 m |= !(glm_dot(d, d) > cut2) ? (1u << i) : 0u;
             }
             m &= (hi - t >= 4u) ? 0xFu : ((1u << (hi - t)) - 1u);
@@ -886,8 +887,7 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
             while (m) {
                 const uint32_t i = (uint32_t)__builtin_ctz(m);
                 m &= m - 1u;
-                const float w = i == 0u ? el[0].w : (i == 1u ? el[1].w : (i == 2u ? el[2].w : el[3].w));
-                emit(__float_as_uint(w), first);
+                emit(gb + t + i, first);
                 first = false;
             }
             if (full || n == lim) return;
@@ -1032,8 +1032,6 @@ __device__ __forceinline__ uint32_t final_column(const Grid& G, const UnitInfo& 
     return (((uint32_t)gp[0] & (G.gx - 1u)) - I.xb) + 1u;
 }
 
-// Tile element of the list build: x*_j and slot j (as float bits).
-
 // ------------------------------------------------- K9: final pair lists
 // For each particle (by position q of the final-cell order) the slots of its
 // in-range pairs in groups of four (list_at(cap, q, k), k < kFinalK); fcnt[q] =
@@ -1047,7 +1045,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
     __shared__ uint32_t colpos[kMaxW + 2];  // own particles per final column, then each column's first position
     __shared__ uint32_t corder[kChunk];     // the item's own slots in final-column order
     extern __shared__ float4 smem[];
-    float4* tp = smem;
+    F3* tp = reinterpret_cast<F3*>(smem);  // x*, 12 bytes a candidate
     // the final x*, packed: the same values as the passes' records R (built
     // from Pf), at half the bytes per candidate
     const float4* __restrict__ Pf = b.Pf;
@@ -1097,10 +1095,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
             rk = atomicAdd(&colpos[kc], 1u);
         }
         if (I.staged)
-            load_runs(S, I, [&](uint32_t t, uint32_t j) {
-                const float4 p = Pf[j];
-                tp[t] = make_float4(p.x, p.y, p.z, __uint_as_float(j));
-            });
+            load_runs(S, I, [&](uint32_t t, uint32_t j) { tp[t] = f3of(Pf[j]); });
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t run = 0;
@@ -1498,18 +1493,20 @@ __global__ __launch_bounds__(256) void k_selftest_inv_sqrt(uint32_t lo, uint32_t
 
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-// Dynamic LDS of each unit kernel: the tile.
-// Dynamic LDS of each unit kernel: the tile, plus kTilePad entries so that
-// the filters read candidates four at a time past a cell's last one without
-// clamping (those reads are masked out).
+// Dynamic LDS of each unit kernel: the tile of 12-byte positions, plus
+// kTilePad entries so that the filters read candidates four at a time past a
+// cell's last one without clamping (those reads are masked out).
 constexpr uint32_t kTilePad = 4;
-size_t smem_of(uint32_t tile_cap) { return (size_t)(tile_cap + kTilePad) * sizeof(float4); }
+static_assert(sizeof(F3) == 12, "the tile holds packed float3 positions");
+size_t smem_of(uint32_t tile_cap) { return (size_t)(tile_cap + kTilePad) * sizeof(F3); }
 
-// The final-list units' tile: at most kFinalTileMax entries whatever the
-// search's tile.  Their own particles are few per unit and taken in
-// final-column order, so the global-memory filter is about as fast as the
-// staged one, and the smaller tile lets more workgroups share a CU.
-uint32_t final_tile(const Tuning& T) { return T.tile_cap < kFinalTileMax ? T.tile_cap : kFinalTileMax; }
+// The final-list units' tile: kFinalTileMax entries whatever the search's
+// tile.  A 32-cell-wide final unit in the dam's lattice (8 particles a cell)
+// has a 9 x 34-cell neighbourhood of 2,448 particles; 2,560 entries stage all
+// of them at four workgroups per CU (1,536 left them to the global-memory
+// filter: 237 against 216 us on the 1M step, 1.95 against 1.63 ms at 8M;
+// 3,072 costs a workgroup per CU and is slower again).
+uint32_t final_tile(const Tuning&) { return kFinalTileMax; }
 
 }  // namespace
 

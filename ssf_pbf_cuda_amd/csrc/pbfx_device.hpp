@@ -211,6 +211,53 @@ __device__ __forceinline__ void for_list8(const uint32_t* __restrict__ nl, uint3
     }
 }
 
+// Gathers of 16-byte records through a buffer resource: one 32-bit byte
+// offset per lane (no 64-bit address arithmetic), and an offset past the
+// n records reads zeros instead of faulting.  Valid for n < kGatherMaxSlots.
+constexpr uint32_t kGatherMaxSlots = 1u << 28;
+struct Gather16 {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Gather16(const float4* p, uint32_t n)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * 16u), 0x00020000)) {}
+    __device__ __forceinline__ float4 operator()(uint32_t j) const {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(j << 4), 0, 0);
+        return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+    }
+};
+
+// for_list8 for a load that takes any index (Gather16): the entries past n
+// in the last word are loaded as they are (whatever the word holds) and not
+// passed to body, which saves the index selects.
+template <class LoadF, class BodyF>
+__device__ __forceinline__ void for_list8_any(const uint32_t* __restrict__ nl, uint32_t cap, uint32_t i, uint32_t n,
+                                              LoadF&& load, BodyF&& body) {
+    if (n == 0u) return;
+    const uint32_t last = (n - 1u) & ~3u;  // the last word holding a written entry
+    uint4 e0 = list_word(nl, cap, i, 0u), e1 = list_word(nl, cap, i, min(4u, last));
+    for (uint32_t k = 0; k < n; k += 8u) {
+        const uint32_t j[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+        float4 el[8];
+#pragma unroll
+        for (uint32_t m = 0; m < 8u; ++m) el[m] = load(j[m]);
+        e0 = list_word(nl, cap, i, min(k + 8u, last));
+        e1 = list_word(nl, cap, i, min(k + 12u, last));
+#pragma unroll
+        for (uint32_t m = 0; m < 8u; ++m)
+            if (k + m < n) body(k + m, el[m]);
+    }
+}
+
+// The Jacobi passes' walk over slot i's list: buffer gathers when the
+// records' byte offsets fit 32 bits, plain loads otherwise.
+template <class BodyF>
+__device__ __forceinline__ void walk_list(const uint32_t* __restrict__ nl, uint32_t cap, uint32_t i, uint32_t n,
+                                          const float4* __restrict__ rec, BodyF&& body) {
+    if (cap < kGatherMaxSlots)
+        for_list8_any(nl, cap, i, n, Gather16(rec, cap), body);
+    else
+        for_list8(nl, cap, i, n, [&](uint32_t j) { return rec[j]; }, body);
+}
+
 // 1.f / sqrtf(r2) with both operations correctly rounded -- the IEEE
 // result -- in about a third of the instructions of the IEEE sequences.  On
 // [2^-96, FLT_MAX]: v_sqrt_f32, then the two-sided fma test of its

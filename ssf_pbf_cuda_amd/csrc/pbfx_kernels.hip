@@ -699,7 +699,7 @@ __global__ __launch_bounds__(256) void k_lambda(Bufs b, Scenes<kB> SC) {
         const uint32_t n = b.ncnt[i];
         float ks = 0.f, kg2s = 0.f;
         F3 kgs = mk3(0.f, 0.f, 0.f);
-        for_list8(nl, cap, i, n, [&](uint32_t j) { return Pf[j]; }, [&](uint32_t, float4 pj) {
+        walk_list(nl, cap, i, n, Pf, [&](uint32_t, float4 pj) {
             float W;
             F3 kg;
             pair_terms(self, f3of(pj), P, W, kg);
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(256) void k_deltap(Bufs b, Scenes<kB> SC) {
         const float self_cl = (si < 0.f) ? -si : 0.f;
         const uint32_t n = b.ncnt[i];
         F3 sum = mk3(0.f, 0.f, 0.f);
-        for_list8(nl, cap, i, n, [&](uint32_t j) { return Q[j]; }, [&](uint32_t, float4 pe) {
+        walk_list(nl, cap, i, n, Q, [&](uint32_t, float4 pe) {
             float W;
             F3 kg;
             pair_terms(self, f3of(pe), P, W, kg);

@@ -1151,6 +1151,33 @@ __device__ __forceinline__ F3 sum_list(const uint32_t* __restrict__ fl, uint32_t
     return sum;
 }
 
+// sum_list for an elem that takes any index: the entries past n in the last
+// word are loaded as they are and not summed.
+template <class ElemF, class TermF>
+__device__ __forceinline__ F3 sum_list_any(const uint32_t* __restrict__ fl, uint32_t cap, uint32_t q, uint32_t n,
+                                           ElemF&& elem, TermF&& term) {
+    F3 sum = mk3(0.f, 0.f, 0.f), cs = mk3(0.f, 0.f, 0.f);
+    uint4 w = n ? list_word(fl, cap, q, 0u) : make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t k = 0; k < n; k += 4u) {
+        const uint32_t e[4] = {w.x, w.y, w.z, w.w};
+        El2 el[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(e[i] & 0x7FFFFFFFu);
+        w = list_word(fl, cap, q, min(k + 4u, (n - 1u) & ~3u));  // next group, in flight during this one
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i)
+            if (k + i < n) {
+                if (e[i] >> 31) {
+                    acc(sum, cs);
+                    cs = mk3(0.f, 0.f, 0.f);
+                }
+                acc(cs, term(el[i]));
+            }
+    }
+    acc(sum, cs);
+    return sum;
+}
+
 // Visits position q of the final-cell order: body(q, slot, stored pair
 // count, scene), grouped by scene (no wave-wide operations inside).
 template <bool kB, class F>
@@ -1210,6 +1237,14 @@ struct PassTerms {
         }
         return El2{R[2 * j], R[2 * j + 1]};
     }
+    // the same through buffer resources (records of slot j at byte 32j of R, 16j of OM4)
+    __device__ __forceinline__ El2 elem_buf(const Gather16& gr, const Gather16& go, uint32_t j) const {
+        if (kPass == PASS_CONF) {
+            const float4 r = gr(2u * j);
+            return El2{make_float4(r.x, r.y, r.z, go(j).w), make_float4(0.f, 0.f, 0.f, 0.f)};
+        }
+        return El2{gr(2u * j), gr(2u * j + 1u)};
+    }
     __device__ __forceinline__ F3 term(const El2& e) const {
         if (kPass == PASS_VORT) return vort_pair(self, selfv, f3of(e.p), f3of(e.q), arg);
         if (kPass == PASS_CONF) return loc_pair(self, f3of(e.p), e.p.w, arg);
@@ -1234,7 +1269,13 @@ template <class Pass>
 __device__ __forceinline__ F3 pass_sum(const Bufs& b, const Grid& G, const Pass& ps, uint32_t q, uint32_t n) {
     auto elem = [&](uint32_t j) { return ps.elem(j); };
     auto term = [&](const El2& e) { return ps.term(e); };
-    if (n <= kFinalK) return sum_list(b.flist, b.cap, q, n, elem, term);
+    if (n <= kFinalK) {
+        if (b.cap < kGatherMaxSlots / 2u) {
+            const Gather16 gr(b.R, 2u * b.cap), go(b.OM4, b.cap);
+            return sum_list_any(b.flist, b.cap, q, n, [&](uint32_t j) { return ps.elem_buf(gr, go, j); }, term);
+        }
+        return sum_list(b.flist, b.cap, q, n, elem, term);
+    }
     if (n == kFinalK + 1u) return sum27_global(G, b.offset, ps.self, ps.cut2, elem, term);
     const size_t k = n - (kFinalK + 2u);
     F3 sum = mk3(0.f, 0.f, 0.f);

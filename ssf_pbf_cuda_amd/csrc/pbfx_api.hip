@@ -243,7 +243,7 @@ void free_bufs(pbfx_ctx* c) {
                     b.OM4,   b.ncnt,    b.nlist,   b.count, b.offset, b.tsum,  b.tprefix, b.units,
                     b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
                     b.tflag, b.ctflag,  b.flist,   b.fcnt,  b.isid,   b.sid,   b.ctr,     b.big,
-                    b.ovf,   b.part,
+                    b.ovf,   b.part,    b.bslot,   b.aux,   b.bseg,
                     c->d_scenes, c->d_kbase, c->d_in_off, c->d_scene_off, c->d_default_sphere};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -463,6 +463,9 @@ pbfx_status create_common(pbfx_ctx* c) {
     PBFX_ALLOC(b.rank, bc);
     PBFX_ALLOC(b.tmp, bc);
     PBFX_ALLOC(b.big, big_list_len(b.cap));
+    PBFX_ALLOC(b.bslot, bc);
+    PBFX_ALLOC(b.aux, bc);
+    PBFX_ALLOC(b.bseg, big_list_len(b.cap));
     PBFX_ALLOC(b.sidx, bc);
     PBFX_ALLOC(b.isid, bc);
     PBFX_ALLOC(b.sid, bc);
@@ -1439,6 +1442,9 @@ size_t carve_static(Bufs& b, uintptr_t base, uint32_t cap, uint32_t CT, uint32_t
     take(b.rank, bc);
     take(b.tmp, bc);
     take(b.big, big_list_len(cap));
+    take(b.bslot, bc);
+    take(b.aux, bc);
+    take(b.bseg, big_list_len(cap));
     take(b.sidx, bc);
     take(b.Pf, bc);
     take(b.Q, bc);

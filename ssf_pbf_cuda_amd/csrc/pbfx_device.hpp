@@ -523,12 +523,19 @@ __device__ __forceinline__ bool sorted_cell(uint32_t n) { return n > kSmallCell;
 
 // Lists cell k (its segment starting at o0) for the segment sort when
 // sorted_cell: cells up to kSortMax at big[nbig[0]++], larger ones at
-// big[huge_base(cap) + nbig[1]++].
+// big[huge_base(cap) + nbig[1]++]; bslot (when given) keeps that index at
+// the cell's first slot.
 __device__ __forceinline__ void list_big_cell(const uint32_t* __restrict__ offset, uint32_t k, uint32_t o0,
-                                              uint32_t* __restrict__ big, uint32_t* __restrict__ nbig, uint32_t cap) {
+                                              uint32_t* __restrict__ big, uint32_t* __restrict__ nbig, uint32_t cap,
+                                              uint32_t* __restrict__ bslot = nullptr) {
     const uint32_t n = offset[k + 1] - o0;
-    if (n > kSortMax) big[huge_base(cap) + atomicAdd(&nbig[1], 1u)] = k;
-    else if (n > kSmallCell) big[atomicAdd(&nbig[0], 1u)] = k;
+    uint32_t d = 0xFFFFFFFFu;
+    if (n > kSortMax) d = huge_base(cap) + atomicAdd(&nbig[1], 1u);
+    else if (n > kSmallCell) d = atomicAdd(&nbig[0], 1u);
+    if (d != 0xFFFFFFFFu) {
+        big[d] = k;
+        if (bslot) bslot[o0] = d;
+    }
 }
 
 // Block -> index range, contiguous per XCD (workgroup b runs on XCD b % 8),

@@ -18,6 +18,10 @@
 //                                                 the same sort keyed by each particle's final cell
 //   flist, fcnt                    u32[64*cap], u32[cap]  in-range pairs of the final passes by
 //                                                 final-order position (groups of four, list_at), counts
+//   bslot, aux                     u32[cap]      big cell -> descriptor (at its first slot); its
+//                                                 slots grouped by image.  After the sort, `rank`
+//                                                 holds each slot's group mark (0: not in a big cell)
+//   bseg                           BigSeg[big_list_len(cap)]  groups and bounding boxes of big cells
 //   ovf, part                      u32[M], f32x4[27*M]  overflow positions and their per-cell
 //                                                 partial sums (M = min(cap, kOvfMax))
 //   ctr                            u32[16*65]    allocation counters, then 64 copies of the
@@ -48,6 +52,27 @@ constexpr uint32_t kSortMax = 16384;
 // huge_base on) the cells above kSortMax.
 __host__ __device__ constexpr uint32_t huge_base(uint32_t cap) { return cap / kSmallCell + 2u; }
 __host__ __device__ constexpr uint32_t big_list_len(uint32_t cap) { return huge_base(cap) + cap / kSortMax + 2u; }
+// Segments of a big cell (more than kSmallCell particles).  The hash wraps,
+// so one cell can hold particles of several images of the grid (a pile on
+// the far wall x = gx*cw hashes into column 0, beside the particles at
+// x = 0) and dense piles far from a scanning particle.  Each big cell's
+// particles are grouped by image (a stable partition into `aux`), and each
+// group's bounding box is kept at the search's positions and at the final
+// ones: a scan skips every group whose box lies out of range, and when one
+// group remains it visits that group's slots only, in ascending order --
+// the same candidates in range, in the same order, as the whole cell.
+constexpr uint32_t kMaxSeg = 8;           // groups per big cell (more: the cell is scanned whole)
+constexpr uint32_t kCodeOther = 64;       // image code of non-finite positions and images beyond [-2, 1]
+constexpr uint32_t kCodes = kCodeOther + 1;
+constexpr uint32_t kBigBlocks = 256;      // workgroups (of 1024) of the per-big-cell kernels
+struct BigSeg {
+    uint32_t nseg;            // groups (0: scan the cell whole)
+    uint32_t end[kMaxSeg];    // end of group k in aux positions from the cell's first slot
+    uint32_t pad[7];
+    float4 slo[kMaxSeg], shi[kMaxSeg];  // box of the group's x* at the search
+    uint4 fmin[kMaxSeg];      // box of the group's final x* as order-preserving keys (fkey), w: a
+    uint4 fmax[kMaxSeg];      // member is non-finite; gathered by atomics
+};
 constexpr uint32_t kFinalK = 64;          // stored in-range pairs per particle for the final passes
 // Final-pass particles with more than kFinalK pairs (piles): up to kOvfMax of
 // them per step get their 27 per-cell partial sums computed in parallel
@@ -139,6 +164,9 @@ struct Bufs {
     uint32_t* tmp;
     uint32_t* big;            // [big_list_len(cap)] cells whose segment of tmp k_sort_cells sorts (huge
                               // cells from huge_base(cap) on)
+    uint32_t* bslot;          // [cap] at a big cell's first slot: its index in big (and bseg)
+    uint32_t* aux;            // [cap] a big cell's slots grouped by image, ascending in each group
+    BigSeg* bseg;             // [big_list_len(cap)] groups and boxes of each big cell
     uint32_t* sidx;
     float4* Pf;
     float4* Q;
@@ -189,6 +217,7 @@ enum KernelId : int {
     K_OUT,
     K_CUR_SORT,
     K_FINAL_LIST,
+    K_BIG_SEG,
     K_NUM
 };
 constexpr int kTimerSlots = 64;

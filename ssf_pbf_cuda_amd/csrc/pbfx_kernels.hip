@@ -228,12 +228,12 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ke
                                                  const uint32_t* __restrict__ offset, uint32_t* __restrict__ tmp,
                                                  uint32_t n, const uint32_t* __restrict__ n_dev,
                                                  uint32_t* __restrict__ big, uint32_t* __restrict__ nbig,
-                                                 uint32_t cap) {
+                                                 uint32_t cap, uint32_t* __restrict__ bslot) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (n_dev ? min(n, *n_dev) : n)) return;
     const uint32_t k = key[i], r = rank[i], o0 = offset[k];
     tmp[o0 + r] = i;
-    if (r == 0u) list_big_cell(offset, k, o0, big, nbig, cap);
+    if (r == 0u) list_big_cell(offset, k, o0, big, nbig, cap, bslot);
 }
 
 // ------------------------------------------------- K4b: large cells in stable order
@@ -400,6 +400,7 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
             f = o0 + r;
         }
         b.sidx[f] = i;
+        b.rank[f] = 0u;  // the histogram ranks are dead: the slot's big-cell group mark (k_big_seg)
         if (kB) b.sid[f] = sc;
         const Phys& P = SC[sc].P;
         const float dt = P.dt;
@@ -420,6 +421,322 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ x, co
         }
     });
     emit_units(b.offset, ek, ef, uw, budget, b.units, &b.ctr[CTR_UNITS]);  // all lanes
+}
+
+// ------------------------------------------------- K5b: groups of big cells
+// Image of a position in the wrapped grid, as a group code:
+// (ix+2) | (iy+2) << 2 | (iz+2) << 4 for the images ix = floor(cx / gx) of
+// the unwrapped cell coordinate cx (the hash's, grid_pos) in [-2, 1] on each
+// axis; kCodeOther for non-finite positions and farther images.  The code
+// only groups particles; the boxes decide every skip.
+__device__ __forceinline__ uint32_t image_code(F3 p, const Grid& G) {
+    if (!(__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z))) return kCodeOther;
+    int gp[3];
+    grid_pos(p, G.cw, gp);
+    const int ix = gp[0] >> (int)G.lgx, iy = gp[1] >> (int)G.lgy, iz = gp[2] >> (31 - __builtin_clz(G.gz));
+    if (ix < -2 || ix > 1 || iy < -2 || iy > 1 || iz < -2 || iz > 1) return kCodeOther;
+    return (uint32_t)(ix + 2) | ((uint32_t)(iy + 2) << 2) | ((uint32_t)(iz + 2) << 4);
+}
+
+// Running box of positions; `bad` once a non-finite one was seen (its group
+// is then never skipped).
+struct Box3 {
+    F3 lo, hi;
+    bool bad;
+};
+__device__ __forceinline__ Box3 box_empty() {
+    const float inf = __builtin_huge_valf();
+    return Box3{mk3(inf, inf, inf), mk3(-inf, -inf, -inf), false};
+}
+__device__ __forceinline__ void box_add(Box3& B, F3 p) {
+    if (!(__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z))) B.bad = true;
+    B.lo = mk3(fminf(B.lo.x, p.x), fminf(B.lo.y, p.y), fminf(B.lo.z, p.z));
+    B.hi = mk3(fmaxf(B.hi.x, p.x), fmaxf(B.hi.y, p.y), fmaxf(B.hi.z, p.z));
+}
+
+// The box over the whole block (every thread calls it); valid in thread 0.
+__device__ Box3 block_box(Box3 B, float (*red)[8]) {
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        B.lo = mk3(fminf(B.lo.x, __shfl_xor(B.lo.x, d, 64)), fminf(B.lo.y, __shfl_xor(B.lo.y, d, 64)),
+                   fminf(B.lo.z, __shfl_xor(B.lo.z, d, 64)));
+        B.hi = mk3(fmaxf(B.hi.x, __shfl_xor(B.hi.x, d, 64)), fmaxf(B.hi.y, __shfl_xor(B.hi.y, d, 64)),
+                   fmaxf(B.hi.z, __shfl_xor(B.hi.z, d, 64)));
+    }
+    B.bad = __any(B.bad);
+    if (lane == 0) {
+        red[wid][0] = B.lo.x;
+        red[wid][1] = B.lo.y;
+        red[wid][2] = B.lo.z;
+        red[wid][3] = B.hi.x;
+        red[wid][4] = B.hi.y;
+        red[wid][5] = B.hi.z;
+        red[wid][6] = B.bad ? 1.f : 0.f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (uint32_t w = 1; w < nw; ++w) {
+            B.lo = mk3(fminf(B.lo.x, red[w][0]), fminf(B.lo.y, red[w][1]), fminf(B.lo.z, red[w][2]));
+            B.hi = mk3(fmaxf(B.hi.x, red[w][3]), fmaxf(B.hi.y, red[w][4]), fmaxf(B.hi.z, red[w][5]));
+            B.bad = B.bad || red[w][6] != 0.f;
+        }
+    __syncthreads();
+    return B;
+}
+
+// The box as stored: a group with a non-finite member spans everything.
+__device__ __forceinline__ void box_store(const Box3& B, float4& lo, float4& hi) {
+    const float inf = __builtin_huge_valf();
+    lo = B.bad ? make_float4(-inf, -inf, -inf, 0.f) : make_float4(B.lo.x, B.lo.y, B.lo.z, 0.f);
+    hi = B.bad ? make_float4(inf, inf, inf, 0.f) : make_float4(B.hi.x, B.hi.y, B.hi.z, 0.f);
+}
+
+// Order-preserving key of a float (a < b  =>  fkey(a) <= fkey(b), -0 below
+// +0), so that boxes gather with integer atomics; funkey inverts it.
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float funkey(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k); }
+
+// Group mark of a big cell's member: descriptor d, group k (0 marks the
+// slots of other cells).
+__device__ __forceinline__ uint32_t big_mark(uint32_t d, uint32_t k) { return d * kMaxSeg + k + 1u; }
+
+// Adds the final x* of the lanes' marked slots to their groups' final boxes.
+// Lanes of one group merge over the wave first and one of them issues the
+// atomics.  Called by every lane of the wave (bm = 0 where nothing is added).
+__device__ void big_box_add(BigSeg* __restrict__ bseg, uint32_t bm, F3 p) {
+    const uint32_t lane = lane_id();
+    const bool bad = !(__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z));
+    const uint32_t k[3] = {fkey(p.x), fkey(p.y), fkey(p.z)};
+    uint64_t todo = __ballot(bm != 0u);
+    while (todo) {
+        const int l = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t key = (uint32_t)__shfl((int)bm, l, 64);
+        const bool mine = bm == key;
+        uint32_t lo[3], hi[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = mine ? k[c] : 0xFFFFFFFFu;
+            hi[c] = mine ? k[c] : 0u;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = min(lo[c], (uint32_t)__shfl_xor((int)lo[c], d, 64));
+                hi[c] = max(hi[c], (uint32_t)__shfl_xor((int)hi[c], d, 64));
+            }
+        const bool anybad = __any(mine && bad);
+        if (lane == (uint32_t)l) {
+            BigSeg& D = bseg[(key - 1u) / kMaxSeg];
+            const uint32_t g = (key - 1u) % kMaxSeg;
+            atomicMin(&D.fmin[g].x, lo[0]);
+            atomicMin(&D.fmin[g].y, lo[1]);
+            atomicMin(&D.fmin[g].z, lo[2]);
+            atomicMax(&D.fmax[g].x, hi[0]);
+            atomicMax(&D.fmax[g].y, hi[1]);
+            atomicMax(&D.fmax[g].z, hi[2]);
+            if (anybad) atomicOr(&D.fmin[g].w, 1u);
+        }
+        todo &= ~__ballot(mine);
+    }
+}
+
+// Big cell of work item `it` (huge cells first: they take longest): its
+// descriptor index d and key c.
+__device__ __forceinline__ uint32_t big_item(const Bufs& b, uint32_t it, uint32_t nh, uint32_t& c) {
+    const uint32_t d = it < nh ? huge_base(b.cap) + it : it - nh;
+    c = b.big[d];
+    return d;
+}
+
+// Groups the slots of every big cell by image code: a stable partition into
+// aux (ascending slots inside each group, the groups in code order), the
+// groups' ends and the boxes of their x* (after k_reorder, before the
+// search).  One workgroup per cell; a cell of one group keeps aux unused.
+template <bool kB>
+__global__ __launch_bounds__(1024) void k_big_seg(Bufs b, Scenes<kB> SC) {
+    __shared__ uint32_t cnt[kCodes];
+    __shared__ uint32_t segof[kCodes];       // group of each code (kMaxSeg: none)
+    __shared__ uint32_t sbeg[kMaxSeg + 1];   // first aux position of each group, then the cell's size
+    __shared__ uint32_t put[kMaxSeg];        // where each group's next chunk goes
+    __shared__ uint32_t wcnt[16][kMaxSeg];   // members of each group per wave in a chunk
+    __shared__ uint32_t nseg_s;
+    __shared__ float red[16][8];
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t nb0 = b.ctr[CTR_BIGCELLS], nh = b.ctr[CTR_HUGECELLS];
+    for (uint32_t it = blockIdx.x; it < nh + nb0; it += gridDim.x) {
+        uint32_t c;
+        const uint32_t d = big_item(b, it, nh, c);
+        const uint32_t o0 = b.offset[c], n = b.offset[c + 1] - o0;
+        const Grid G = SC[scene_index<kB>(b.kbase, b.nscenes, c)].G;
+        for (uint32_t k = threadIdx.x; k < kCodes; k += blockDim.x) cnt[k] = 0u;
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) atomicAdd(&cnt[image_code(f3of(b.Pf[o0 + p]), G)], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t ns = 0, run = 0;
+            for (uint32_t k = 0; k < kCodes; ++k) {
+                segof[k] = kMaxSeg;
+                if (cnt[k]) {
+                    if (ns < kMaxSeg) {
+                        segof[k] = ns;
+                        sbeg[ns] = run;
+                        put[ns] = run;
+                    }
+                    ++ns;
+                    run += cnt[k];
+                }
+            }
+            nseg_s = ns <= kMaxSeg ? ns : 0u;
+            if (ns <= kMaxSeg) sbeg[ns] = n;
+        }
+        __syncthreads();
+        const uint32_t ns = nseg_s;
+        if (ns >= 2u) {
+            for (uint32_t q0 = 0; q0 < n; q0 += blockDim.x) {
+                const uint32_t p = q0 + threadIdx.x;
+                const uint32_t sg = p < n ? segof[image_code(f3of(b.Pf[o0 + p]), G)] : kMaxSeg;
+                uint32_t lrank = 0;
+                for (uint32_t k = 0; k < ns; ++k) {
+                    const uint64_t m = __ballot(sg == k);
+                    if (sg == k) lrank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    if (lane == 0) wcnt[wid][k] = (uint32_t)__popcll(m);
+                }
+                __syncthreads();
+                if (sg < kMaxSeg) {
+                    uint32_t o = put[sg] + lrank;
+                    for (uint32_t w = 0; w < wid; ++w) o += wcnt[w][sg];
+                    b.aux[o0 + o] = o0 + p;
+                    b.rank[o0 + p] = big_mark(d, sg);
+                }
+                __syncthreads();
+                if (threadIdx.x < ns) {
+                    uint32_t t = 0;
+                    for (uint32_t w = 0; w < nw; ++w) t += wcnt[w][threadIdx.x];
+                    put[threadIdx.x] += t;
+                }
+                __syncthreads();
+            }
+        }
+        BigSeg& D = b.bseg[d];
+        if (ns == 1u)
+            for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) b.rank[o0 + p] = big_mark(d, 0u);
+        for (uint32_t k = 0; k < ns; ++k) {
+            const uint32_t p0 = sbeg[k], p1 = sbeg[k + 1];
+            Box3 B = box_empty();
+            for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x)
+                box_add(B, f3of(b.Pf[ns == 1u ? o0 + p : b.aux[o0 + p]]));
+            B = block_box(B, red);
+            if (threadIdx.x == 0) {
+                box_store(B, D.slo[k], D.shi[k]);
+                D.end[k] = p1;
+                // the final box, gathered by k_cur_hash
+                D.fmin[k] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);
+                D.fmax[k] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+        if (threadIdx.x == 0) D.nseg = ns;
+        __syncthreads();
+    }
+}
+
+// What a scan of big cell D (n particles) for a particle at `self` must
+// visit.  kFinal: the final positions, whose pair test passes
+// !(glm_dot(d, d) > thr); otherwise the search's, which passes
+// fmaf-dot(d, d) < thr.  Every component of a member's difference to self
+// lies, in magnitude, between the rounded gap from self to the group's box
+// and the rounded distance to the box's far side (rounding is monotone), and
+// both dot products are monotone in the components' magnitudes: a group
+// whose gap fails the test holds no pair, and one whose far corner passes it
+// holds only pairs.  A non-finite self skips nothing.
+struct BigPart {
+    uint32_t how;     // 0: no member passes; 1: visit the whole cell; 2: visit aux positions [p0, p1) only
+    uint32_t p0, p1;  // the one group that may pass (how 2, or how 1 with a single group: the whole cell)
+    bool all_in;      // every member of that group passes
+    uint32_t inside;  // members of the cell's groups whose every member passes
+};
+__device__ __forceinline__ float box_gap(float x, float lo, float hi) {
+    return (x < lo) ? (lo - x) : ((x > hi) ? (x - hi) : 0.f);
+}
+__device__ __forceinline__ float box_far(float x, float lo, float hi) { return fmaxf(x - lo, hi - x); }
+template <bool kFinal>
+__device__ __forceinline__ bool pair_test(F3 d, float thr) {
+    return kFinal ? !(glm_dot(d, d) > thr) : (__builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x)) < thr);
+}
+template <bool kFinal>
+__device__ __forceinline__ BigPart big_part(const BigSeg* __restrict__ D, F3 self, float thr, uint32_t n) {
+    BigPart r{1u, 0u, n, false, 0u};
+    const uint32_t ns = D->nseg;
+    if (ns == 0u) return r;
+    uint32_t hits = 0, beg = 0, p0 = 0, p1 = 0;
+    bool in0 = false;
+    for (uint32_t k = 0; k < ns; ++k) {
+        const uint32_t end = D->end[k];
+        float4 lo, hi;
+        if (kFinal) {
+            const uint4 a = D->fmin[k], z = D->fmax[k];
+            const float inf = __builtin_huge_valf();
+            lo = a.w ? make_float4(-inf, -inf, -inf, 0.f) : make_float4(funkey(a.x), funkey(a.y), funkey(a.z), 0.f);
+            hi = a.w ? make_float4(inf, inf, inf, 0.f) : make_float4(funkey(z.x), funkey(z.y), funkey(z.z), 0.f);
+        } else {
+            lo = D->slo[k];
+            hi = D->shi[k];
+        }
+        const F3 g = mk3(box_gap(self.x, lo.x, hi.x), box_gap(self.y, lo.y, hi.y), box_gap(self.z, lo.z, hi.z));
+        const F3 f = mk3(box_far(self.x, lo.x, hi.x), box_far(self.y, lo.y, hi.y), box_far(self.z, lo.z, hi.z));
+        const bool in = pair_test<kFinal>(f, thr);
+        if (pair_test<kFinal>(g, thr)) {
+            if (hits == 0u) {
+                p0 = beg;
+                p1 = end;
+                in0 = in;
+            }
+            ++hits;
+        }
+        if (in) r.inside += end - beg;
+        beg = end;
+    }
+    if (hits == 0u) {
+        r.how = 0u;
+    } else if (hits == 1u) {
+        r.how = ns == 1u ? 1u : 2u;
+        r.p0 = p0;
+        r.p1 = p1;
+        r.all_in = in0;
+    }
+    return r;
+}
+
+// What the global-memory scans read of a step's cells: the cell offsets and
+// the big cells' descriptors (passed by value, so that no kernel argument
+// block is copied to scratch memory for its address).
+struct CellView {
+    const uint32_t* __restrict__ offset;
+    const uint32_t* __restrict__ bslot;
+    const uint32_t* __restrict__ aux;
+    const BigSeg* __restrict__ bseg;
+};
+__device__ __forceinline__ CellView cell_view(const Bufs& b) { return CellView{b.offset, b.bslot, b.aux, b.bseg}; }
+
+// Whether this step has cells of more than kSmallCell particles (a
+// grid-uniform value: the scans without them keep their plain loops).
+__device__ __forceinline__ bool big_cells(const Bufs& b) { return (b.ctr[CTR_BIGCELLS] | b.ctr[CTR_HUGECELLS]) != 0u; }
+
+// a[i] by selects (i in 0..3, not known at compile time): no array in
+// scratch memory
+__device__ __forceinline__ uint32_t pick4(const uint32_t (&a)[4], uint32_t i) {
+    const uint32_t lo = (i & 1u) ? a[1] : a[0], hi = (i & 1u) ? a[3] : a[2];
+    return (i & 2u) ? hi : lo;
+}
+
+// Whether visiting one group through aux beats scanning the whole cell (the
+// indirection costs a load per candidate): when it skips a quarter or more.
+__device__ __forceinline__ bool aux_pays(const BigPart& bp, uint32_t n) {
+    return bp.how == 2u && 4u * (bp.p1 - bp.p0) <= 3u * n;
 }
 
 
@@ -645,6 +962,88 @@ __device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, u
     return cnt;
 }
 
+// The same filter from global memory, for units whose neighbourhood exceeds
+// the tile: candidate x*_j = Pf[j] by slot.  In a big cell, groups out of
+// range are skipped, a group wholly in range is listed without tests, and a
+// single group left is scanned through aux when that skips a quarter of the
+// cell or more.  Every scan is one loop whose slots come, per lane, from aux
+// or from the cell's own run, the next group's slots in flight.
+__device__ __forceinline__ uint32_t ns_filter_global(const UnitShared& S, uint32_t W, uint32_t kc, F3 self,
+                                                     float h2, uint32_t ncap, const float4* __restrict__ Pf,
+                                                     CellView b, uint32_t* __restrict__ nl, uint32_t cap,
+                                                     uint32_t g, bool& trunc) {
+    uint32_t* __restrict__ dst = nl + list_at(cap, g, 0u);  // where entry cnt goes
+    const size_t jump = (size_t)4u * cap - 3u;              // from entry 4q+3 to 4q+4
+    uint32_t cnt = 0;
+    for (uint32_t c = 0; c < 27u; ++c) {
+        const uint32_t e = (c / 3u) * (W + 2u) + kc - 1u + (c % 3u);
+        const uint32_t o0 = S.gstart[e], ncell = S.tstart[e + 1] - S.tstart[e];
+        uint32_t p0 = 0, p1 = ncell;
+        bool via = false;
+        if (ncell > kSmallCell) {
+            const BigPart bp = big_part<false>(b.bseg + b.bslot[o0], self, h2, ncell);
+            if (bp.how == 0u) continue;
+            if (bp.all_in) {
+                // every member of the group is a pair: its first slots, ascending
+                const uint32_t room = ncap - cnt, size = bp.p1 - bp.p0, take = min(room, size);
+                for (uint32_t q = 0; q < take; ++q) {
+                    *dst = bp.how == 2u ? b.aux[o0 + bp.p0 + q] : o0 + bp.p0 + q;
+                    ++cnt;
+                    dst += ((cnt & 3u) == 0u) ? jump : 1u;
+                }
+                if (size > room) {
+                    trunc = true;
+                    return cnt;
+                }
+                continue;
+            }
+            if (aux_pays(bp, ncell)) {
+                p0 = bp.p0;
+                p1 = bp.p1;
+                via = true;
+            }
+        }
+        if (p1 == p0) continue;
+        const uint32_t* __restrict__ ax = b.aux + o0;
+        uint32_t jn[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) jn[i] = via ? ax[min(p0 + i, p1 - 1u)] : o0 + min(p0 + i, p1 - 1u);
+        for (uint32_t p = p0; p < p1; p += 4u) {
+            const uint32_t j[4] = {jn[0], jn[1], jn[2], jn[3]};
+            F3 pj[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) pj[i] = f3of(Pf[j[i]]);
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i)
+                jn[i] = via ? ax[min(p + 4u + i, p1 - 1u)] : o0 + min(p + 4u + i, p1 - 1u);
+            uint32_t m = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) m |= pair_test<false>(self - pj[i], h2) ? (1u << i) : 0u;
+            m &= (p1 - p >= 4u) ? 0xFu : ((1u << (p1 - p)) - 1u);
+            // keep the first ncap - cnt hits of the group; any beyond them
+            // truncate the list
+            const uint32_t room = ncap - cnt;
+            bool full = false;
+            while ((uint32_t)__builtin_popcount(m) > room) {
+                m &= ~(0x80000000u >> __builtin_clz(m));
+                full = true;
+            }
+            while (m) {
+                const uint32_t i = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                *dst = pick4(j, i);
+                ++cnt;
+                dst += ((cnt & 3u) == 0u) ? jump : 1u;
+            }
+            if (full) {
+                trunc = true;
+                return cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
 template <bool kB>
 __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t tile_cap) {
     __shared__ UnitShared S;
@@ -673,7 +1072,7 @@ __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t t
             bool trunc = false;
             const uint32_t cnt =
                 I.staged ? ns_filter<true>(S, I.W, kc, tile[ts], h2, ncap, tile, nl, cap, g, trunc)
-                         : ns_filter<false>(S, I.W, kc, f3of(Pf[g]), h2, ncap, Pf, nl, cap, g, trunc);
+                         : ns_filter_global(S, I.W, kc, f3of(Pf[g]), h2, ncap, Pf, cell_view(b), nl, cap, g, trunc);
             b.ncnt[g] = cnt;
             tl.trunc += trunc ? 1u : 0u;
         }
@@ -803,6 +1202,9 @@ __global__ __launch_bounds__(256) void k_cur_hash(Bufs b, Scenes<kB> SC, uint32_
         grid_pos(p, G.cw, gp);
         k = ((scene_key(gp[0], gp[1], gp[2], G) >> lguw) << lgnc) | (blockIdx.x & ((1u << lgnc) - 1u));
     });
+    // the final boxes of the big cells' groups (the marks k_big_seg left)
+    const uint32_t bm = active ? b.rank[s] : 0u;
+    if (__any(bm != 0u)) big_box_add(b.bseg, bm, active ? f3of(b.Pf[s]) : mk3(0.f, 0.f, 0.f));
     const uint32_t r = wave_rank_any(b.ccount, k, active, count_slot(k, CR, lgnc));
     if (active) {
         mark_tile(b.ctflag, k >> lgT);
@@ -897,10 +1299,14 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
 
 // The same from global memory, row by row (dz outer, dy inner), the next
 // row's three cell ranges loaded while the current row is filtered; pos(j)
-// gives x*_j.  Stops once n reaches lim.
-template <class PosF, class EmitF>
-__device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self, float cut2,
-                                            const uint32_t& n, uint32_t lim, PosF&& pos, EmitF&& emit) {
+// gives x*_j.  kBig (the step has big cells): a big cell's groups out of
+// range are skipped by their final boxes, one wholly in range is listed
+// without tests, and the scan stops at once when such groups alone take the
+// list to lim.  Stops once n reaches lim.
+template <bool kBig, class PosF, class EmitF>
+__device__ __forceinline__ void list_global(const Grid& G, CellView b, F3 self, float cut2, uint32_t& n,
+                                            uint32_t lim, PosF&& pos, EmitF&& emit) {
+    const uint32_t* __restrict__ offset = b.offset;
     int gp[3];
     grid_pos(self, G.cw, gp);
     auto row = [&](int r, uint32_t* st, uint32_t* en) {
@@ -919,22 +1325,72 @@ __device__ __forceinline__ void list_global(const Grid& G, const uint32_t* __res
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
             bool first = true;
-            for (uint32_t j = st[dx]; j < en[dx]; j += 4u) {
+            if (!kBig) {
+                // a step without big cells: the plain scan
+                for (uint32_t j = st[dx]; j < en[dx]; j += 4u) {
+                    F3 pj[4];
+#pragma unroll
+                    for (uint32_t i = 0; i < 4u; ++i) pj[i] = pos(min(j + i, en[dx] - 1u));
+                    uint32_t m = 0;
+#pragma unroll
+                    for (uint32_t i = 0; i < 4u; ++i) m |= pair_test<true>(pj[i] - self, cut2) ? (1u << i) : 0u;
+                    m &= (en[dx] - j >= 4u) ? 0xFu : ((1u << (en[dx] - j)) - 1u);
+                    const bool full = clip_hits(m, lim - n);
+                    while (m) {
+                        const uint32_t i = (uint32_t)__builtin_ctz(m);
+                        m &= m - 1u;
+                        emit(j + i, first);
+                        first = false;
+                    }
+                    if (full || n == lim) return;
+                }
+                continue;
+            }
+            const uint32_t o0 = st[dx], ncell = en[dx] - st[dx];
+            uint32_t p0 = 0, p1 = ncell;
+            bool via = false;
+            if (ncell > kSmallCell) {
+                const BigPart bp = big_part<true>(b.bseg + b.bslot[o0], self, cut2, ncell);
+                if (bp.how == 0u) continue;
+                if (n + bp.inside >= lim) {  // the list reaches lim here: no need to store it
+                    n = lim;
+                    return;
+                }
+                if (bp.all_in) {  // fewer than lim - n members, all pairs
+                    for (uint32_t q = bp.p0; q < bp.p1; ++q) {
+                        emit(bp.how == 2u ? b.aux[o0 + q] : o0 + q, first);
+                        first = false;
+                    }
+                    continue;
+                }
+                if (aux_pays(bp, ncell)) {
+                    p0 = bp.p0;
+                    p1 = bp.p1;
+                    via = true;
+                }
+            }
+            if (p1 == p0) continue;
+            const uint32_t* __restrict__ ax = b.aux + o0;
+            uint32_t jn[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) jn[i] = via ? ax[min(p0 + i, p1 - 1u)] : o0 + min(p0 + i, p1 - 1u);
+            for (uint32_t p = p0; p < p1; p += 4u) {
+                const uint32_t jj[4] = {jn[0], jn[1], jn[2], jn[3]};
                 F3 pj[4];
 #pragma unroll
-                for (uint32_t i = 0; i < 4u; ++i) pj[i] = pos(min(j + i, en[dx] - 1u));
+                for (uint32_t i = 0; i < 4u; ++i) pj[i] = pos(jj[i]);
+#pragma unroll
+                for (uint32_t i = 0; i < 4u; ++i)
+                    jn[i] = via ? ax[min(p + 4u + i, p1 - 1u)] : o0 + min(p + 4u + i, p1 - 1u);
                 uint32_t m = 0;
 #pragma unroll
-                for (uint32_t i = 0; i < 4u; ++i) {
-                    const F3 d = pj[i] - self;
-                    m |= !(glm_dot(d, d) > cut2) ? (1u << i) : 0u;
-                }
-                m &= (en[dx] - j >= 4u) ? 0xFu : ((1u << (en[dx] - j)) - 1u);
+                for (uint32_t i = 0; i < 4u; ++i) m |= pair_test<true>(pj[i] - self, cut2) ? (1u << i) : 0u;
+                m &= (p1 - p >= 4u) ? 0xFu : ((1u << (p1 - p)) - 1u);
                 const bool full = clip_hits(m, lim - n);
                 while (m) {
                     const uint32_t i = (uint32_t)__builtin_ctz(m);
                     m &= m - 1u;
-                    emit(j + i, first);
+                    emit(pick4(jj, i), first);
                     first = false;
                 }
                 if (full || n == lim) return;
@@ -962,21 +1418,55 @@ __device__ __forceinline__ uint32_t cell27(const int gp[3], uint32_t c, const Gr
 }
 
 // Partial sum of one cell (key) in slot order: the per-cell sum of
-// calcContribution; elem(j) builds the pass's element of slot j.
-template <class ElemF, class TermF>
-__device__ __forceinline__ F3 sum_cell(const uint32_t* __restrict__ offset, uint32_t key, F3 self, float cut2,
-                                       ElemF&& elem, TermF&& term) {
-    const uint32_t st = offset[key], en = offset[key + 1];
+// calcContribution; elem(j) builds the pass's element of slot j.  A big
+// cell's groups out of range (final boxes) add nothing and are skipped; a
+// single group left is summed through aux, in slot order.
+template <bool kBig, class ElemF, class TermF>
+__device__ __forceinline__ F3 sum_cell(CellView b, uint32_t key, F3 self, float cut2, ElemF&& elem,
+                                       TermF&& term) {
+    const uint32_t st = b.offset[key], n = b.offset[key + 1] - st;
     F3 cs = mk3(0.f, 0.f, 0.f);
+    if (!kBig) {
+        // four candidates' records loaded together, then added in order
+        for (uint32_t j = st; j < st + n; j += 4u) {
+            El2 el[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(min(j + i, st + n - 1u));
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) {
+                const F3 d = f3of(el[i].p) - self;
+                if (j + i < st + n && !(glm_dot(d, d) > cut2)) acc(cs, term(el[i]));
+            }
+        }
+        return cs;
+    }
+    uint32_t p0 = 0, p1 = n;
+    bool via = false;
+    if (n > kSmallCell) {
+        const BigPart bp = big_part<true>(b.bseg + b.bslot[st], self, cut2, n);
+        if (bp.how == 0u) return cs;
+        if (aux_pays(bp, n)) {
+            p0 = bp.p0;
+            p1 = bp.p1;
+            via = true;
+        }
+    }
+    if (p1 == p0) return cs;
+    const uint32_t* __restrict__ ax = b.aux + st;
+    uint32_t jn[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) jn[i] = via ? ax[min(p0 + i, p1 - 1u)] : st + min(p0 + i, p1 - 1u);
     // four candidates' records loaded together, then added in order
-    for (uint32_t j = st; j < en; j += 4u) {
+    for (uint32_t p = p0; p < p1; p += 4u) {
         El2 el[4];
 #pragma unroll
-        for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(min(j + i, en - 1u));
+        for (uint32_t i = 0; i < 4u; ++i) el[i] = elem(jn[i]);
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) jn[i] = via ? ax[min(p + 4u + i, p1 - 1u)] : st + min(p + 4u + i, p1 - 1u);
 #pragma unroll
         for (uint32_t i = 0; i < 4u; ++i) {
             const F3 d = f3of(el[i].p) - self;
-            if (j + i < en && !(glm_dot(d, d) > cut2)) acc(cs, term(el[i]));
+            if (p + i < p1 && !(glm_dot(d, d) > cut2)) acc(cs, term(el[i]));
         }
     }
     return cs;
@@ -984,13 +1474,13 @@ __device__ __forceinline__ F3 sum_cell(const uint32_t* __restrict__ offset, uint
 
 // Direct sum over the 27 cells from global memory, for particles whose list
 // overflowed: the cells' partial sums added in z,y,x order.
-template <class ElemF, class TermF>
-__device__ __forceinline__ F3 sum27_global(const Grid& G, const uint32_t* __restrict__ offset, F3 self, float cut2,
-                                           ElemF&& elem, TermF&& term) {
+template <bool kBig, class ElemF, class TermF>
+__device__ __forceinline__ F3 sum27_global(const Grid& G, CellView b, F3 self, float cut2, ElemF&& elem,
+                                           TermF&& term) {
     int gp[3];
     grid_pos(self, G.cw, gp);
     F3 sum = mk3(0.f, 0.f, 0.f);
-    for (uint32_t c = 0; c < 27u; ++c) acc(sum, sum_cell(offset, cell27(gp, c, G), self, cut2, elem, term));
+    for (uint32_t c = 0; c < 27u; ++c) acc(sum, sum_cell<kBig>(b, cell27(gp, c, G), self, cut2, elem, term));
     return sum;
 }
 
@@ -1051,9 +1541,12 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
     const float4* __restrict__ Pf = b.Pf;
     const uint32_t cap = b.cap;
     uint32_t* __restrict__ fl = b.flist;
+    const CellView cv = cell_view(b);
+    const bool any_big = big_cells(b);
     Tally tl;
     UnitInfo I;
-    auto build = [&](uint32_t q, uint32_t g, bool tiled, const Grid& G, float cut2) {
+    // inlined at both call sites: a call would copy the kernel arguments to scratch memory
+    auto build = [&](uint32_t q, uint32_t g, bool tiled, const Grid& G, float cut2) __attribute__((always_inline)) {
         const F3 self = f3of(Pf[g]);
         uint32_t n = 0;
         uint32_t* __restrict__ dst = fl + list_at(cap, q, 0u);  // where entry n goes
@@ -1067,7 +1560,10 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
             ++n;
         };
         if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, n, kFinalK + 1u, emit);
-        else list_global(G, b.offset, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(Pf[j]); }, emit);
+        else if (any_big)
+            list_global<true>(G, cv, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(Pf[j]); }, emit);
+        else
+            list_global<false>(G, cv, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(Pf[j]); }, emit);
         b.fcnt[q] = n;
         tl.listovf += n > kFinalK ? 1u : 0u;
     };
@@ -1257,8 +1753,11 @@ template <class Pass>
 __device__ __forceinline__ F3 pass_cell(const Bufs& b, const Grid& G, const Pass& ps, uint32_t c) {
     int gp[3];
     grid_pos(ps.self, G.cw, gp);
-    return sum_cell(b.offset, cell27(gp, c, G), ps.self, ps.cut2, [&](uint32_t j) { return ps.elem(j); },
-                    [&](const El2& e) { return ps.term(e); });
+    auto elem = [&](uint32_t j) { return ps.elem(j); };
+    auto term = [&](const El2& e) { return ps.term(e); };
+    const uint32_t key = cell27(gp, c, G);
+    return big_cells(b) ? sum_cell<true>(cell_view(b), key, ps.self, ps.cut2, elem, term)
+                        : sum_cell<false>(cell_view(b), key, ps.self, ps.cut2, elem, term);
 }
 
 // The pass's sum for final-order position q with stored count n: the list
@@ -1276,7 +1775,9 @@ __device__ __forceinline__ F3 pass_sum(const Bufs& b, const Grid& G, const Pass&
         }
         return sum_list(b.flist, b.cap, q, n, elem, term);
     }
-    if (n == kFinalK + 1u) return sum27_global(G, b.offset, ps.self, ps.cut2, elem, term);
+    if (n == kFinalK + 1u)
+        return big_cells(b) ? sum27_global<true>(G, cell_view(b), ps.self, ps.cut2, elem, term)
+                            : sum27_global<false>(G, cell_view(b), ps.self, ps.cut2, elem, term);
     const size_t k = n - (kFinalK + 2u);
     F3 sum = mk3(0.f, 0.f, 0.f);
     for (uint32_t c = 0; c < 27u; ++c) acc(sum, f3of(b.part[(size_t)c * b.ovf_max + k]));
@@ -1581,7 +2082,7 @@ const char* kernel_name(int id) {
     static const char* names[K_NUM] = {"k_predict_hash", "k_scan_tiles", "k_scan_cells", "k_scatter",
                                        "k_reorder",      "k_ns",         "k_lambda",     "k_deltap",
                                        "k_vort",         "k_confine",    "k_xsph_out",   "k_out",
-                                       "k_cur_sort",     "k_final_list"};
+                                       "k_cur_sort",     "k_final_list",  "k_big_seg"};
     return (id >= 0 && id < K_NUM) ? names[id] : "?";
 }
 
@@ -1631,7 +2132,7 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     {
         Mark m(tm, s, K_SCATTER);
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, b.key, b.rank, b.offset, b.tmp, n,
-                           kB ? b.in_off + b.nscenes : (const uint32_t*)nullptr, b.big, &b.ctr[CTR_BIGCELLS], b.cap);
+                           kB ? b.in_off + b.nscenes : (const uint32_t*)nullptr, b.big, &b.ctr[CTR_BIGCELLS], b.cap, b.bslot);
         // the histogram ranks are dead after the scatter: their array holds the huge cells' chunks
         static_assert(CTR_HUGECELLS == CTR_BIGCELLS + 1, "the sort reads the two cell counts together");
         hipLaunchKernelGGL(k_sort_cells, dim3(256), dim3(1024), 0, s, b.offset, b.big, &b.ctr[CTR_BIGCELLS], b.tmp,
@@ -1642,6 +2143,10 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     {
         Mark m(tm, s, K_REORDER);
         hipLaunchKernelGGL(k_reorder<kB>, dim3(nb), dim3(256), 0, s, x, v, b, SC, b.key, b.tmp, n, uw, budget);
+    }
+    {
+        Mark m(tm, s, K_BIG_SEG);
+        hipLaunchKernelGGL(k_big_seg<kB>, dim3(kBigBlocks), dim3(1024), 0, s, b, SC);
     }
     {
         Mark m(tm, s, K_NS);

@@ -372,11 +372,11 @@ pbfx_status pbfx_adj_backward(pbfx_ctx* ctx, const double* gx, const double* gv,
  *  x, v, num, lo, hi, iters, stream  as for pbfx_step_async (the count after
  *               deletion is not reported: pass an area that deletes nothing,
  *               or use a context)
- * Scratch needed: about (400 + 4 * stride) bytes per particle, stride =
- * max_pair - 1 rounded up to a multiple of 4 (528 B at 30 slots), plus 16 bytes
+ * Scratch needed: about (417 + 4 * stride) bytes per particle, stride =
+ * max_pair - 1 rounded up to a multiple of 4 (545 B at 30 slots), plus 16 bytes
  * per grid cell (the grid padded to a power of two or to 4096 cells) and
- * ~8 KB; PBFX_CAPACITY when that exceeds PBFX_STATIC_SCRATCH_BYTES (e.g. the
- * 32,768-particle dam on its 32 x 32 x 16 grid needs ~17.6 MB).  Every call on
+ * ~1.8 MB; PBFX_CAPACITY when that exceeds PBFX_STATIC_SCRATCH_BYTES (e.g. the
+ * 32,768-particle dam on its 32 x 32 x 16 grid needs ~19.9 MB).  Every call on
  * a GPU shares that GPU's array: calls that overlap on one GPU (concurrent
  * streams or host threads) race and must be serialised by the caller.  The
  * array keeps the last call's contents; each call clears what must start at

@@ -1522,6 +1522,26 @@ __device__ __forceinline__ uint32_t final_column(const Grid& G, const UnitInfo& 
     return (((uint32_t)gp[0] & (G.gx - 1u)) - I.xb) + 1u;
 }
 
+// ------------------------------------------------- overflow lists
+// A final-order position whose list overflowed is listed in ovf, up to
+// ovf_max of them, as its build ends (one atomic per group of lanes that end
+// together); its count becomes kFinalK + 2 + its index, so that the passes
+// add its per-cell partials, computed for all 27 cells at once by
+// k_ovf_part, instead of summing the cells one after another.  Past ovf_max
+// it keeps kFinalK + 1 (the direct sum).  Called by the lanes whose list
+// overflowed (any subset of the wave).
+__device__ __forceinline__ uint32_t ovf_entry(const Bufs& b, uint32_t q) {
+    const uint64_t m = __ballot(true);
+    const uint32_t lane = lane_id(), leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&b.ctr[CTR_OVF], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (k >= b.ovf_max) return kFinalK + 1u;
+    b.ovf[k] = q;
+    return kFinalK + 2u + k;
+}
+
 // ------------------------------------------------- K9: final pair lists
 // For each particle (by position q of the final-cell order) the slots of its
 // in-range pairs in groups of four (list_at(cap, q, k), k < kFinalK); fcnt[q] =
@@ -1564,6 +1584,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
             list_global<true>(G, cv, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(Pf[j]); }, emit);
         else
             list_global<false>(G, cv, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(Pf[j]); }, emit);
+        if (n > kFinalK) n = ovf_entry(b, q);
         b.fcnt[q] = n;
         tl.listovf += n > kFinalK ? 1u : 0u;
     };
@@ -1784,29 +1805,6 @@ __device__ __forceinline__ F3 pass_sum(const Bufs& b, const Grid& G, const Pass&
     return sum;
 }
 
-// ------------------------------------------------- overflow lists
-// The final-order positions whose list overflowed (fcnt = kFinalK + 1) are
-// listed in ovf, up to ovf_max of them (one atomic per wave); a listed
-// position's fcnt becomes kFinalK + 2 + its index, so that the passes add its
-// per-cell partials, computed for all 27 cells at once by k_ovf_part, instead
-// of summing the cells one after another.  The others keep the direct sum.
-__global__ __launch_bounds__(256) void k_ovf_list(Bufs b) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool mine = q < b.offset[b.CT] && b.fcnt[q] == kFinalK + 1u;
-    const uint64_t m = __ballot(mine);
-    if (!m) return;  // uniform over the wave
-    const uint32_t lane = lane_id();
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&b.ctr[CTR_OVF], (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, 0, 64);
-    if (mine) {
-        const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (k < b.ovf_max) {
-            b.ovf[k] = q;
-            b.fcnt[q] = kFinalK + 2u + k;
-        }
-    }
-}
 
 // Per-cell partials of pass kPass for the listed overflow positions.  A wave
 // task is (group of 64 list entries, cell c): each lane sums cell c of its
@@ -2195,7 +2193,6 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
         Mark m(tm, s, K_FINAL_LIST);
         const uint32_t ft = final_tile(T);
         hipLaunchKernelGGL(k_final_list<kB>, grid(K_FINAL_LIST), dim3(kBlock), smem_of(ft), s, b, SC, ft, lguw, lgnc);
-        hipLaunchKernelGGL(k_ovf_list, dim3(nb), dim3(256), 0, s, b);
     }
     // the overflow positions' per-cell partials, before each pass that adds them
     const dim3 ovf_grid(kOvfBlocks);

@@ -76,3 +76,15 @@ def test_aliased_piles_two_steps_bit_exact():
     bx, bv = bit_mismatches(got["x"], ref["x"]), bit_mismatches(got["v"], ref["v"])
     print(f"aliased 2 steps: bitdiff_x={bx} bitdiff_v={bv}", flush=True)
     assert bx == 0 and bv == 0
+
+
+def test_aliased_piles_in_a_batch_bit_exact():
+    """The same piles as one scene of a batch beside a plain dam: big cells
+    are grouped per scene (each scene's own grid and key range), and each
+    scene's state equals the oracle's plain step of that scene."""
+    from test_gpu_batch import _check_against_oracle, _run_batch
+
+    # a batch runs one iteration count: the pile scene's (2)
+    scs = [scenes.small_dam(12, 10, 8, seed=3, iters=2), aliased_scene(far=1500, near=900, point=300, seed=14)]
+    print(f"batch with aliased piles: {[s.n for s in scs]}", flush=True)
+    _check_against_oracle(scs, _run_batch(scs, 1), 1)

@@ -67,3 +67,19 @@ def test_static_step_rejects_oversize():
     with pytest.raises(pbfx.PbfxError) as e:
         pbfx.step_static(sc.params, sc.box_hi, x, v, sc.n, sc.area_lo, sc.area_hi, sc.iters)
     assert e.value.status == pbfx.PBFX_CAPACITY
+
+
+def test_static_step_with_big_cells_matches_context_step():
+    """The allocation-free step on piles that the wrapping hash puts into one
+    cell and a pile of identical points: the big cells' groups and boxes
+    (k_big_seg) live in the static array too, after a call of another size."""
+    from test_gpu_aliased_piles import aliased_scene
+
+    small = scenes.small_dam(12, 10, 8)
+    pile = aliased_scene(far=1500, near=900, point=300, seed=13)
+    for sc in (small, pile):
+        print(f"static step: {sc.name} n={sc.n}", flush=True)
+        sx, sv = _static_step(sc)
+        cx, cv = _context_step(sc)
+        assert np.array_equal(sx.view(np.uint32), cx.view(np.uint32)), sc.name
+        assert np.array_equal(sv.view(np.uint32), cv.view(np.uint32)), sc.name

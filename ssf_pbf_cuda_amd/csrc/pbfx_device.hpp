@@ -260,31 +260,34 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ nl, uint3
 
 // 1.f / sqrtf(r2) with both operations correctly rounded -- the IEEE
 // result -- in about a third of the instructions of the IEEE sequences.  On
-// [2^-96, FLT_MAX]: v_sqrt_f32, then the two-sided fma test of its
-// neighbours (s - 1 ulp is taken when fma(-(s - ulp), s, r2) <= 0, s + 1 ulp
-// when fma(-(s + ulp), s, r2) > 0), then v_rcp_f32 and one Newton step
+// [2^-96, FLT_MAX] (normal_r2): v_sqrt_f32, then the two-sided fma test of
+// its neighbours (s - 1 ulp is taken when fma(-(s - ulp), s, r2) <= 0, s + 1
+// ulp when fma(-(s + ulp), s, r2) > 0), then v_rcp_f32 and one Newton step
 // y + y * fma(-s, y, 1).  Outside that range (0, below 2^-96, inf, NaN) the
 // IEEE operations themselves.  pbfx_selftest_inv_sqrt compares this function
 // with the IEEE form on every float of the range.
+__device__ __forceinline__ bool normal_r2(float r2) { return r2 >= 0x1p-96f && r2 <= 0x1.fffffep+127f; }
+
+__device__ __forceinline__ float inv_sqrt_normal(float r2) {
+    const float s0 = __builtin_amdgcn_sqrtf(r2);
+    const uint32_t si = __float_as_uint(s0);
+    const float sd = __uint_as_float(si - 1u), su = __uint_as_float(si + 1u);
+    const float vd = __builtin_fmaf(-sd, s0, r2), vu = __builtin_fmaf(-su, s0, r2);
+    float s = (vd <= 0.f) ? sd : s0;
+    s = (vu > 0.f) ? su : s;
+    const float y = __builtin_amdgcn_rcpf(s);
+    return __builtin_fmaf(__builtin_fmaf(-s, y, 1.f), y, y);
+}
+
 __device__ __forceinline__ float inv_sqrt(float r2) {
-    if (__builtin_expect(r2 >= 0x1p-96f && r2 <= 0x1.fffffep+127f, 1)) {
-        const float s0 = __builtin_amdgcn_sqrtf(r2);
-        const uint32_t si = __float_as_uint(s0);
-        const float sd = __uint_as_float(si - 1u), su = __uint_as_float(si + 1u);
-        const float vd = __builtin_fmaf(-sd, s0, r2), vu = __builtin_fmaf(-su, s0, r2);
-        float s = (vd <= 0.f) ? sd : s0;
-        s = (vu > 0.f) ? su : s;
-        const float y = __builtin_amdgcn_rcpf(s);
-        return __builtin_fmaf(__builtin_fmaf(-s, y, 1.f), y, y);
-    }
+    if (__builtin_expect(normal_r2(r2), 1)) return inv_sqrt_normal(r2);
     return 1.f / __builtin_sqrtf(r2);
 }
 
 // pbf/solver/cuda/pbf_kernel_memorization.cu:15-56 (memorizeKernelPair):
-// W and grad W of one listed pair (rsqrtf restated as 1/sqrtf).
-__device__ __forceinline__ void pair_terms(F3 self, F3 pj, const Phys& P, float& W, F3& kg) {
-    F3 d = self - pj;
-    float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+// W and grad W of one listed pair from d = self - x_j and r2 = |d|^2
+// (rsqrtf restated as 1/sqrtf), for any r2.
+__device__ __forceinline__ void pair_core(F3 d, float r2, const Phys& P, float& W, F3& kg) {
     float inv_r = inv_sqrt(r2);
     float q2 = r2 * P.inv_h * P.inv_h;
     W = 0.f;
@@ -298,6 +301,33 @@ __device__ __forceinline__ void pair_terms(F3 self, F3 pj, const Phys& P, float&
     if (P.inv_h <= inv_r) kd = P.ad * u * u;
     kg = mk3(0.f, 0.f, 0.f);
     if (r2 > 0.f) kg = mulsv(kd, mulvs(d, inv_r));
+}
+
+// pair_core for r2 in [2^-96, FLT_MAX], with the same results: the r2 > 0
+// test always holds there, and t = max(1 - q2, 0) is 1 - q2 when q2 <= 1 and
+// exactly 0 otherwise (q2 finite), so W = a t^3 needs no select.
+__device__ __forceinline__ void pair_core_normal(F3 d, float r2, const Phys& P, float& W, F3& kg) {
+    const float inv_r = inv_sqrt_normal(r2);
+    const float q2 = r2 * P.inv_h * P.inv_h;
+    const float t = __builtin_fmaxf(1.f - q2, 0.f);
+    W = P.a * t * t * t;
+    const float r = r2 * inv_r;
+    const float u = __builtin_fmaf(-r, P.inv_h, 1.f);
+    float kd = 0.f;
+    if (P.inv_h <= inv_r) kd = P.ad * u * u;
+    kg = mulsv(kd, mulvs(d, inv_r));
+}
+
+// W and grad W of the pair (self, pj).  A wave whose active lanes all have
+// r2 in [2^-96, FLT_MAX] -- every pair but coincident particles and
+// non-finite positions -- takes pair_core_normal.
+__device__ __forceinline__ void pair_terms(F3 self, F3 pj, const Phys& P, float& W, F3& kg) {
+    const F3 d = self - pj;
+    const float r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(!normal_r2(r2)) == 0ull, 1))
+        pair_core_normal(d, r2, P, W, kg);
+    else
+        pair_core(d, r2, P, W, kg);
 }
 
 // pbf/solver/cuda/boundary/pbf_plane_boundary.cu:6-45

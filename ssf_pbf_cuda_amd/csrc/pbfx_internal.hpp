@@ -15,7 +15,9 @@
 //   tsum, tflag, tprefix           u32[nT+1]     per-4096-cell-tile totals, occupied flags, scan
 //   units                          uint2[cap+]   work items (first cell key, width | chunk << 5)
 //   ckey, crank, cperm, ccount, coffset, ctsum, ctprefix, cunits
-//                                                 the same sort keyed by each particle's final cell
+//                                                 the same sort keyed by each particle's final cell;
+//                                                 once placed, crank holds the queue of final-order
+//                                                 positions whose list build k_final_heavy does
 //   flist, fcnt                    u32[64*cap], u32[cap]  in-range pairs of the final passes by
 //                                                 final-order position (groups of four, list_at), counts
 //   bslot, aux                     u32[cap]      big cell -> descriptor (at its first slot); its
@@ -79,6 +81,12 @@ constexpr uint32_t kFinalK = 64;          // stored in-range pairs per particle 
 // (kOvfBlocks workgroups); the rest sum the cells one after another.
 constexpr uint32_t kOvfMax = 65536;
 constexpr uint32_t kOvfBlocks = 2048;
+// A final-list build from global memory whose next cell would take it past
+// kHeavyScan candidates (a particle beside a pile) is handed to
+// k_final_heavy, which scans it with a whole wave (kHeavyBlocks workgroups of
+// 16 waves).
+constexpr uint32_t kHeavyScan = 2048;
+constexpr uint32_t kHeavyBlocks = 256;
 
 enum Ctr : int {
     CTR_UNITS = 0,
@@ -88,7 +96,7 @@ enum Ctr : int {
     CTR_XSPHNAN,
     CTR_SLOWUNITS,
     CTR_SLOWPART,
-    CTR_RESERVED7,
+    CTR_HEAVY,      // final-list builds handed to k_final_heavy (allocation counter of the queue in crank)
     CTR_MAXTILE,
     CTR_CUNITS,
     CTR_LISTOVF,

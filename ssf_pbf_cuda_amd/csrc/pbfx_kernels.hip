@@ -1302,10 +1302,11 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
 // gives x*_j.  kBig (the step has big cells): a big cell's groups out of
 // range are skipped by their final boxes, one wholly in range is listed
 // without tests, and the scan stops at once when such groups alone take the
-// list to lim.  Stops once n reaches lim.
+// list to lim.  Stops once n reaches lim.  Returns false, leaving the list
+// unfinished, when `budget` groups of four candidates were tested first.
 template <bool kBig, class PosF, class EmitF>
-__device__ __forceinline__ void list_global(const Grid& G, CellView b, F3 self, float cut2, uint32_t& n,
-                                            uint32_t lim, PosF&& pos, EmitF&& emit) {
+__device__ __forceinline__ bool list_global(const Grid& G, CellView b, F3 self, float cut2, uint32_t& n,
+                                            uint32_t lim, uint32_t budget, PosF&& pos, EmitF&& emit) {
     const uint32_t* __restrict__ offset = b.offset;
     int gp[3];
     grid_pos(self, G.cw, gp);
@@ -1327,6 +1328,8 @@ __device__ __forceinline__ void list_global(const Grid& G, CellView b, F3 self, 
             bool first = true;
             if (!kBig) {
                 // a step without big cells: the plain scan
+                if (en[dx] - st[dx] > budget) return false;
+                budget -= en[dx] - st[dx];
                 for (uint32_t j = st[dx]; j < en[dx]; j += 4u) {
                     F3 pj[4];
 #pragma unroll
@@ -1342,7 +1345,7 @@ __device__ __forceinline__ void list_global(const Grid& G, CellView b, F3 self, 
                         emit(j + i, first);
                         first = false;
                     }
-                    if (full || n == lim) return;
+                    if (full || n == lim) return true;
                 }
                 continue;
             }
@@ -1354,7 +1357,7 @@ __device__ __forceinline__ void list_global(const Grid& G, CellView b, F3 self, 
                 if (bp.how == 0u) continue;
                 if (n + bp.inside >= lim) {  // the list reaches lim here: no need to store it
                     n = lim;
-                    return;
+                    return true;
                 }
                 if (bp.all_in) {  // fewer than lim - n members, all pairs
                     for (uint32_t q = bp.p0; q < bp.p1; ++q) {
@@ -1370,6 +1373,8 @@ __device__ __forceinline__ void list_global(const Grid& G, CellView b, F3 self, 
                 }
             }
             if (p1 == p0) continue;
+            if (p1 - p0 > budget) return false;
+            budget -= p1 - p0;
             const uint32_t* __restrict__ ax = b.aux + o0;
             uint32_t jn[4];
 #pragma unroll
@@ -1393,7 +1398,7 @@ __device__ __forceinline__ void list_global(const Grid& G, CellView b, F3 self, 
                     emit(pick4(jj, i), first);
                     first = false;
                 }
-                if (full || n == lim) return;
+                if (full || n == lim) return true;
             }
         }
 #pragma unroll
@@ -1402,6 +1407,7 @@ __device__ __forceinline__ void list_global(const Grid& G, CellView b, F3 self, 
             en[dx] = ne[dx];
         }
     }
+    return true;
 }
 
 // A candidate's records for a pass's term: p = (x*_j, pass-specific w),
@@ -1542,6 +1548,19 @@ __device__ __forceinline__ uint32_t ovf_entry(const Bufs& b, uint32_t q) {
     return kFinalK + 2u + k;
 }
 
+// Queues final-order position q for k_final_heavy (wave-aggregated append
+// to crank, dead once the regrouping has placed every particle).  A position
+// is queued at most once per step, so the queue never exceeds cap.
+__device__ __forceinline__ void heavy_push(const Bufs& b, uint32_t q) {
+    const uint64_t m = __ballot(true);
+    const uint32_t lane = lane_id(), leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&b.ctr[CTR_HEAVY], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (k < b.cap) b.crank[k] = q;
+}
+
 // ------------------------------------------------- K9: final pair lists
 // For each particle (by position q of the final-cell order) the slots of its
 // in-range pairs in groups of four (list_at(cap, q, k), k < kFinalK); fcnt[q] =
@@ -1579,11 +1598,19 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
             }
             ++n;
         };
-        if (tiled) list_tile(I, S, tp, final_column(G, I, self), self, cut2, n, kFinalK + 1u, emit);
+        bool done = true;
+        if (tiled)
+            list_tile(I, S, tp, final_column(G, I, self), self, cut2, n, kFinalK + 1u, emit);
         else if (any_big)
-            list_global<true>(G, cv, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(Pf[j]); }, emit);
+            done = list_global<true>(G, cv, self, cut2, n, kFinalK + 1u, kHeavyScan,
+                                     [&](uint32_t j) { return f3of(Pf[j]); }, emit);
         else
-            list_global<false>(G, cv, self, cut2, n, kFinalK + 1u, [&](uint32_t j) { return f3of(Pf[j]); }, emit);
+            done = list_global<false>(G, cv, self, cut2, n, kFinalK + 1u, kHeavyScan,
+                                      [&](uint32_t j) { return f3of(Pf[j]); }, emit);
+        if (!done) {  // a long scan: k_final_heavy builds this list with a whole wave
+            heavy_push(b, q);
+            return;
+        }
         if (n > kFinalK) n = ovf_entry(b, q);
         b.fcnt[q] = n;
         tl.listovf += n > kFinalK ? 1u : 0u;
@@ -1637,6 +1664,96 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         tl.slowpart++;
         by_scene<kB>(kB ? b.sid[g] : 0u, true, [&](uint32_t sc) { build(q, g, false, SC[sc].G, cut2_of(sc)); });
     });
+    flush_tally(b.ctr, tl);
+}
+
+// The final list of one queued particle (x* = self, final-order position q),
+// built by a whole wave: each cell's candidates (the same ranges and group
+// skips as list_global) 64 at a time, lane l testing the l-th; the hits are
+// appended in slot order through a ballot.  Returns the pair count, or lim
+// once it reaches lim = kFinalK + 1 (the list then overflows and is not
+// read).  Called by all lanes of the wave; every value it depends on is the
+// same on every lane.
+template <bool kBig>
+__device__ uint32_t heavy_list(const Grid& G, CellView b, const float4* __restrict__ Pf, F3 self, float cut2,
+                               uint32_t* __restrict__ fl, uint32_t cap, uint32_t q) {
+    const uint32_t lim = kFinalK + 1u;
+    const uint32_t lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    int gp[3];
+    grid_pos(self, G.cw, gp);
+    uint32_t n = 0;
+    for (int c = 0; c < 27; ++c) {
+        const int r = c / 3, dx = c % 3;
+        const uint32_t key = scene_key(gp[0] + dx - 1, gp[1] + (r % 3) - 1, gp[2] + (r / 3) - 1, G);
+        const uint32_t o0 = b.offset[key], ncell = b.offset[key + 1] - o0;
+        uint32_t p0 = 0, p1 = ncell;
+        bool via = false;
+        if (kBig && ncell > kSmallCell) {
+            const BigPart bp = big_part<true>(b.bseg + b.bslot[o0], self, cut2, ncell);
+            if (bp.how == 0u) continue;
+            if (n + bp.inside >= lim) return lim;
+            if (bp.all_in) {  // fewer than lim - n members, all pairs, ascending
+                const uint32_t size = bp.p1 - bp.p0;
+                for (uint32_t t = lane; t < size; t += 64u) {
+                    const uint32_t j = bp.how == 2u ? b.aux[o0 + bp.p0 + t] : o0 + bp.p0 + t;
+                    if (n + t < kFinalK) fl[list_at(cap, q, n + t)] = j | (t == 0u ? 0x80000000u : 0u);
+                }
+                n += size;
+                continue;
+            }
+            if (aux_pays(bp, ncell)) {
+                p0 = bp.p0;
+                p1 = bp.p1;
+                via = true;
+            }
+        }
+        bool any = false;  // a pair of this cell was appended
+        for (uint32_t p = p0; p < p1; p += 64u) {
+            const uint32_t t = p + lane;
+            uint32_t j = 0;
+            bool hit = false;
+            if (t < p1) {
+                j = via ? b.aux[o0 + t] : o0 + t;
+                hit = pair_test<true>(f3of(Pf[j]) - self, cut2);
+            }
+            const uint64_t m = __ballot(hit);
+            const uint32_t k = n + (uint32_t)__popcll(m & below);
+            if (hit && k < kFinalK) fl[list_at(cap, q, k)] = j | ((!any && (m & below) == 0ull) ? 0x80000000u : 0u);
+            n += (uint32_t)__popcll(m);
+            any = any || m != 0ull;
+            if (n >= lim) return lim;
+        }
+    }
+    return n;
+}
+
+// The final lists k_final_list queued (builds of more than kHeavyScan
+// candidates): one wave per particle, then its count or
+// overflow entry as k_final_list would have written them.
+template <bool kB>
+__global__ __launch_bounds__(1024) void k_final_heavy(Bufs b, Scenes<kB> SC) {
+    const uint32_t nh = min(b.ctr[CTR_HEAVY], b.cap);
+    if (nh == 0u) return;  // uniform over the grid
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const CellView cv = cell_view(b);
+    const bool any_big = big_cells(b);
+    Tally tl;
+    for (uint32_t h = w0; h < nh; h += waves) {
+        const uint32_t q = __builtin_amdgcn_readfirstlane(b.crank[h]);
+        const uint32_t g = __builtin_amdgcn_readfirstlane(b.cperm[q]);
+        const uint32_t sc = kB ? b.sid[g] : 0u;
+        const Grid G = SC[sc].G;
+        const float cut2 = fmaxf(SC[sc].P.cut2_w, SC[sc].P.cut2_wd);
+        const F3 self = f3of(b.Pf[g]);
+        const uint32_t n = any_big ? heavy_list<true>(G, cv, b.Pf, self, cut2, b.flist, b.cap, q)
+                                   : heavy_list<false>(G, cv, b.Pf, self, cut2, b.flist, b.cap, q);
+        if (lane_id() == 0u) {
+            b.fcnt[q] = n > kFinalK ? ovf_entry(b, q) : n;
+            tl.listovf += n > kFinalK ? 1u : 0u;
+        }
+    }
     flush_tally(b.ctr, tl);
 }
 
@@ -2193,6 +2310,7 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
         Mark m(tm, s, K_FINAL_LIST);
         const uint32_t ft = final_tile(T);
         hipLaunchKernelGGL(k_final_list<kB>, grid(K_FINAL_LIST), dim3(kBlock), smem_of(ft), s, b, SC, ft, lguw, lgnc);
+        hipLaunchKernelGGL(k_final_heavy<kB>, dim3(kHeavyBlocks), dim3(1024), 0, s, b, SC);
     }
     // the overflow positions' per-cell partials, before each pass that adds them
     const dim3 ovf_grid(kOvfBlocks);

@@ -100,7 +100,8 @@ class KernelTime(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "num_in", "num_live", "truncated", "sweep_cap_hits", "max_sweeps_used", "xsph_nan", "units", "slow_units",
-        "slow_particles", "tile_overflow", "max_tile", "steps", "final_units", "list_overflow", "scan_tiles")]
+        "slow_particles", "tile_overflow", "max_tile", "steps", "final_units", "list_overflow", "scan_tiles",
+        "heavy_builds")]
 
 
 _lib = None

@@ -67,6 +67,24 @@ def test_aliased_piles_bit_exact(tile_cap):
     assert bx == 0 and bv == 0
 
 
+def test_heavy_final_builds_bit_exact():
+    """Particles beside the piles have final-list builds of thousands of
+    candidates from global memory; those are handed to a wave each
+    (k_final_heavy).  Their lists, counts and overflow entries must give the
+    oracle's state bit for bit."""
+    sc = aliased_scene(far=3000, near=2000, point=500, seed=15)
+    print(f"heavy builds: {sc.n} particles", flush=True)
+    got = gpu_run(sc)
+    st = got["stats"]
+    print(f"gpu stats: {st}", flush=True)
+    ref = oracle_run(sc)
+    assert st["heavy_builds"] > 0
+    assert got["num"] == ref["num"]
+    bx, bv = bit_mismatches(got["x"], ref["x"]), bit_mismatches(got["v"], ref["v"])
+    print(f"heavy builds step: bitdiff_x={bx} bitdiff_v={bv}", flush=True)
+    assert bx == 0 and bv == 0
+
+
 def test_aliased_piles_two_steps_bit_exact():
     sc = aliased_scene(far=1500, near=900, point=300, seed=12)
     print(f"aliased piles, 2 steps: {sc.n} particles", flush=True)

@@ -86,6 +86,7 @@ constexpr uint32_t kOvfBlocks = 2048;
 // k_final_heavy, which scans it with a whole wave (kHeavyBlocks workgroups of
 // 16 waves).
 constexpr uint32_t kHeavyScan = 2048;
+static_assert(kHeavyScan >= 27u * kSmallCell, "a build without big cells never reaches the budget");
 constexpr uint32_t kHeavyBlocks = 256;
 
 enum Ctr : int {

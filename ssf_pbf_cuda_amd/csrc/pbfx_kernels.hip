@@ -1302,8 +1302,9 @@ __device__ __forceinline__ void list_tile(const UnitInfo& I, const UnitShared& S
 // gives x*_j.  kBig (the step has big cells): a big cell's groups out of
 // range are skipped by their final boxes, one wholly in range is listed
 // without tests, and the scan stops at once when such groups alone take the
-// list to lim.  Stops once n reaches lim.  Returns false, leaving the list
-// unfinished, when `budget` groups of four candidates were tested first.
+// list to lim.  Stops once n reaches lim.  kBig: returns false, leaving the
+// list unfinished, before a cell that would take the candidates tested past
+// `budget` (at least 27 * kSmallCell).
 template <bool kBig, class PosF, class EmitF>
 __device__ __forceinline__ bool list_global(const Grid& G, CellView b, F3 self, float cut2, uint32_t& n,
                                             uint32_t lim, uint32_t budget, PosF&& pos, EmitF&& emit) {
@@ -1327,9 +1328,8 @@ __device__ __forceinline__ bool list_global(const Grid& G, CellView b, F3 self, 
         for (int dx = 0; dx < 3; ++dx) {
             bool first = true;
             if (!kBig) {
-                // a step without big cells: the plain scan
-                if (en[dx] - st[dx] > budget) return false;
-                budget -= en[dx] - st[dx];
+                // a step without big cells: the plain scan (27 cells of at
+                // most kSmallCell particles stay within any budget)
                 for (uint32_t j = st[dx]; j < en[dx]; j += 4u) {
                     F3 pj[4];
 #pragma unroll
@@ -1598,18 +1598,17 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
             }
             ++n;
         };
-        bool done = true;
-        if (tiled)
+        if (tiled) {
             list_tile(I, S, tp, final_column(G, I, self), self, cut2, n, kFinalK + 1u, emit);
-        else if (any_big)
-            done = list_global<true>(G, cv, self, cut2, n, kFinalK + 1u, kHeavyScan,
-                                     [&](uint32_t j) { return f3of(Pf[j]); }, emit);
-        else
-            done = list_global<false>(G, cv, self, cut2, n, kFinalK + 1u, kHeavyScan,
-                                      [&](uint32_t j) { return f3of(Pf[j]); }, emit);
-        if (!done) {  // a long scan: k_final_heavy builds this list with a whole wave
-            heavy_push(b, q);
-            return;
+        } else if (any_big) {
+            if (!list_global<true>(G, cv, self, cut2, n, kFinalK + 1u, kHeavyScan,
+                                   [&](uint32_t j) { return f3of(Pf[j]); }, emit)) {
+                heavy_push(b, q);  // a long scan: k_final_heavy builds this list with a whole wave
+                return;
+            }
+        } else {
+            list_global<false>(G, cv, self, cut2, n, kFinalK + 1u, kHeavyScan, [&](uint32_t j) { return f3of(Pf[j]); },
+                               emit);
         }
         if (n > kFinalK) n = ovf_entry(b, q);
         b.fcnt[q] = n;

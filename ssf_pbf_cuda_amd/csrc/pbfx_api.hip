@@ -365,7 +365,7 @@ pbfx_status read_counters(pbfx_ctx* c, hipStream_t s) {
     uint32_t* h = c->h_pinned;
     for (int k = 0; k < CTR_NUM; ++k) {
         if (k == CTR_UNITS || k == CTR_CUNITS || k == CTR_BIGCELLS || k == CTR_HUGECELLS || k == CTR_OVF ||
-            k == CTR_HEAVY)
+            k == CTR_HEAVY || k == CTR_NSHEAVY)
             continue;
         const bool is_max = (k == CTR_MAXTILE || k == CTR_MAXSWEEP);
         uint32_t v = 0;
@@ -880,7 +880,7 @@ pbfx_status pbfx_get_stats(pbfx_ctx* c, void* stream, pbfx_stats* out) {
     out->final_units = h[CTR_CUNITS];
     out->list_overflow = h[CTR_LISTOVF];
     out->scan_tiles = h[CTR_SCANTILES];
-    out->heavy_builds = h[CTR_HEAVY];
+    out->heavy_builds = h[CTR_HEAVY] + h[CTR_NSHEAVY];
     return PBFX_OK;
 }
 

@@ -17,7 +17,9 @@
 //   ckey, crank, cperm, ccount, coffset, ctsum, ctprefix, cunits
 //                                                 the same sort keyed by each particle's final cell;
 //                                                 once placed, crank holds the queue of final-order
-//                                                 positions whose list build k_final_heavy does
+//                                                 positions whose list build k_final_heavy does (and,
+//                                                 between the search and the regrouping, the slots
+//                                                 whose search list k_ns_heavy builds)
 //   flist, fcnt                    u32[64*cap], u32[cap]  in-range pairs of the final passes by
 //                                                 final-order position (groups of four, list_at), counts
 //   bslot, aux                     u32[cap]      big cell -> descriptor (at its first slot); its
@@ -81,10 +83,10 @@ constexpr uint32_t kFinalK = 64;          // stored in-range pairs per particle 
 // (kOvfBlocks workgroups); the rest sum the cells one after another.
 constexpr uint32_t kOvfMax = 65536;
 constexpr uint32_t kOvfBlocks = 2048;
-// A final-list build from global memory whose next cell would take it past
-// kHeavyScan candidates (a particle beside a pile) is handed to
-// k_final_heavy, which scans it with a whole wave (kHeavyBlocks workgroups of
-// 16 waves).
+// A final-list or search-list build from global memory whose next cell would
+// take it past kHeavyScan candidates (a particle beside a pile) is handed to
+// k_final_heavy / k_ns_heavy, which scan it with a whole wave (kHeavyBlocks
+// workgroups of 16 waves).
 constexpr uint32_t kHeavyScan = 2048;
 static_assert(kHeavyScan >= 27u * kSmallCell, "a build without big cells never reaches the budget");
 constexpr uint32_t kHeavyBlocks = 256;
@@ -105,6 +107,7 @@ enum Ctr : int {
     CTR_HUGECELLS,  // cells with more than kSortMax particles (chunk-sorted, then merged)
     CTR_SCANTILES,  // occupied tiles of the cell scan (statistics)
     CTR_OVF,        // final-pass positions whose list overflowed (allocation counter of ovf)
+    CTR_NSHEAVY,    // search lists handed to waves (allocation counter of the queue in crank)
     CTR_NUM = 16
 };
 constexpr uint32_t kStatRep = 64;                           // copies of the statistics counters

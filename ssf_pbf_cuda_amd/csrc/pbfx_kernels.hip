@@ -908,6 +908,22 @@ __device__ __forceinline__ uint32_t own_column(const UnitShared& S, uint32_t W, 
 // sorted slots in groups of four (list_at), with the count in
 // ncnt[slot].  Candidates come from the unit's staged tile, four at a time.
 
+// Appends v to the queue of long list builds counted by ctr[which], a
+// wave-aggregated append to crank (dead from the final passes to the next
+// regrouping: the search's queue is read before it, the final lists' queue
+// after its placement).  A slot or position is queued at most once per step,
+// so a queue never exceeds cap.
+constexpr uint32_t kDeferred = 0xFFFFFFFFu;  // a filter's count when it left its list to a wave
+__device__ __forceinline__ void queue_push(const Bufs& b, uint32_t which, uint32_t v) {
+    const uint64_t m = __ballot(true);
+    const uint32_t lane = lane_id(), leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&b.ctr[which], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (k < b.cap) b.crank[k] = v;
+}
+
 // The filter of one own particle (column kc of the unit, slot g): candidate
 // tile index t is read at src[sb + t] -- the tile itself (kStaged) or the
 // sorted positions, with sb the cell's slot base.  Returns the number kept
@@ -967,11 +983,14 @@ __device__ __forceinline__ uint32_t ns_filter(const UnitShared& S, uint32_t W, u
 // range are skipped, a group wholly in range is listed without tests, and a
 // single group left is scanned through aux when that skips a quarter of the
 // cell or more.  Every scan is one loop whose slots come, per lane, from aux
-// or from the cell's own run, the next group's slots in flight.
+// or from the cell's own run, the next group's slots in flight.  Returns
+// kDeferred, leaving the list unfinished, before a range that would take the
+// candidates tested past `budget` (only a neighbourhood holding a cell of
+// more than kSmallCell particles can: 27 * kSmallCell < kHeavyScan).
 __device__ __forceinline__ uint32_t ns_filter_global(const UnitShared& S, uint32_t W, uint32_t kc, F3 self,
                                                      float h2, uint32_t ncap, const float4* __restrict__ Pf,
                                                      CellView b, uint32_t* __restrict__ nl, uint32_t cap,
-                                                     uint32_t g, bool& trunc) {
+                                                     uint32_t g, uint32_t budget, bool& trunc) {
     uint32_t* __restrict__ dst = nl + list_at(cap, g, 0u);  // where entry cnt goes
     const size_t jump = (size_t)4u * cap - 3u;              // from entry 4q+3 to 4q+4
     uint32_t cnt = 0;
@@ -1004,6 +1023,8 @@ __device__ __forceinline__ uint32_t ns_filter_global(const UnitShared& S, uint32
             }
         }
         if (p1 == p0) continue;
+        if (p1 - p0 > budget) return kDeferred;
+        budget -= p1 - p0;
         const uint32_t* __restrict__ ax = b.aux + o0;
         uint32_t jn[4];
 #pragma unroll
@@ -1072,11 +1093,107 @@ __global__ __launch_bounds__(kBlock) void k_ns(Bufs b, Scenes<kB> SC, uint32_t t
             bool trunc = false;
             const uint32_t cnt =
                 I.staged ? ns_filter<true>(S, I.W, kc, tile[ts], h2, ncap, tile, nl, cap, g, trunc)
-                         : ns_filter_global(S, I.W, kc, f3of(Pf[g]), h2, ncap, Pf, cell_view(b), nl, cap, g, trunc);
-            b.ncnt[g] = cnt;
-            tl.trunc += trunc ? 1u : 0u;
+                         : ns_filter_global(S, I.W, kc, f3of(Pf[g]), h2, ncap, Pf, cell_view(b), nl, cap, g,
+                                            kHeavyScan, trunc);
+            if (cnt == kDeferred) {  // a long scan: k_ns_heavy builds this list with a whole wave
+                queue_push(b, CTR_NSHEAVY, g);
+            } else {
+                b.ncnt[g] = cnt;
+                tl.trunc += trunc ? 1u : 0u;
+            }
         }
         __syncthreads();
+    }
+    flush_tally(b.ctr, tl);
+}
+
+// The search list of one queued slot g (x* = self), built by a whole wave:
+// the 27 cells around its cell in z,y,x order with the same ranges and group
+// skips as ns_filter_global, 64 candidates at a time, lane l testing the
+// l-th, the hits appended in slot order through a ballot; the first ncap
+// are kept and trunc is set when there are more.  Called by all lanes of the
+// wave; every value it depends on is the same on every lane.
+template <bool kBig>
+__device__ uint32_t ns_heavy_list(const Grid& G, CellView b, const float4* __restrict__ Pf, F3 self, float h2,
+                                  uint32_t ncap, uint32_t* __restrict__ nl, uint32_t cap, uint32_t g, bool& trunc) {
+    const uint32_t lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    int gp[3];
+    grid_pos(self, G.cw, gp);
+    uint32_t n = 0;
+    for (int c = 0; c < 27; ++c) {
+        const int r = c / 3, dx = c % 3;
+        const uint32_t key = scene_key(gp[0] + dx - 1, gp[1] + (r % 3) - 1, gp[2] + (r / 3) - 1, G);
+        const uint32_t o0 = b.offset[key], ncell = b.offset[key + 1] - o0;
+        uint32_t p0 = 0, p1 = ncell;
+        bool via = false;
+        if (kBig && ncell > kSmallCell) {
+            const BigPart bp = big_part<false>(b.bseg + b.bslot[o0], self, h2, ncell);
+            if (bp.how == 0u) continue;
+            if (bp.all_in) {  // every member is a pair: the first ones, ascending
+                const uint32_t size = bp.p1 - bp.p0, take = min(ncap - n, size);
+                for (uint32_t t = lane; t < take; t += 64u)
+                    nl[list_at(cap, g, n + t)] = bp.how == 2u ? b.aux[o0 + bp.p0 + t] : o0 + bp.p0 + t;
+                n += take;
+                if (size > take) {
+                    trunc = true;
+                    return n;
+                }
+                continue;
+            }
+            if (aux_pays(bp, ncell)) {
+                p0 = bp.p0;
+                p1 = bp.p1;
+                via = true;
+            }
+        }
+        for (uint32_t p = p0; p < p1; p += 64u) {
+            const uint32_t t = p + lane;
+            uint32_t j = 0;
+            bool hit = false;
+            if (t < p1) {
+                j = via ? b.aux[o0 + t] : o0 + t;
+                hit = pair_test<false>(self - f3of(Pf[j]), h2);
+            }
+            const uint64_t m = __ballot(hit);
+            const uint32_t k = n + (uint32_t)__popcll(m & below);
+            if (hit && k < ncap) nl[list_at(cap, g, k)] = j;
+            n += (uint32_t)__popcll(m);
+            if (n > ncap) {
+                trunc = true;
+                return ncap;
+            }
+        }
+    }
+    return n;
+}
+
+// The search lists k_ns queued (global-memory filters that would test more
+// than kHeavyScan candidates): one wave per slot, then its count and
+// truncation tally as k_ns would have written them.
+template <bool kB>
+__global__ __launch_bounds__(1024) void k_ns_heavy(Bufs b, Scenes<kB> SC) {
+    const uint32_t nh = min(b.ctr[CTR_NSHEAVY], b.cap);
+    if (nh == 0u) return;  // uniform over the grid
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const CellView cv = cell_view(b);
+    const bool any_big = big_cells(b);
+    Tally tl;
+    for (uint32_t h = w0; h < nh; h += waves) {
+        const uint32_t g = __builtin_amdgcn_readfirstlane(b.crank[h]);
+        const uint32_t sc = kB ? __builtin_amdgcn_readfirstlane(b.sid[g]) : 0u;
+        const Grid G = SC[sc].G;
+        const float h2 = SC[sc].P.h2;
+        const uint32_t ncap = (uint32_t)SC[sc].P.cap;
+        const F3 self = f3of(b.Pf[g]);
+        bool trunc = false;
+        const uint32_t n = any_big ? ns_heavy_list<true>(G, cv, b.Pf, self, h2, ncap, b.nlist, b.cap, g, trunc)
+                                   : ns_heavy_list<false>(G, cv, b.Pf, self, h2, ncap, b.nlist, b.cap, g, trunc);
+        if (lane_id() == 0u) {
+            b.ncnt[g] = n;
+            tl.trunc += trunc ? 1u : 0u;
+        }
     }
     flush_tally(b.ctr, tl);
 }
@@ -1548,19 +1665,6 @@ __device__ __forceinline__ uint32_t ovf_entry(const Bufs& b, uint32_t q) {
     return kFinalK + 2u + k;
 }
 
-// Queues final-order position q for k_final_heavy (wave-aggregated append
-// to crank, dead once the regrouping has placed every particle).  A position
-// is queued at most once per step, so the queue never exceeds cap.
-__device__ __forceinline__ void heavy_push(const Bufs& b, uint32_t q) {
-    const uint64_t m = __ballot(true);
-    const uint32_t lane = lane_id(), leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&b.ctr[CTR_HEAVY], (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, (int)leader, 64);
-    const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (k < b.cap) b.crank[k] = q;
-}
-
 // ------------------------------------------------- K9: final pair lists
 // For each particle (by position q of the final-cell order) the slots of its
 // in-range pairs in groups of four (list_at(cap, q, k), k < kFinalK); fcnt[q] =
@@ -1603,7 +1707,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
         } else if (any_big) {
             if (!list_global<true>(G, cv, self, cut2, n, kFinalK + 1u, kHeavyScan,
                                    [&](uint32_t j) { return f3of(Pf[j]); }, emit)) {
-                heavy_push(b, q);  // a long scan: k_final_heavy builds this list with a whole wave
+                queue_push(b, CTR_HEAVY, q);  // a long scan: k_final_heavy builds this list with a whole wave
                 return;
             }
         } else {
@@ -2265,6 +2369,7 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
     {
         Mark m(tm, s, K_NS);
         hipLaunchKernelGGL(k_ns<kB>, grid(K_NS), dim3(kBlock), smem, s, b, SC, T.tile_cap);
+        hipLaunchKernelGGL(k_ns_heavy<kB>, dim3(kHeavyBlocks), dim3(1024), 0, s, b, SC);
     }
     const bool pow4 = !kB && SC.s0.P.sn == 4;  // a batch's scenes may differ: general powi
     for (int it = 0; it < iters; ++it) {

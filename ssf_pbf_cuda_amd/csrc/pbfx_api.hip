@@ -243,7 +243,7 @@ void free_bufs(pbfx_ctx* c) {
                     b.OM4,   b.ncnt,    b.nlist,   b.count, b.offset, b.tsum,  b.tprefix, b.units,
                     b.ckey,  b.crank,   b.cperm,   b.ccount, b.coffset, b.ctsum, b.ctprefix, b.cunits,
                     b.tflag, b.ctflag,  b.flist,   b.fcnt,  b.isid,   b.sid,   b.ctr,     b.big,
-                    b.ovf,   b.part,    b.bslot,   b.aux,   b.bseg,
+                    b.ovf,   b.oslot,   b.part,    b.bslot, b.aux,    b.bseg,
                     c->d_scenes, c->d_kbase, c->d_in_off, c->d_scene_off, c->d_default_sphere};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -493,9 +493,10 @@ pbfx_status create_common(pbfx_ctx* c) {
     PBFX_ALLOC(b.cunits, items);
     PBFX_ALLOC(b.flist, (size_t)kFinalK * bc);
     PBFX_ALLOC(b.fcnt, bc);
-    b.ovf_max = b.cap < kOvfMax ? b.cap : kOvfMax;
+    b.ovf_max = b.cap / 64u + 1u;  // every block of 64 positions can take a slot
     PBFX_ALLOC(b.ovf, b.ovf_max);
-    PBFX_ALLOC(b.part, 27 * (size_t)b.ovf_max);
+    PBFX_ALLOC(b.oslot, (size_t)b.cap / 64u + 1u);
+    PBFX_ALLOC(b.part, (size_t)b.ovf_max * 27u * 64u);
     PBFX_ALLOC(b.tflag, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctflag, (size_t)b.nT + 1);
     PBFX_ALLOC(b.ctr, kCtrWords);
@@ -1393,8 +1394,9 @@ static __device__ __align__(256) unsigned char scratch[PBFX_STATIC_SCRATCH_BYTES
 namespace {
 
 constexpr int kMaxDevices = 64;
-// overflow entries of the allocation-free step (their partials take 432 B each)
-constexpr uint32_t kStaticOvfMax = 4096;
+// overflow slots of the allocation-free step (64 positions each, their
+// partials 27 KiB a slot)
+constexpr uint32_t kStaticOvfSlots = 64;
 
 struct StaticDev {
     bool ready = false;
@@ -1473,9 +1475,10 @@ size_t carve_static(Bufs& b, uintptr_t base, uint32_t cap, uint32_t CT, uint32_t
     take(b.cunits, items);
     take(b.flist, (size_t)kFinalK * bc);
     take(b.fcnt, bc);
-    b.ovf_max = cap < kStaticOvfMax ? cap : kStaticOvfMax;
+    b.ovf_max = cap / 64u + 1u < kStaticOvfSlots ? cap / 64u + 1u : kStaticOvfSlots;
     take(b.ovf, b.ovf_max);
-    take(b.part, 27 * (size_t)b.ovf_max);
+    take(b.oslot, (size_t)cap / 64u + 1u);
+    take(b.part, (size_t)b.ovf_max * 27u * 64u);
     take(b.ctr, kCtrWords);
     return off;
 }

@@ -26,8 +26,10 @@
 //                                                 slots grouped by image.  After the sort, `rank`
 //                                                 holds each slot's group mark (0: not in a big cell)
 //   bseg                           BigSeg[big_list_len(cap)]  groups and bounding boxes of big cells
-//   ovf, part                      u32[M], f32x4[27*M]  overflow positions and their per-cell
-//                                                 partial sums (M = min(cap, kOvfMax))
+//   ovf, oslot, part               u32[M], u32[cap/64+1], f32x4[M*27*64]  overflow slots:
+//                                                 each slot's block of 64 final-order positions,
+//                                                 each block's slot, and the slots' per-cell partial
+//                                                 sums (M = cap/64 + 1 in a context)
 //   ctr                            u32[16*65]    allocation counters, then 64 copies of the
 //                                                 statistics counters of the last step
 #pragma once
@@ -78,11 +80,17 @@ struct BigSeg {
     uint4 fmax[kMaxSeg];      // member is non-finite; gathered by atomics
 };
 constexpr uint32_t kFinalK = 64;          // stored in-range pairs per particle for the final passes
-// Final-pass particles with more than kFinalK pairs (piles): up to kOvfMax of
-// them per step get their 27 per-cell partial sums computed in parallel
-// (kOvfBlocks workgroups); the rest sum the cells one after another.
-constexpr uint32_t kOvfMax = 65536;
+// Final-pass particles with more than kFinalK pairs (piles) get their 27
+// per-cell partial sums computed in parallel before each pass (kOvfBlocks
+// workgroups), by block of 64 consecutive final-order positions: a block
+// holding such a particle takes one of ovf_max slots (every block can have one
+// in a context; the allocation-free step has kStaticOvfSlots), and its
+// partials are stored slot by slot, cell by cell, 64 lanes a row.  A block
+// that finds no slot left sums its cells one after another in the pass.
 constexpr uint32_t kOvfBlocks = 2048;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;       // oslot: no particle of the block overflowed (yet)
+constexpr uint32_t kPendingSlot = 0xFFFFFFFEu;  // oslot: being allocated (never seen across kernels)
+constexpr uint32_t kFullSlot = 0xFFFFFFFDu;     // oslot: no slot was left
 // A final-list or search-list build from global memory whose next cell would
 // take it past kHeavyScan candidates (a particle beside a pile) is handed to
 // k_final_heavy / k_ns_heavy, which scan it with a whole wave (kHeavyBlocks
@@ -204,9 +212,10 @@ struct Bufs {
     uint32_t* ctprefix;
     uint2* cunits;
     uint32_t* flist;   // final pair lists, kFinalK entries per position in groups of four (list_at)
-    uint32_t* fcnt;    // pairs per final-order position (kFinalK + 1: direct sum; kFinalK + 2 + k: overflow entry k)
-    uint32_t* ovf;     // [ovf_max] final-order positions of the overflow entries
-    float4* part;      // [27 * ovf_max] their per-cell partial sums, cell-major
+    uint32_t* fcnt;    // pairs per final-order position (kFinalK + 1: more, see the block's slot)
+    uint32_t* ovf;     // [ovf_max] the block (final-order position / 64) of each overflow slot
+    uint32_t* oslot;   // [cap / 64 + 1] each block's overflow slot (kNoSlot, kFullSlot)
+    float4* part;      // [ovf_max * 27 * 64] the slots' per-cell partial sums: (slot, cell, lane)
     uint32_t ovf_max;
     uint32_t* ctr;
     uint32_t lgT;

@@ -1331,16 +1331,19 @@ __global__ __launch_bounds__(256) void k_cur_hash(Bufs b, Scenes<kB> SC, uint32_
 }
 
 // Places each particle at its run's position and emits the work items of
-// the runs holding at least kSparse particles (one per kChunk of them).
+// the runs holding at least kSparse particles (one per kChunk of them);
+// clears the overflow slots of the final-order blocks of 64.
 __global__ __launch_bounds__(256) void k_cur_scatter(const uint32_t* __restrict__ offset, uint32_t CT,
                                                      const uint32_t* __restrict__ ckey,
                                                      const uint32_t* __restrict__ crank,
                                                      const uint32_t* __restrict__ coffset,
                                                      uint32_t* __restrict__ cperm, uint32_t uw, uint32_t lguw,
                                                      uint32_t lgnc, uint2* __restrict__ cunits,
-                                                     uint32_t* __restrict__ ctr) {
+                                                     uint32_t* __restrict__ ctr, uint32_t* __restrict__ oslot) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= offset[CT]) return;
+    const uint32_t nl = offset[CT];
+    if (s >= nl) return;
+    if (s <= (nl - 1u) / 64u) oslot[s] = kNoSlot;  // the final passes' blocks of 64 positions
     const uint32_t k = ckey[s];
     const uint32_t slot = coffset[k] + crank[s];
     cperm[slot] = s;
@@ -1645,24 +1648,26 @@ __device__ __forceinline__ uint32_t final_column(const Grid& G, const UnitInfo& 
     return (((uint32_t)gp[0] & (G.gx - 1u)) - I.xb) + 1u;
 }
 
-// ------------------------------------------------- overflow lists
-// A final-order position whose list overflowed is listed in ovf, up to
-// ovf_max of them, as its build ends (one atomic per group of lanes that end
-// together); its count becomes kFinalK + 2 + its index, so that the passes
-// add its per-cell partials, computed for all 27 cells at once by
-// k_ovf_part, instead of summing the cells one after another.  Past ovf_max
-// it keeps kFinalK + 1 (the direct sum).  Called by the lanes whose list
-// overflowed (any subset of the wave).
-__device__ __forceinline__ uint32_t ovf_entry(const Bufs& b, uint32_t q) {
+// ------------------------------------------------- overflow slots
+// A final-order position whose list overflowed keeps the count kFinalK + 1;
+// the first such position of each block of 64 takes the block's overflow
+// slot (oslot, one atomic per block; ovf lists the slots' blocks), so that
+// the passes add its per-cell partials, computed for all 27 cells at once by
+// k_ovf_part, instead of summing the cells one after another (which they do
+// when no slot was left).  Called by the lanes whose list overflowed (any
+// subset of the wave); a wave's positions are consecutive, so its lanes of
+// one block are contiguous and only the first of them competes for the slot.
+__device__ __forceinline__ uint32_t ovf_mark(const Bufs& b, uint32_t q) {
     const uint64_t m = __ballot(true);
-    const uint32_t lane = lane_id(), leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&b.ctr[CTR_OVF], (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, (int)leader, 64);
-    const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (k >= b.ovf_max) return kFinalK + 1u;
-    b.ovf[k] = q;
-    return kFinalK + 2u + k;
+    const uint32_t lane = lane_id(), blk = q >> 6;
+    const uint64_t prev = m & ((1ull << lane) - 1ull);
+    const uint32_t pb = prev ? (uint32_t)__shfl((int)blk, 63 - __builtin_clzll(prev), 64) : kNoSlot;
+    if (pb != blk && atomicCAS(&b.oslot[blk], kNoSlot, kPendingSlot) == kNoSlot) {
+        const uint32_t k = atomicAdd(&b.ctr[CTR_OVF], 1u);
+        if (k < b.ovf_max) b.ovf[k] = blk;
+        b.oslot[blk] = k < b.ovf_max ? k : kFullSlot;
+    }
+    return kFinalK + 1u;
 }
 
 // ------------------------------------------------- K9: final pair lists
@@ -1714,7 +1719,7 @@ __global__ __launch_bounds__(kBlock) void k_final_list(Bufs b, Scenes<kB> SC, ui
             list_global<false>(G, cv, self, cut2, n, kFinalK + 1u, kHeavyScan, [&](uint32_t j) { return f3of(Pf[j]); },
                                emit);
         }
-        if (n > kFinalK) n = ovf_entry(b, q);
+        if (n > kFinalK) n = ovf_mark(b, q);
         b.fcnt[q] = n;
         tl.listovf += n > kFinalK ? 1u : 0u;
     };
@@ -1853,7 +1858,7 @@ __global__ __launch_bounds__(1024) void k_final_heavy(Bufs b, Scenes<kB> SC) {
         const uint32_t n = any_big ? heavy_list<true>(G, cv, b.Pf, self, cut2, b.flist, b.cap, q)
                                    : heavy_list<false>(G, cv, b.Pf, self, cut2, b.flist, b.cap, q);
         if (lane_id() == 0u) {
-            b.fcnt[q] = n > kFinalK ? ovf_entry(b, q) : n;
+            b.fcnt[q] = n > kFinalK ? ovf_mark(b, q) : n;
             tl.listovf += n > kFinalK ? 1u : 0u;
         }
     }
@@ -2002,9 +2007,10 @@ __device__ __forceinline__ F3 pass_cell(const Bufs& b, const Grid& G, const Pass
 }
 
 // The pass's sum for final-order position q with stored count n: the list
-// (n <= kFinalK), the direct 27-cell sum (n = kFinalK + 1), or the 27
-// per-cell partials that k_ovf_part left for overflow entry n - (kFinalK + 2),
-// added in the same z,y,x order as the direct sum adds its cells.
+// (n <= kFinalK), or, for n = kFinalK + 1, the 27 per-cell partials that
+// k_ovf_part left in its block's overflow slot, added in the same z,y,x order
+// as the direct 27-cell sum adds its cells (which it does when the block has
+// no slot).
 template <class Pass>
 __device__ __forceinline__ F3 pass_sum(const Bufs& b, const Grid& G, const Pass& ps, uint32_t q, uint32_t n) {
     auto elem = [&](uint32_t j) { return ps.elem(j); };
@@ -2016,36 +2022,39 @@ __device__ __forceinline__ F3 pass_sum(const Bufs& b, const Grid& G, const Pass&
         }
         return sum_list(b.flist, b.cap, q, n, elem, term);
     }
-    if (n == kFinalK + 1u)
+    const uint32_t k = b.oslot[q >> 6];
+    if (k >= b.ovf_max)
         return big_cells(b) ? sum27_global<true>(G, cell_view(b), ps.self, ps.cut2, elem, term)
                             : sum27_global<false>(G, cell_view(b), ps.self, ps.cut2, elem, term);
-    const size_t k = n - (kFinalK + 2u);
+    const float4* __restrict__ pk = b.part + (size_t)k * 27u * 64u + (q & 63u);
     F3 sum = mk3(0.f, 0.f, 0.f);
-    for (uint32_t c = 0; c < 27u; ++c) acc(sum, f3of(b.part[(size_t)c * b.ovf_max + k]));
+    for (uint32_t c = 0; c < 27u; ++c) acc(sum, f3of(pk[c * 64u]));
     return sum;
 }
 
 
-// Per-cell partials of pass kPass for the listed overflow positions.  A wave
-// task is (group of 64 list entries, cell c): each lane sums cell c of its
-// particle's 27 in slot order (sum_cell, as the direct sum does) into
-// part[c * ovf_max + k].  The lanes of a group are neighbours in final-cell
-// order, so they mostly scan the same cell and load the same records.
+// Per-cell partials of pass kPass for the overflow slots.  A wave task is
+// (slot k, cell c): lane l sums cell c of the 27 around final-order position
+// 64 * ovf[k] + l, when that position overflowed, in slot order (sum_cell, as
+// the direct sum does) into part[(k * 27 + c) * 64 + l].  The lanes are
+// neighbours in final-cell order, so they mostly scan the same cell and load
+// the same records, and the pass reads each row of 64 whole.
 template <bool kB, int kPass>
 __global__ __launch_bounds__(256) void k_ovf_part(Bufs b, Scenes<kB> SC) {
-    const uint32_t nov = min(b.ctr[CTR_OVF], b.ovf_max);
-    const uint32_t ntask = (nov + 63u) / 64u * 27u;
+    const uint32_t ns = min(b.ctr[CTR_OVF], b.ovf_max);
+    const uint32_t ntask = ns * 27u;
+    const uint32_t nl = b.offset[b.CT];
     const uint32_t lane = lane_id();
     const uint32_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t t = w0; t < ntask; t += nw) {
-        const uint32_t grp = t / 27u, c = t - grp * 27u;
-        const uint32_t k = grp * 64u + lane;
-        const bool active = k < nov;
-        const uint32_t g = active ? b.cperm[b.ovf[k]] : 0u;
+        const uint32_t k = t / 27u, c = t - k * 27u;
+        const uint32_t q = b.ovf[k] * 64u + lane;
+        const bool active = q < nl && b.fcnt[q] == kFinalK + 1u;
+        const uint32_t g = active ? b.cperm[q] : 0u;
         by_scene<kB>((kB && active) ? b.sid[g] : 0u, active, [&](uint32_t sc) {
             const PassTerms<kPass> ps(b, SC[sc].P, g);
             const F3 cs = pass_cell(b, SC[sc].G, ps, c);
-            b.part[(size_t)c * b.ovf_max + k] = make_float4(cs.x, cs.y, cs.z, 0.f);
+            b.part[((size_t)k * 27u + c) * 64u + lane] = make_float4(cs.x, cs.y, cs.z, 0.f);
         });
     }
 }
@@ -2408,7 +2417,7 @@ hipError_t step_seq(const Bufs& b, const Scenes<kB>& SC, const Tuning& T, float*
         hipLaunchKernelGGL(k_scan_cells, dim3(nTr + 1), dim3(kScanBlock), 0, s, b.ccount, b.coffset, b.ctsum, b.ctflag,
                            b.ctprefix, nTr, lgTr, CR, lgnc, (uint32_t*)nullptr);
         hipLaunchKernelGGL(k_cur_scatter, dim3(nb), dim3(256), 0, s, b.offset, CT, b.ckey, b.crank, b.coffset,
-                           b.cperm, uw, lguw, lgnc, b.cunits, b.ctr);
+                           b.cperm, uw, lguw, lgnc, b.cunits, b.ctr, b.oslot);
     }
     {
         Mark m(tm, s, K_FINAL_LIST);

@@ -267,7 +267,7 @@ typedef struct pbfx_stats {
     uint32_t max_tile;         /* largest neighbourhood tile seen */
     uint32_t steps;            /* steps run by this context */
     uint32_t final_units;      /* work units of the final passes (final-cell runs) */
-    uint32_t list_overflow;    /* particles with more than 64 in-range final pairs (summed directly) */
+    uint32_t list_overflow;    /* particles with more than 64 in-range final pairs (summed cell by cell) */
     uint32_t scan_tiles;       /* occupied 4096-cell tiles of the cell scan (the others are not read) */
     uint32_t heavy_builds;     /* search and final-pass lists of more than 2048 candidates, a wave each */
 } pbfx_stats;

@@ -269,7 +269,8 @@ typedef struct pbfx_stats {
     uint32_t final_units;      /* work units of the final passes (final-cell runs) */
     uint32_t list_overflow;    /* particles with more than 64 in-range final pairs (summed cell by cell) */
     uint32_t scan_tiles;       /* occupied 4096-cell tiles of the cell scan (the others are not read) */
-    uint32_t heavy_builds;     /* search and final-pass lists of more than 2048 candidates, a wave each */
+    uint32_t heavy_builds;     /* final-pass lists of more than 2048 candidates, built by a wave each */
+    uint32_t heavy_search;     /* search lists of more than 2048 candidates, built by a wave each */
 } pbfx_stats;
 
 /* Statistics of the last step (synchronises `stream`). */

@@ -881,7 +881,8 @@ pbfx_status pbfx_get_stats(pbfx_ctx* c, void* stream, pbfx_stats* out) {
     out->final_units = h[CTR_CUNITS];
     out->list_overflow = h[CTR_LISTOVF];
     out->scan_tiles = h[CTR_SCANTILES];
-    out->heavy_builds = h[CTR_HEAVY] + h[CTR_NSHEAVY];
+    out->heavy_builds = h[CTR_HEAVY];
+    out->heavy_search = h[CTR_NSHEAVY];
     return PBFX_OK;
 }
 

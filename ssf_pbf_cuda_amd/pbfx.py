@@ -101,7 +101,7 @@ class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "num_in", "num_live", "truncated", "sweep_cap_hits", "max_sweeps_used", "xsph_nan", "units", "slow_units",
         "slow_particles", "tile_overflow", "max_tile", "steps", "final_units", "list_overflow", "scan_tiles",
-        "heavy_builds")]
+        "heavy_builds", "heavy_search")]
 
 
 _lib = None

@@ -68,17 +68,22 @@ def test_aliased_piles_bit_exact(tile_cap):
 
 
 def test_heavy_final_builds_bit_exact():
-    """Particles beside the piles have final-list builds of thousands of
-    candidates from global memory; those are handed to a wave each
-    (k_final_heavy).  Their lists, counts and overflow entries must give the
-    oracle's state bit for bit."""
+    """Particles beside the piles have search and final-list builds of
+    thousands of candidates from global memory (every unit filtered from
+    global memory here); those are handed to a wave each (k_ns_heavy,
+    k_final_heavy).  Their lists must equal the oracle's, and their counts
+    and overflow slots give the oracle's state bit for bit."""
     sc = aliased_scene(far=3000, near=2000, point=500, seed=15)
     print(f"heavy builds: {sc.n} particles", flush=True)
-    got = gpu_run(sc)
+    got = gpu_run(sc, with_ns=True, tile_cap=0, unit_width=16)
     st = got["stats"]
     print(f"gpu stats: {st}", flush=True)
-    ref = oracle_run(sc)
-    assert st["heavy_builds"] > 0
+    ref = oracle_run(sc, want_ns=True)
+    assert st["heavy_builds"] > 0 and st["heavy_search"] > 0
+    nl = got["num"]
+    m = sc.params.max_pair
+    np.testing.assert_array_equal(got["neighbor_count"], ref["neighbor_count"][:nl])
+    np.testing.assert_array_equal(got["neighbors"], ref["neighbors"].reshape(-1, m)[:nl])
     assert got["num"] == ref["num"]
     bx, bv = bit_mismatches(got["x"], ref["x"]), bit_mismatches(got["v"], ref["v"])
     print(f"heavy builds step: bitdiff_x={bx} bitdiff_v={bv}", flush=True)

@@ -43,4 +43,4 @@ def test_params_default_and_status_strings():
 def test_header_struct_layout_matches_binding():
     """pbfx_params / pbfx_stats sizes in the binding match the header's field lists."""
     assert ctypes.sizeof(pbfx.Params) == 4 * 19 + 4 * 2
-    assert ctypes.sizeof(pbfx.Stats) == 4 * 16
+    assert ctypes.sizeof(pbfx.Stats) == 4 * 17
